@@ -1,0 +1,209 @@
+/*
+ * msc_slab.h -- C ABI of the MI355X (gfx950) slab Foldy-Lax library
+ * (libmsc_slab.so).
+ *
+ * The library solves (I - G0 T0) psi = psi0 for an M x M x N_z slab of cubic
+ * voxels with 9 complex unknowns each (3 displacement + 6 Voigt strain,
+ * order (z, x, y) / (zz, xx, yy, xy, zy, zx)).  All arrays are caller-owned
+ * device pointers (interleaved complex, C-contiguous (N_z, M, M, 9) unless
+ * stated) and all work is ordered on the caller's HIP stream (`stream` is a
+ * hipStream_t, NULL = the default stream).  Every entry point returns an
+ * msc_status; msc_last_error() describes the last failure of the calling
+ * thread.  One host thread per plan.
+ *
+ * The reference exposes a Python API only; each function below cites the
+ * reference interface (path:line in tmnestor/MultipleScatteringCalculations)
+ * it replaces.
+ */
+#ifndef MSC_SLAB_H
+#define MSC_SLAB_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum msc_status {
+  MSC_OK = 0,
+  MSC_ERR_INVALID = 1,     /* bad argument          -> ValueError   */
+  MSC_ERR_UNSUPPORTED = 2, /* valid but not handled -> ValueError   */
+  MSC_ERR_HIP = 3,         /* HIP runtime failure   -> RuntimeError */
+  MSC_ERR_NOT_READY = 4    /* plan not initialised  -> RuntimeError */
+} msc_status;
+
+typedef enum msc_precision {
+  MSC_C128 = 0, /* complex128 throughout                                   */
+  MSC_C64 = 1   /* complex64 operator (FFTs, block multiply, basis)        */
+} msc_precision;
+
+typedef enum msc_solver_mode {
+  MSC_SOLVE_FP64 = 0,  /* plain GMRES(m) in complex128                      */
+  MSC_SOLVE_MIXED = 1  /* complex64 GMRES cycles inside FP64 iterative      */
+                       /* refinement, FP64 true residual judges each cycle  */
+} msc_solver_mode;
+
+typedef enum msc_x0 {
+  MSC_X0_ZERO = 0, /* x0 = 0                                                */
+  MSC_X0_BORN = 1, /* x0 = b (the reference default, "born")                */
+  MSC_X0_GIVEN = 2 /* x holds the initial guess on entry                    */
+} msc_x0;
+
+typedef struct msc_plan msc_plan;
+typedef struct msc_arnoldi msc_arnoldi;
+
+typedef struct msc_gmres_opts {
+  int restart;       /* Krylov dimension per cycle (>= 1)                    */
+  int max_iter;      /* cap on Arnoldi steps over all cycles                 */
+  double tol;        /* relative FP64 residual ||b - A x|| / ||b||          */
+  int x0;            /* msc_x0                                               */
+  int mode;          /* msc_solver_mode                                      */
+  double inner_tol;  /* MIXED: relative reduction asked of each c64 cycle    */
+  int orth;          /* 0 adaptive CGS (reorthogonalise when needed),       */
+                     /* 1 CGS once, 2 CGS twice                              */
+} msc_gmres_opts;
+
+typedef struct msc_gmres_stats {
+  int iterations;      /* Arnoldi steps (all cycles, all precisions)         */
+  int cycles;          /* restart / refinement cycles                        */
+  int matvecs_c128;    /* complex128 operator applications                   */
+  int matvecs_c64;     /* complex64 operator applications                    */
+  int converged;       /* 1 if rel_residual <= tol                           */
+  int fell_back;       /* MIXED: 1 if it switched to FP64 cycles             */
+  double rel_residual; /* final true FP64 ||b - A x|| / ||b||               */
+  double bnorm;        /* ||b||                                              */
+} msc_gmres_stats;
+
+/* Last error message of the calling thread ("" if none). */
+const char* msc_last_error(void);
+
+/* Library version string. */
+const char* msc_version(void);
+
+/*
+ * Plan for an M x M x N_z slab of cubes of half-width a (side d = 2a).
+ * pz, pxy: padded FFT lengths (powers of two); 0 picks the smallest power of
+ * two >= 2N_z-1 (z) and >= 2M-1 (x, y), or pxy = M when periodic.  Periodic
+ * mode (circular lateral convolution) needs M to be a power of two.
+ * Replaces the geometry handling of _build_slab_kernels_gpu /
+ * _slab_matvec_gpu (cubic_scattering/slab_scattering_gpu.py:34-155) and
+ * SlabGeometry (cubic_scattering/slab_scattering.py:40-99).
+ */
+int msc_plan_create(msc_plan** plan, int M, int N_z, double a, int periodic, int pz, int pxy);
+int msc_plan_destroy(msc_plan* plan);
+/* out[0..5] = M, N_z, Pz, Pxy, NUV (table columns), NC (table depth). */
+int msc_plan_info(const msc_plan* plan, long long* out6);
+
+/*
+ * Frequency constants of the Rayleigh cube T-matrix on the host:
+ * out8 = (Gamma0, A^c, B^c, C^c) as (re, im) pairs.
+ * Replaces _compute_Gamma0_analytical and _compute_ABC_polynomial
+ * (cubic_scattering/effective_contrasts.py:112-152, 278-357).
+ */
+int msc_cube_coefficients(double omega, double a, double alpha, double beta, double rho,
+                          double* out8);
+
+/*
+ * Spectral G0 table on the device (FP64 evaluation, stored complex128 and
+ * complex64).  Replaces _build_slab_kernels (slab_scattering.py:267-351) and
+ * _build_slab_kernels_gpu (slab_scattering_gpu.py:34-92).  volume_averaged
+ * != 0 is not supported by this build (MSC_ERR_UNSUPPORTED).
+ */
+int msc_build_g0(msc_plan* plan, double omega, double alpha, double beta, double rho,
+                 int volume_averaged, int n_orders, void* stream);
+
+/*
+ * Per-voxel T0 from contrast arrays (device float64, (N_z, M, M) each) and
+ * the frequency constants of msc_cube_coefficients.  Replaces
+ * compute_slab_tmatrices (slab_scattering.py:169-205).
+ */
+int msc_set_tmatrix(msc_plan* plan, const double* dlambda, const double* dmu, const double* drho,
+                    double omega, const double* coef8, void* stream);
+
+/*
+ * Per-voxel T0 given directly as 4 complex128 parameters (t_u, t_d, t_o, t_s)
+ * per voxel: T0 = blockdiag(t_u I3, [[t_d, t_o, t_o], [t_o, t_d, t_o],
+ * [t_o, t_o, t_d]], t_s I3).  get copies them out.
+ */
+int msc_set_tparams(msc_plan* plan, const void* tparams_c128, void* stream);
+int msc_get_tparams(const msc_plan* plan, void* out_c128, void* stream);
+
+/*
+ * Incident plane wave psi0 = inc9 * exp(i k khat . x_c) (complex128 out).
+ * inc18 = 9 complex (re, im) on the host: polarisation and Voigt strain.
+ * Replaces _build_slab_incident_field (slab_scattering.py:426-478).
+ */
+int msc_incident_field(msc_plan* plan, double k_mag, const double* khat3, const double* inc18,
+                       void* psi0_c128, void* stream);
+
+/*
+ * y = (I - G0 T0) x in the given precision (x, y complex128 or complex64).
+ * Replaces _slab_matvec_gpu (slab_scattering_gpu.py:95-155) and
+ * _slab_matvec (slab_scattering.py:359-418).
+ */
+int msc_matvec(msc_plan* plan, const void* x, void* y, int precision, void* stream);
+
+/* r = b - (I - G0 T0) x, fused (complex128 or complex64). */
+int msc_residual(msc_plan* plan, const void* b, const void* x, void* r, int precision,
+                 void* stream);
+
+/*
+ * GMRES solve of (I - G0 T0) x = b (b, x complex128).  Replaces torch_gmres
+ * (cubic_scattering/torch_gmres.py:83-205) as called by
+ * compute_slab_scattering_gpu (slab_scattering_gpu.py:229-241) and the SciPy
+ * gmres call of compute_slab_scattering (slab_scattering.py:532-552).
+ * Synchronises `stream` once per cycle.  Non-convergence is not an error
+ * (stats->converged = 0), as the reference only warns.
+ */
+int msc_gmres(msc_plan* plan, const void* b, void* x, const msc_gmres_opts* opts,
+              msc_gmres_stats* stats, void* stream);
+
+/*
+ * Sums of the sources tau = T0 psi (psi complex128): per layer
+ * layer_sum (N_z, 9) and per-layer phased sums phased (N_z, 2, 9) of
+ * exp(-i k r_hat . x_v) tau_v for k = kP, kS.  Device outputs, complex128.
+ * Back end of slab_rpp_periodic (slab_scattering.py:672-735) and
+ * slab_reflected_field (slab_scattering.py:589-669).
+ */
+int msc_source_sums(msc_plan* plan, const void* psi, double kP, double kS, const double* rhat3,
+                    void* layer_sum, void* phased, void* stream);
+
+/*
+ * Batched FFT along one axis of data viewed as [outer][n][inner] (complex,
+ * in-place allowed), forward unscaled / inverse unscaled.
+ */
+int msc_fft_axis(const void* in, void* out, int outer, int n, long long inner, int inverse,
+                 int precision, void* stream);
+
+/*
+ * Arnoldi building blocks over a caller-owned basis V ((m+1) rows of ldv
+ * complex, precision of the context) for a caller-supplied operator -- the
+ * engine of the generic torch_gmres (torch_gmres.py:83-205).
+ *   start : V[0] = r / ||r||, g = (||r||, 0, ...); the cycle stops when
+ *           |g_{j+1}| <= tol_abs.
+ *   step  : the caller has written A V[j] into V[j+1]; orthogonalise,
+ *           update the Givens QR and normalise V[j+1].
+ *   state : out4 = (steps j, done flag, |g_{j+1}|, ||r||) after a sync.
+ *   solution: x += sum_{i<j} y_i V[i] (x in the context precision).
+ */
+int msc_arnoldi_create(msc_arnoldi** ctx, long long n, int m, int precision);
+int msc_arnoldi_destroy(msc_arnoldi* ctx);
+int msc_arnoldi_start(msc_arnoldi* ctx, void* V, long long ldv, const void* r, double tol_abs,
+                      void* stream);
+int msc_arnoldi_step(msc_arnoldi* ctx, void* V, long long ldv, int j, int orth, void* stream);
+int msc_arnoldi_state(msc_arnoldi* ctx, double* out4, void* stream);
+int msc_arnoldi_solution(msc_arnoldi* ctx, const void* V, long long ldv, void* x, void* stream);
+
+/*
+ * Kernel timing with HIP events recorded around every launch of the
+ * block-multiply pass (pass 3) on the launch stream: enable resets the
+ * counters; read returns (launches, total milliseconds).
+ */
+int msc_timing_enable(msc_plan* plan, int enable);
+int msc_timing_read(msc_plan* plan, double* out2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSC_SLAB_H */

@@ -1,0 +1,40 @@
+"""MI355X-native FFT-accelerated slab Foldy-Lax solver.
+
+Drop-in for the slab path of tmnestor/MultipleScatteringCalculations
+(cubic_scattering/slab_scattering.py, slab_scattering_gpu.py, torch_gmres.py,
+solver_config.py): same public names and signatures, with every numerical step
+in the HIP library libmsc_slab.so (include/msc_slab.h).
+"""
+
+from .effective_contrasts import (
+    CubeTMatrixResult,
+    MaterialContrast,
+    ReferenceMedium,
+    compute_cube_tmatrix,
+    effective_stiffness_voigt,
+)
+from .slab_scattering import (
+    SlabGeometry,
+    SlabMaterial,
+    SlabPlan,
+    SlabResult,
+    compute_slab_scattering,
+    compute_slab_tmatrices,
+    random_iid_slab_material,
+    random_slab_material,
+    slab_reflected_field,
+    slab_rpp_periodic,
+    uniform_slab_material,
+)
+from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
+from .solver_config import load_config, run_from_config
+from .torch_gmres import get_device, select_dtype, to_numpy, to_torch, torch_gmres
+
+__all__ = [
+    "CubeTMatrixResult", "MaterialContrast", "ReferenceMedium", "compute_cube_tmatrix",
+    "effective_stiffness_voigt", "SlabGeometry", "SlabMaterial", "SlabPlan", "SlabResult",
+    "compute_slab_scattering", "compute_slab_scattering_gpu", "compute_slab_tmatrices",
+    "random_iid_slab_material", "random_slab_material", "slab_reflected_field",
+    "slab_rpp_periodic", "uniform_slab_material", "SlabOperator", "load_config",
+    "run_from_config", "get_device", "select_dtype", "to_numpy", "to_torch", "torch_gmres",
+]

@@ -1,0 +1,126 @@
+"""ctypes binding of libmsc_slab.so (C ABI declared in include/msc_slab.h).
+
+The library is loaded after ``import torch`` so that it binds to the HIP runtime
+PyTorch already loaded.  There is no CPU fallback: if the library is missing or
+no HIP device is present, every solver entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_longlong, c_void_p
+from pathlib import Path
+
+import torch  # noqa: F401  (loads the HIP runtime first)
+
+LIB_PATH = Path(__file__).resolve().parent / "libmsc_slab.so"
+
+MSC_OK = 0
+MSC_ERR_INVALID = 1
+MSC_ERR_UNSUPPORTED = 2
+MSC_ERR_HIP = 3
+MSC_ERR_NOT_READY = 4
+
+MSC_C128 = 0
+MSC_C64 = 1
+
+# Every symbol the header declares, with (restype, argtypes).
+SIGNATURES = {
+    "msc_last_error": (ctypes.c_char_p, []),
+    "msc_version": (ctypes.c_char_p, []),
+    "msc_plan_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_int, c_int, c_int]),
+    "msc_plan_destroy": (c_int, [c_void_p]),
+    "msc_plan_info": (c_int, [c_void_p, POINTER(c_longlong)]),
+    "msc_cube_coefficients": (c_int, [c_double, c_double, c_double, c_double, c_double,
+                                      POINTER(c_double)]),
+    "msc_build_g0": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_int, c_int,
+                             c_void_p]),
+    "msc_set_tmatrix": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                POINTER(c_double), c_void_p]),
+    "msc_set_tparams": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "msc_get_tparams": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "msc_incident_field": (c_int, [c_void_p, c_double, POINTER(c_double), POINTER(c_double),
+                                   c_void_p, c_void_p]),
+    "msc_matvec": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "msc_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "msc_gmres": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "msc_source_sums": (c_int, [c_void_p, c_void_p, c_double, c_double, POINTER(c_double),
+                                c_void_p, c_void_p, c_void_p]),
+    "msc_fft_axis": (c_int, [c_void_p, c_void_p, c_int, c_int, c_longlong, c_int, c_int,
+                             c_void_p]),
+    "msc_arnoldi_create": (c_int, [POINTER(c_void_p), c_longlong, c_int, c_int]),
+    "msc_arnoldi_destroy": (c_int, [c_void_p]),
+    "msc_arnoldi_start": (c_int, [c_void_p, c_void_p, c_longlong, c_void_p, c_double, c_void_p]),
+    "msc_arnoldi_step": (c_int, [c_void_p, c_void_p, c_longlong, c_int, c_int, c_void_p]),
+    "msc_arnoldi_state": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
+    "msc_arnoldi_solution": (c_int, [c_void_p, c_void_p, c_longlong, c_void_p, c_void_p]),
+    "msc_timing_enable": (c_int, [c_void_p, c_int]),
+    "msc_timing_read": (c_int, [c_void_p, POINTER(c_double)]),
+}
+
+
+class GmresOpts(ctypes.Structure):
+    _fields_ = [("restart", c_int), ("max_iter", c_int), ("tol", c_double), ("x0", c_int),
+                ("mode", c_int), ("inner_tol", c_double), ("orth", c_int)]
+
+
+class GmresStats(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("cycles", c_int), ("matvecs_c128", c_int),
+                ("matvecs_c64", c_int), ("converged", c_int), ("fell_back", c_int),
+                ("rel_residual", c_double), ("bnorm", c_double)]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load (once) and return the library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH.name} is not built; run `python -m multiplescatteringcalculations_amd.build` "
+            "(this package has no CPU fallback)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    """Map a status code to the reference's exception types."""
+    if rc == MSC_OK:
+        return
+    msg = load().msc_last_error().decode(errors="replace")
+    if rc in (MSC_ERR_INVALID, MSC_ERR_UNSUPPORTED):
+        raise ValueError(msg)
+    raise RuntimeError(f"libmsc_slab error {rc}: {msg}")
+
+
+def require_device() -> torch.device:
+    """The HIP device every solve runs on; raises when there is none."""
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "no HIP device available: the slab solver runs on AMD GPUs only (no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr(device: torch.device | None = None) -> c_void_p:
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t: torch.Tensor) -> c_void_p:
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return c_void_p(t.data_ptr())
+
+
+def dbl_array(vals) -> ctypes.Array:
+    arr = (c_double * len(vals))()
+    for i, v in enumerate(vals):
+        arr[i] = float(v)
+    return arr

@@ -1,0 +1,650 @@
+// C ABI of libmsc_slab.so (declared in include/msc_slab.h): plan lifecycle,
+// host-side frequency constants, solver drivers (FP64 GMRES(m) and GMRES-IR).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/msc_slab.h"
+#include "msc_gmres.hpp"
+#include "msc_plan.hpp"
+#include "msc_setup.hpp"
+
+namespace msc {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* get_error() { return g_err.c_str(); }
+
+// Twiddles tw[m] = exp(-2 pi i m / kTwMax), one copy per device.
+struct Twiddles {
+  double2* t128 = nullptr;
+  float2* t64 = nullptr;
+};
+static std::vector<Twiddles> g_tw;
+
+static int twiddles(int dev, Twiddles** out) {
+  if ((int)g_tw.size() <= dev) g_tw.resize(dev + 1);
+  Twiddles& t = g_tw[dev];
+  if (t.t128 == nullptr) {
+    constexpr int N = 1024;  // kTwMax
+    std::vector<double2> h128(N);
+    std::vector<float2> h64(N);
+    for (int m = 0; m < N; ++m) {
+      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * m / N;
+      h128[m] = make_double2((double)cosl(ang), (double)sinl(ang));
+      h64[m] = make_float2((float)cosl(ang), (float)sinl(ang));
+    }
+    MSC_HIP_CHECK(hipMalloc(&t.t128, N * sizeof(double2)));
+    MSC_HIP_CHECK(hipMalloc(&t.t64, N * sizeof(float2)));
+    MSC_HIP_CHECK(hipMemcpy(t.t128, h128.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+    MSC_HIP_CHECK(hipMemcpy(t.t64, h64.data(), N * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  *out = &t;
+  return kOk;
+}
+
+static int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// ── frequency constants of the Rayleigh cube T-matrix (host, FP64) ──────────
+// Restates effective_contrasts.py:109-357: static Kelvin/Eshelby parts plus
+// the smooth radiation parts from the Taylor coefficients phi_n, psi_n and the
+// exact cube moments S0, S1, S2, S11 (N_TAYLOR = 8 terms).
+struct cd {
+  double re, im;
+};
+static cd cmul_h(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+static cd cadd_h(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
+static cd cscl_h(cd a, double s) { return {a.re * s, a.im * s}; }
+static cd cpow_i(double k, int m) {  // (i k)^m
+  double mag = std::pow(k, m);
+  switch (m & 3) {
+    case 0: return {mag, 0.0};
+    case 1: return {0.0, mag};
+    case 2: return {-mag, 0.0};
+    default: return {0.0, -mag};
+  }
+}
+static double fact(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return f;
+}
+
+static void cube_coefficients(double omega, double a, double alpha, double beta, double rho,
+                              cd out[4]) {
+  constexpr int NT = 8;
+  const double pi = M_PI;
+  const double den = 8.0 * pi * rho * alpha * alpha * beta * beta;
+  const double a0 = (alpha * alpha + beta * beta) / den;
+  const double b0 = (alpha * alpha - beta * beta) / den;
+  // g0 = int_{[-1,1]^3} 1/|x| (Pell form of the closed expression)
+  const double g0 = (4.0 / 3.0) * std::log(70226.0 + 40545.0 * std::sqrt(3.0)) - 2.0 * pi;
+  cd phi[NT], psi[NT];
+  for (int n = 0; n < NT; ++n) {
+    phi[n] = cscl_h(cpow_i(omega / beta, 2 * n + 1), 1.0 / (fact(2 * n + 1) * 4.0 * pi * rho * beta * beta));
+    cd pa = cscl_h(cpow_i(omega / alpha, 2 * n + 3), 1.0 / (alpha * alpha));
+    cd pb = cscl_h(cpow_i(omega / beta, 2 * n + 3), -1.0 / (beta * beta));
+    psi[n] = cscl_h(cadd_h(pa, pb), 1.0 / (fact(2 * n + 3) * 4.0 * pi * rho));
+  }
+  // moments of [-a, a]^3: mu_k = 2 a^(2k+1)/(2k+1), trinomial expansion
+  double mu[NT + 3];
+  for (int k = 0; k < NT + 3; ++k) mu[k] = 2.0 * std::pow(a, 2 * k + 1) / (2 * k + 1);
+  double S0[NT] = {0}, S1[NT] = {0}, S2[NT] = {0}, S11[NT] = {0};
+  for (int m = 0; m < NT; ++m) {
+    for (int p = 0; p <= m; ++p) {
+      for (int q = 0; q <= m - p; ++q) {
+        const int r = m - p - q;
+        const double c = fact(m) / (fact(p) * fact(q) * fact(r));
+        S0[m] += c * mu[p] * mu[q] * mu[r];
+        S1[m] += c * mu[p + 1] * mu[q] * mu[r];
+        S2[m] += c * mu[p + 2] * mu[q] * mu[r];
+        S11[m] += c * mu[p + 1] * mu[q + 1] * mu[r];
+      }
+    }
+  }
+  cd G0 = {a * a * (a0 + b0 / 3.0) * g0, 0.0};
+  for (int n = 0; n < NT; ++n) {
+    G0 = cadd_h(G0, cscl_h(phi[n], S0[n]));
+    G0 = cadd_h(G0, cscl_h(psi[n], S1[n]));
+  }
+  const double j1 = 2.0 * pi / 3.0;
+  const double j2 = -2.0 * (std::sqrt(3.0) - pi) / 9.0;
+  const double k1 = 2.0 * (2.0 * std::sqrt(3.0) + pi) / 9.0;
+  cd A = {2.0 * (-a0 * j1 - 3.0 * b0 * j2), 0.0};
+  cd B = {2.0 * b0 * (j1 - 3.0 * j2), 0.0};
+  cd C = {6.0 * b0 * (3.0 * j2 - k1), 0.0};
+  for (int n = 0; n < NT; ++n) {
+    B = cadd_h(B, cscl_h(psi[n], S0[n]));
+    if (n >= 1) {
+      A = cadd_h(A, cscl_h(phi[n], 2.0 * n * S0[n - 1]));
+      A = cadd_h(A, cscl_h(psi[n], 2.0 * n * S1[n - 1]));
+      B = cadd_h(B, cscl_h(psi[n], 4.0 * n * S1[n - 1]));
+    }
+    if (n >= 2) {
+      const double f = 4.0 * n * (n - 1);
+      A = cadd_h(A, cscl_h(phi[n], f * S1[n - 2]));
+      A = cadd_h(A, cscl_h(psi[n], f * S11[n - 2]));
+      B = cadd_h(B, cscl_h(psi[n], f * S11[n - 2]));
+      C = cadd_h(C, cscl_h(psi[n], f * (S2[n - 2] - 3.0 * S11[n - 2])));
+    }
+  }
+  out[0] = G0;
+  out[1] = A;
+  out[2] = B;
+  out[3] = C;
+}
+
+}  // namespace msc
+
+using namespace msc;
+
+struct msc_plan {
+  Plan p;
+  Krylov k128;
+  Krylov k64;
+  double2* r128 = nullptr;  // residual buffer
+};
+
+struct msc_arnoldi {
+  Krylov k;
+  int precision = 0;
+};
+
+namespace {
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int alloc_plan_buffers(Plan& p) {
+  const size_t b1 = (size_t)p.Nz * p.M * p.Pxy * 9;
+  const size_t b2 = (size_t)p.Nz * p.Pxy * p.Pxy * 9;
+  const size_t tab = (size_t)p.NUV * kNE * p.NC;
+  MSC_HIP_CHECK(hipMalloc(&p.buf1, b1 * sizeof(double2)));
+  MSC_HIP_CHECK(hipMalloc(&p.buf2, b2 * sizeof(double2)));
+  MSC_HIP_CHECK(hipMalloc(&p.table128, tab * sizeof(double2)));
+  MSC_HIP_CHECK(hipMalloc(&p.table64, tab * sizeof(float2)));
+  MSC_HIP_CHECK(hipMalloc(&p.tp128, p.nvox * 4 * sizeof(double2)));
+  MSC_HIP_CHECK(hipMalloc(&p.tp64, p.nvox * 4 * sizeof(float2)));
+  return kOk;
+}
+
+void free_plan_buffers(Plan& p) {
+  if (p.buf1) (void)hipFree(p.buf1);
+  if (p.buf2) (void)hipFree(p.buf2);
+  if (p.table128) (void)hipFree(p.table128);
+  if (p.table64) (void)hipFree(p.table64);
+  if (p.tp128) (void)hipFree(p.tp128);
+  if (p.tp64) (void)hipFree(p.tp64);
+  p.buf1 = p.buf2 = nullptr;
+  p.table128 = nullptr;
+  p.table64 = nullptr;
+  p.tp128 = nullptr;
+  p.tp64 = nullptr;
+}
+
+int check_opts(const msc_gmres_opts* o) {
+  MSC_REQUIRE(o != nullptr, "opts must not be NULL");
+  MSC_REQUIRE(o->restart >= 1, "restart must be >= 1");
+  MSC_REQUIRE(o->max_iter >= 0, "max_iter must be >= 0");
+  MSC_REQUIRE(o->tol > 0.0 && std::isfinite(o->tol), "tol must be positive and finite");
+  MSC_REQUIRE(o->x0 >= 0 && o->x0 <= 2, "x0 must be 0 (zero), 1 (born) or 2 (given)");
+  MSC_REQUIRE(o->mode == MSC_SOLVE_FP64 || o->mode == MSC_SOLVE_MIXED, "unknown solver mode");
+  MSC_REQUIRE(o->orth >= 0 && o->orth <= 2, "orth must be 0, 1 or 2");
+  return kOk;
+}
+
+}  // namespace
+
+namespace {
+
+// One cycle's Arnoldi steps; checks the device flag every `poll` steps.
+template <typename R>
+int run_cycle(msc_plan* pl, Krylov& k, int steps, int orth, hipStream_t st, int* taken) {
+  Plan& p = pl->p;
+  constexpr int poll = 6;
+  KState hs{};
+  for (int j = 0; j < steps; ++j) {
+    cx<R>* V = reinterpret_cast<cx<R>*>(k.V);
+    int rc = matvec<R>(p, V + (size_t)j * k.ldv, V + (size_t)(j + 1) * k.ldv, st, &k.st->done);
+    if (rc) return rc;
+    rc = krylov_step<R>(k, j, orth, st);
+    if (rc) return rc;
+    if ((j + 1) % poll == 0 && j + 1 < steps) {
+      rc = krylov_read_state(k, &hs, st);
+      if (rc) return rc;
+      if (hs.done) break;
+    }
+  }
+  int rc = krylov_read_state(k, &hs, st);
+  if (rc) return rc;
+  *taken = hs.j;
+  return kOk;
+}
+
+int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
+          msc_gmres_stats* s, hipStream_t st) {
+  Plan& p = pl->p;
+  const size_t n = p.n;
+  msc_gmres_stats out{};
+  const int m = std::max(1, std::min(o.restart, std::max(o.max_iter, 1)));
+  int rc = pl->k128.ensure(n, m, sizeof(double2));
+  if (rc) return rc;
+  Krylov& k = pl->k128;
+  if (o.mode == MSC_SOLVE_MIXED) {
+    rc = pl->k64.ensure(n, m, sizeof(float2));
+    if (rc) return rc;
+  }
+  if (pl->r128 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r128, n * sizeof(double2)));
+  double2* r = pl->r128;
+
+  // ||b||
+  rc = krylov_norm<double>(k, b, &k.scal[0], st);
+  if (rc) return rc;
+  double bnorm = 0.0;
+  MSC_HIP_CHECK(hipMemcpyAsync(&bnorm, &k.scal[0], sizeof(double), hipMemcpyDeviceToHost, st));
+  MSC_HIP_CHECK(hipStreamSynchronize(st));
+  out.bnorm = bnorm;
+  if (o.x0 == MSC_X0_ZERO) {
+    MSC_HIP_CHECK(hipMemsetAsync(x, 0, n * sizeof(double2), st));
+  } else if (o.x0 == MSC_X0_BORN) {
+    MSC_HIP_CHECK(hipMemcpyAsync(x, b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  }
+  if (bnorm < 1e-30) {  // zero right-hand side (torch_gmres.py:115-116)
+    out.converged = 1;
+    out.rel_residual = 0.0;
+    MSC_HIP_CHECK(hipStreamSynchronize(st));
+    *s = out;
+    return kOk;
+  }
+  bool mixed = o.mode == MSC_SOLVE_MIXED;
+  double prev_rel = INFINITY;
+  const double eta_min = o.inner_tol > 0.0 ? o.inner_tol : 1e-4;
+  for (;;) {
+    // true FP64 residual of the current iterate
+    rc = residual<double>(p, b, x, r, st, nullptr);
+    if (rc) return rc;
+    ++out.matvecs_c128;
+    Krylov& kk = mixed ? pl->k64 : k;
+    rc = krylov_norm<double>(kk, r, &kk.st->rnorm, st);
+    if (rc) return rc;
+    double rnorm = 0.0;
+    MSC_HIP_CHECK(hipMemcpyAsync(&rnorm, &kk.st->rnorm, sizeof(double), hipMemcpyDeviceToHost, st));
+    MSC_HIP_CHECK(hipStreamSynchronize(st));
+    const double rel = rnorm / bnorm;
+    out.rel_residual = rel;
+    if (!(rel > o.tol) || out.iterations >= o.max_iter) break;
+    if (mixed && out.cycles > 0 && rel > 0.9 * prev_rel) {
+      // a complex64 cycle failed to make progress: finish with FP64 cycles
+      mixed = false;
+      out.fell_back = 1;
+      rc = krylov_norm<double>(k, r, &k.st->rnorm, st);
+      if (rc) return rc;
+    }
+    prev_rel = rel;
+    const int steps = std::min(m, o.max_iter - out.iterations);
+    int taken = 0;
+    if (mixed) {
+      // inner tolerance: no tighter than needed to reach tol, no looser than eta
+      const double need = 0.3 * o.tol / rel;
+      const double eta = std::min(std::max(eta_min, need), 0.5);
+      Krylov& k6 = pl->k64;
+      rc = krylov_cycle_init<double, float>(k6, r, eta, true, st);
+      if (rc) return rc;
+      rc = run_cycle<float>(pl, k6, steps, o.orth, st, &taken);
+      if (rc) return rc;
+      rc = krylov_finish<float, double>(k6, x, &k6.st->rnorm, st);
+      if (rc) return rc;
+      out.matvecs_c64 += taken;
+    } else {
+      rc = krylov_cycle_init<double, double>(k, r, o.tol * bnorm, false, st);
+      if (rc) return rc;
+      rc = run_cycle<double>(pl, k, steps, o.orth, st, &taken);
+      if (rc) return rc;
+      rc = krylov_finish<double, double>(k, x, nullptr, st);
+      if (rc) return rc;
+      out.matvecs_c128 += taken;
+    }
+    out.iterations += taken;
+    ++out.cycles;
+    if (taken == 0) {  // breakdown at the first step: nothing more to gain
+      rc = residual<double>(p, b, x, r, st, nullptr);
+      if (rc) return rc;
+      ++out.matvecs_c128;
+      rc = krylov_norm<double>(k, r, &k.scal[0], st);
+      if (rc) return rc;
+      MSC_HIP_CHECK(hipMemcpyAsync(&rnorm, &k.scal[0], sizeof(double), hipMemcpyDeviceToHost, st));
+      MSC_HIP_CHECK(hipStreamSynchronize(st));
+      out.rel_residual = rnorm / bnorm;
+      break;
+    }
+  }
+  out.converged = out.rel_residual <= o.tol ? 1 : 0;
+  *s = out;
+  return kOk;
+}
+
+}  // namespace
+
+// ── exported C ABI ───────────────────────────────────────────────────────────
+
+extern "C" {
+
+const char* msc_last_error(void) { return get_error(); }
+const char* msc_version(void) { return "msc_slab 0.1 (gfx950)"; }
+
+int msc_plan_create(msc_plan** plan, int M, int N_z, double a, int periodic, int pz, int pxy) {
+  MSC_REQUIRE(plan != nullptr, "plan must not be NULL");
+  *plan = nullptr;
+  MSC_REQUIRE(M >= 1, "M must be >= 1");
+  MSC_REQUIRE(N_z >= 1, "N_z must be >= 1");
+  MSC_REQUIRE(a > 0.0 && std::isfinite(a), "a must be > 0");
+  if (pz == 0) pz = std::max(2, next_pow2(2 * N_z - 1));
+  if (pxy == 0) pxy = periodic ? M : std::max(2, next_pow2(2 * M - 1));
+  MSC_REQUIRE(is_pow2(pz) && pz >= 2 * N_z - 1 && pz >= 2 && pz <= 256,
+              "pz must be a power of two in [max(2, 2N_z-1), 256]");
+  if (periodic) {
+    if (!(is_pow2(M) && pxy == M && M >= 2 && M <= 512)) {
+      set_error("periodic mode needs M to be a power of two in [2, 512] and pxy == M");
+      return kUnsupported;
+    }
+  } else {
+    MSC_REQUIRE(is_pow2(pxy) && pxy >= 2 * M - 1 && pxy >= 2 && pxy <= 512,
+                "pxy must be a power of two in [max(2, 2M-1), 512]");
+  }
+  msc_plan* pl = new (std::nothrow) msc_plan();
+  MSC_REQUIRE(pl != nullptr, "out of host memory");
+  Plan& p = pl->p;
+  p.M = M;
+  p.Nz = N_z;
+  p.a = a;
+  p.periodic = periodic != 0;
+  p.Pz = pz;
+  p.Pxy = pxy;
+  p.nvox = (size_t)N_z * M * M;
+  p.n = p.nvox * 9;
+  p.NC = pz / 2 + 1;
+  const int h = pxy / 2;
+  p.NUV = (h + 1) * (h + 2) / 2;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    delete pl;
+    set_error(std::string("hipGetDevice: ") + hipGetErrorString(e));
+    return kHip;
+  }
+  p.device = dev;
+  Twiddles* tw = nullptr;
+  int rc = twiddles(dev, &tw);
+  if (rc == kOk) rc = alloc_plan_buffers(p);
+  if (rc) {
+    free_plan_buffers(p);
+    delete pl;
+    return rc;
+  }
+  p.tw128 = tw->t128;
+  p.tw64 = tw->t64;
+  *plan = pl;
+  return kOk;
+}
+
+int msc_plan_destroy(msc_plan* plan) {
+  if (plan == nullptr) return kOk;
+  (void)hipDeviceSynchronize();
+  free_plan_buffers(plan->p);
+  if (plan->r128) (void)hipFree(plan->r128);
+  for (hipEvent_t e : plan->p.ev_pool) (void)hipEventDestroy(e);
+  delete plan;
+  return kOk;
+}
+
+int msc_plan_info(const msc_plan* plan, long long* out6) {
+  MSC_REQUIRE(plan != nullptr && out6 != nullptr, "plan and out must not be NULL");
+  const Plan& p = plan->p;
+  out6[0] = p.M;
+  out6[1] = p.Nz;
+  out6[2] = p.Pz;
+  out6[3] = p.Pxy;
+  out6[4] = p.NUV;
+  out6[5] = p.NC;
+  return kOk;
+}
+
+int msc_cube_coefficients(double omega, double a, double alpha, double beta, double rho,
+                          double* out8) {
+  MSC_REQUIRE(out8 != nullptr, "out must not be NULL");
+  MSC_REQUIRE(omega > 0 && a > 0 && alpha > 0 && beta > 0 && rho > 0,
+              "omega, a, alpha, beta and rho must be positive");
+  cd c[4];
+  cube_coefficients(omega, a, alpha, beta, rho, c);
+  for (int i = 0; i < 4; ++i) {
+    out8[2 * i] = c[i].re;
+    out8[2 * i + 1] = c[i].im;
+  }
+  return kOk;
+}
+
+int msc_build_g0(msc_plan* plan, double omega, double alpha, double beta, double rho,
+                 int volume_averaged, int n_orders, void* stream) {
+  MSC_REQUIRE(plan != nullptr, "plan must not be NULL");
+  MSC_REQUIRE(omega > 0 && alpha > 0 && beta > 0 && rho > 0,
+              "omega, alpha, beta and rho must be positive");
+  (void)n_orders;
+  if (volume_averaged) {
+    set_error("volume_averaged=True (inter_voxel_propagator_9x9 near-neighbour blocks) is not "
+              "supported by this build");
+    return kUnsupported;
+  }
+  return build_table(plan->p, omega, alpha, beta, rho, S(stream));
+}
+
+int msc_set_tmatrix(msc_plan* plan, const double* dlambda, const double* dmu, const double* drho,
+                    double omega, const double* coef8, void* stream) {
+  MSC_REQUIRE(plan != nullptr && dlambda && dmu && drho && coef8, "NULL argument");
+  return set_tparams_from_contrasts(plan->p, dlambda, dmu, drho, omega, coef8, S(stream));
+}
+
+int msc_set_tparams(msc_plan* plan, const void* tparams, void* stream) {
+  MSC_REQUIRE(plan != nullptr && tparams != nullptr, "NULL argument");
+  return set_tparams_direct(plan->p, reinterpret_cast<const double2*>(tparams), S(stream));
+}
+
+int msc_get_tparams(const msc_plan* plan, void* out, void* stream) {
+  MSC_REQUIRE(plan != nullptr && out != nullptr, "NULL argument");
+  if (!plan->p.have_tparams) {
+    set_error("T0 parameters not set");
+    return kNotReady;
+  }
+  MSC_HIP_CHECK(hipMemcpyAsync(out, plan->p.tp128, plan->p.nvox * 4 * sizeof(double2),
+                               hipMemcpyDeviceToDevice, S(stream)));
+  return kOk;
+}
+
+int msc_incident_field(msc_plan* plan, double k_mag, const double* khat3, const double* inc18,
+                       void* psi0, void* stream) {
+  MSC_REQUIRE(plan != nullptr && khat3 && inc18 && psi0, "NULL argument");
+  return incident_field(plan->p, k_mag, khat3, inc18, reinterpret_cast<double2*>(psi0), S(stream));
+}
+
+int msc_matvec(msc_plan* plan, const void* x, void* y, int precision, void* stream) {
+  MSC_REQUIRE(plan != nullptr && x != nullptr && y != nullptr, "NULL argument");
+  MSC_REQUIRE(x != y, "x and y must not alias");
+  if (precision == MSC_C128)
+    return matvec<double>(plan->p, reinterpret_cast<const double2*>(x),
+                          reinterpret_cast<double2*>(y), S(stream), nullptr);
+  if (precision == MSC_C64)
+    return matvec<float>(plan->p, reinterpret_cast<const float2*>(x),
+                         reinterpret_cast<float2*>(y), S(stream), nullptr);
+  MSC_REQUIRE(false, "precision must be 0 (complex128) or 1 (complex64)");
+}
+
+int msc_residual(msc_plan* plan, const void* b, const void* x, void* r, int precision,
+                 void* stream) {
+  MSC_REQUIRE(plan != nullptr && b && x && r, "NULL argument");
+  MSC_REQUIRE(x != r, "x and r must not alias");
+  if (precision == MSC_C128)
+    return residual<double>(plan->p, reinterpret_cast<const double2*>(b),
+                            reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(r),
+                            S(stream), nullptr);
+  if (precision == MSC_C64)
+    return residual<float>(plan->p, reinterpret_cast<const float2*>(b),
+                           reinterpret_cast<const float2*>(x), reinterpret_cast<float2*>(r),
+                           S(stream), nullptr);
+  MSC_REQUIRE(false, "precision must be 0 (complex128) or 1 (complex64)");
+}
+
+int msc_gmres(msc_plan* plan, const void* b, void* x, const msc_gmres_opts* opts,
+              msc_gmres_stats* stats, void* stream) {
+  MSC_REQUIRE(plan != nullptr && b != nullptr && x != nullptr && stats != nullptr,
+              "NULL argument");
+  MSC_REQUIRE(b != x, "b and x must not alias");
+  int rc = check_opts(opts);
+  if (rc) return rc;
+  if (!plan->p.have_table || !plan->p.have_tparams) {
+    set_error("plan needs msc_build_g0 and msc_set_tmatrix before msc_gmres");
+    return kNotReady;
+  }
+  return solve(plan, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x), *opts,
+               stats, S(stream));
+}
+
+int msc_source_sums(msc_plan* plan, const void* psi, double kP, double kS, const double* rhat3,
+                    void* layer_sum, void* phased, void* stream) {
+  MSC_REQUIRE(plan != nullptr && psi && rhat3 && layer_sum && phased, "NULL argument");
+  if (!plan->p.have_tparams) {
+    set_error("T0 parameters not set");
+    return kNotReady;
+  }
+  return source_sums(plan->p, reinterpret_cast<const double2*>(psi), kP, kS, rhat3,
+                     reinterpret_cast<double2*>(layer_sum), reinterpret_cast<double2*>(phased),
+                     S(stream));
+}
+
+int msc_fft_axis(const void* in, void* out, int outer, int n, long long inner, int inverse,
+                 int precision, void* stream) {
+  MSC_REQUIRE(in != nullptr && out != nullptr, "NULL argument");
+  MSC_REQUIRE(outer >= 1 && inner >= 1 && n >= 1, "sizes must be positive");
+  int dev = 0;
+  MSC_HIP_CHECK(hipGetDevice(&dev));
+  Twiddles* tw = nullptr;
+  int rc = twiddles(dev, &tw);
+  if (rc) return rc;
+  if (precision == MSC_C128)
+    return fft_axis<double>(reinterpret_cast<const double2*>(in), reinterpret_cast<double2*>(out),
+                            outer, n, (size_t)inner, inverse != 0, tw->t128, S(stream));
+  if (precision == MSC_C64)
+    return fft_axis<float>(reinterpret_cast<const float2*>(in), reinterpret_cast<float2*>(out),
+                           outer, n, (size_t)inner, inverse != 0, tw->t64, S(stream));
+  MSC_REQUIRE(false, "precision must be 0 (complex128) or 1 (complex64)");
+}
+
+int msc_arnoldi_create(msc_arnoldi** ctx, long long n, int m, int precision) {
+  MSC_REQUIRE(ctx != nullptr, "ctx must not be NULL");
+  *ctx = nullptr;
+  MSC_REQUIRE(n >= 1 && m >= 1, "n and m must be >= 1");
+  MSC_REQUIRE(precision == MSC_C128 || precision == MSC_C64, "precision must be 0 or 1");
+  msc_arnoldi* c = new (std::nothrow) msc_arnoldi();
+  MSC_REQUIRE(c != nullptr, "out of host memory");
+  c->precision = precision;
+  const int rc = c->k.ensure_state((size_t)n, m, precision == MSC_C128 ? 16 : 8);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *ctx = c;
+  return kOk;
+}
+
+int msc_arnoldi_destroy(msc_arnoldi* ctx) {
+  if (ctx == nullptr) return kOk;
+  (void)hipDeviceSynchronize();
+  delete ctx;
+  return kOk;
+}
+
+int msc_arnoldi_start(msc_arnoldi* ctx, void* V, long long ldv, const void* r, double tol_abs,
+                      void* stream) {
+  MSC_REQUIRE(ctx != nullptr && V != nullptr && r != nullptr, "NULL argument");
+  MSC_REQUIRE(ldv >= (long long)ctx->k.n, "ldv must be >= n");
+  Krylov& k = ctx->k;
+  k.V = V;
+  k.ldv = (size_t)ldv;
+  int rc;
+  if (ctx->precision == MSC_C128) {
+    rc = krylov_norm<double>(k, reinterpret_cast<const double2*>(r), &k.st->rnorm, S(stream));
+    if (!rc) rc = krylov_cycle_init<double, double>(k, reinterpret_cast<const double2*>(r), tol_abs,
+                                                    false, S(stream));
+  } else {
+    rc = krylov_norm<float>(k, reinterpret_cast<const float2*>(r), &k.st->rnorm, S(stream));
+    if (!rc) rc = krylov_cycle_init<float, float>(k, reinterpret_cast<const float2*>(r), tol_abs,
+                                                  false, S(stream));
+  }
+  return rc;
+}
+
+int msc_arnoldi_step(msc_arnoldi* ctx, void* V, long long ldv, int j, int orth, void* stream) {
+  MSC_REQUIRE(ctx != nullptr && V != nullptr, "NULL argument");
+  MSC_REQUIRE(j >= 0 && j < ctx->k.m, "step index out of range");
+  MSC_REQUIRE(orth >= 0 && orth <= 2, "orth must be 0, 1 or 2");
+  Krylov& k = ctx->k;
+  k.V = V;
+  k.ldv = (size_t)ldv;
+  return ctx->precision == MSC_C128 ? krylov_step<double>(k, j, orth, S(stream))
+                                    : krylov_step<float>(k, j, orth, S(stream));
+}
+
+int msc_arnoldi_state(msc_arnoldi* ctx, double* out4, void* stream) {
+  MSC_REQUIRE(ctx != nullptr && out4 != nullptr, "NULL argument");
+  KState s{};
+  const int rc = krylov_read_state(ctx->k, &s, S(stream));
+  if (rc) return rc;
+  out4[0] = s.j;
+  out4[1] = s.done;
+  out4[2] = s.resid_est;
+  out4[3] = s.rnorm;
+  return kOk;
+}
+
+int msc_arnoldi_solution(msc_arnoldi* ctx, const void* V, long long ldv, void* x, void* stream) {
+  MSC_REQUIRE(ctx != nullptr && V != nullptr && x != nullptr, "NULL argument");
+  Krylov& k = ctx->k;
+  k.V = const_cast<void*>(V);
+  k.ldv = (size_t)ldv;
+  if (ctx->precision == MSC_C128)
+    return krylov_finish<double, double>(k, reinterpret_cast<double2*>(x), nullptr, S(stream));
+  return krylov_finish<float, float>(k, reinterpret_cast<float2*>(x), nullptr, S(stream));
+}
+
+int msc_timing_enable(msc_plan* plan, int enable) {
+  MSC_REQUIRE(plan != nullptr, "plan must not be NULL");
+  plan->p.timing = enable != 0;
+  plan->p.ev_used = 0;
+  return kOk;
+}
+
+int msc_timing_read(msc_plan* plan, double* out2) {
+  MSC_REQUIRE(plan != nullptr && out2 != nullptr, "NULL argument");
+  double total = 0.0;
+  size_t nl = 0;
+  const Plan& p = plan->p;
+  for (size_t i = 0; i + 1 < p.ev_used; i += 2) {
+    MSC_HIP_CHECK(hipEventSynchronize(p.ev_pool[i + 1]));
+    float ms = 0.0f;
+    MSC_HIP_CHECK(hipEventElapsedTime(&ms, p.ev_pool[i], p.ev_pool[i + 1]));
+    total += ms;
+    ++nl;
+  }
+  out2[0] = (double)nl;
+  out2[1] = total;
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// GMRES workspace and building blocks (device-resident Arnoldi, FP64 state).
+#pragma once
+
+#include "msc_common.hpp"
+
+namespace msc {
+
+enum OrthMode : int { kOrthAdaptive = 0, kOrthCGS1 = 1, kOrthCGS2 = 2 };
+
+// Device-resident step state of one GMRES cycle (FP64).
+struct KState {
+  int done;          // set when the cycle must stop (converged, breakdown, full)
+  int j;             // Arnoldi steps completed in the cycle
+  int reorth;        // second Gram-Schmidt pass requested for the current step
+  int pad;
+  double rnorm;      // ||r|| of the cycle's starting residual
+  double tol_abs;    // stop when |g_{j+1}| <= tol_abs
+  double wn_before;  // ||w||^2 before the first projection
+  double resid_est;  // |g_{j+1}| after the last step
+};
+
+struct Krylov {
+  size_t n = 0, ldv = 0;
+  int m = 0, ldh = 0, nblk = 0, elem_bytes = 0;
+  void* V = nullptr;          // (m+1) x ldv basis, complex64 or complex128
+  KState* st = nullptr;
+  double2* H = nullptr;       // (m+1) x m Hessenberg, column-major
+  double2* cs = nullptr;      // rotations: cs[2i] = (c, 0), cs[2i+1] = s
+  double2* g = nullptr;       // rotated right-hand side (m+1)
+  double2* hcur = nullptr;    // projection coefficients of the current pass
+  double2* y = nullptr;       // least-squares solution
+  double2* tmp = nullptr;
+  double* scal = nullptr;     // 4 device scalars (norms, scale factors)
+  double2* part = nullptr;    // per-workgroup partials
+  double* hnext = nullptr;    // ||w|| of each step (unrotated H[j+1, j])
+  void* host = nullptr;       // pinned staging of KState
+  bool owns_V = false;
+
+  ~Krylov();
+  void release();
+  // state only (caller-owned basis) / state plus an owned basis
+  int ensure_state(size_t n, int m, int elem_bytes);
+  int ensure(size_t n, int m, int elem_bytes);
+};
+
+template <typename R>
+int krylov_step(Krylov& k, int j, int orth_mode, hipStream_t s);
+template <typename R>
+int krylov_norm(Krylov& k, const cx<R>* r, double* out_dev, hipStream_t s);
+template <typename RS, typename R>
+int krylov_cycle_init(Krylov& k, const cx<RS>* r, double tol_abs, bool unit_rhs, hipStream_t s);
+template <typename R, typename RX>
+int krylov_finish(Krylov& k, cx<RX>* x, const double* scale_dev, hipStream_t s);
+int krylov_read_state(Krylov& k, KState* out, hipStream_t s);
+
+}  // namespace msc

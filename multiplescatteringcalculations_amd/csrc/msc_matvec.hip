@@ -1,0 +1,561 @@
+// FFT-accelerated Foldy-Lax operator y = (I - G0 T0) x on gfx950.
+//
+// Replaces _slab_matvec_gpu (cubic_scattering/slab_scattering_gpu.py:95-155)
+// and _slab_matvec (cubic_scattering/slab_scattering.py:359-418).  The 3-D
+// linear convolution of tau = T0 x with the spatial propagator is done on a
+// power-of-two grid (Pz, Pxy, Pxy) >= (2N_z-1, 2M-1, 2M-1) with the kernel
+// stored circularly (offset d at index d mod P), so the valid output window
+// is [0, N_z) x [0, M) x [0, M).  Five LDS-staged passes, each one HBM
+// round trip, with the zero padding never read or written:
+//   1. y-forward  : T0 gather + zero pad + FFT along y      (N_z*M rows)
+//   2. x-forward  : FFT along x of the M non-zero rows       (N_z planes)
+//   3. z-multiply : FFT along z, 9x9 block multiply with the compressed
+//                   table, inverse FFT along z, keep z < N_z (Pxy^2 columns)
+//   4. x-inverse  : inverse FFT along x, keep x < M
+//   5. y-inverse  : inverse FFT along y, keep y < M, y = x - acc (or the
+//                   residual b - x + acc)
+#include <cstdint>
+
+#include "msc_fft.hpp"
+#include "msc_plan.hpp"
+
+namespace msc {
+
+constexpr int kNT = 256;  // threads per workgroup for every pass
+
+// Component sign flips of the lattice reflections (z, x, y) acting on the
+// 9-vector (u_z, u_x, u_y, e_zz, e_xx, e_yy, e_xy, e_zy, e_zx):
+// a component flips when it carries an odd number of the reflected index.
+constexpr unsigned kFlipZ = (1u << 0) | (1u << 7) | (1u << 8);
+constexpr unsigned kFlipX = (1u << 1) | (1u << 6) | (1u << 8);
+constexpr unsigned kFlipY = (1u << 2) | (1u << 6) | (1u << 7);
+
+template <typename R>
+__device__ __forceinline__ cx<R> flip_if(cx<R> v, unsigned mask, int c) {
+  return ((mask >> c) & 1u) ? mk<R>(-v.x, -v.y) : v;
+}
+
+// x<->y swap of the 9-vector: u_x<->u_y, e_xx<->e_yy, e_zy<->e_zx.
+template <typename R>
+__device__ __forceinline__ void swap_xy(cx<R>* t) {
+  cx<R> q;
+  q = t[1]; t[1] = t[2]; t[2] = q;
+  q = t[4]; t[4] = t[5]; t[5] = q;
+  q = t[7]; t[7] = t[8]; t[8] = q;
+}
+
+// o = K t for the canonical spectral propagator K = [[G, C], [H, S]] stored
+// as 45 entries (G sym 6, C 3x6, upper triangle of S' = S W^-1 21) with
+// H = W^-1 C^T and S = S' W, W = diag(1,1,1,1/2,1/2,1/2).
+template <typename R>
+__device__ __forceinline__ void kmul45(const cx<R>* __restrict__ tab, int NC, int c,
+                                       const cx<R>* t, cx<R>* o) {
+#define KE(e) tab[(e) * NC + c]
+  {
+    const cx<R> g00 = KE(0), g01 = KE(1), g02 = KE(2), g11 = KE(3), g12 = KE(4), g22 = KE(5);
+    o[0] = cmul<R>(g00, t[0]);
+    cfma<R>(o[0], g01, t[1]);
+    cfma<R>(o[0], g02, t[2]);
+    o[1] = cmul<R>(g01, t[0]);
+    cfma<R>(o[1], g11, t[1]);
+    cfma<R>(o[1], g12, t[2]);
+    o[2] = cmul<R>(g02, t[0]);
+    cfma<R>(o[2], g12, t[1]);
+    cfma<R>(o[2], g22, t[2]);
+  }
+#pragma unroll
+  for (int al = 0; al < 6; ++al) {
+    const cx<R> c0 = KE(6 + al), c1 = KE(12 + al), c2 = KE(18 + al);
+    const R wi = al < 3 ? R(1) : R(2);
+    cfma<R>(o[0], c0, t[3 + al]);
+    cfma<R>(o[1], c1, t[3 + al]);
+    cfma<R>(o[2], c2, t[3 + al]);
+    cx<R> h = cmul<R>(c0, t[0]);
+    cfma<R>(h, c1, t[1]);
+    cfma<R>(h, c2, t[2]);
+    o[3 + al] = cscale<R>(h, wi);
+  }
+  cx<R> tw[6];
+#pragma unroll
+  for (int be = 0; be < 6; ++be) tw[be] = be < 3 ? t[3 + be] : cscale<R>(t[3 + be], R(0.5));
+#pragma unroll
+  for (int al = 0; al < 6; ++al) {
+#pragma unroll
+    for (int be = al; be < 6; ++be) {
+      const cx<R> s = KE(24 + al * 6 - (al * (al - 1)) / 2 + (be - al));
+      cfma<R>(o[3 + al], s, tw[be]);
+      if (be != al) cfma<R>(o[3 + be], s, tw[al]);
+    }
+  }
+#undef KE
+}
+
+// ── pass 1: T0 gather + zero pad + forward FFT along y ─────────────────────
+template <typename R, int P, int B>
+__global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
+                                              const cx<R>* __restrict__ tp,
+                                              cx<R>* __restrict__ buf1, int M, int nrows,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  constexpr int NTR = 9 * B;
+  constexpr int LD = lds_ld<NTR>::value;
+  __shared__ cx<R> s[P * LD];
+  if (done != nullptr && *done) return;
+  const int row0 = blockIdx.x * B;
+  for (int w = threadIdx.x; w < B * P * 9; w += kNT) {
+    const int b = w / (P * 9);
+    const int rem = w - b * P * 9;
+    const int y = rem / 9;
+    const int c = rem - y * 9;
+    const int row = row0 + b;
+    cx<R> v = czero<R>();
+    if (y < M && row < nrows) v = x[(size_t)row * M * 9 + rem];
+    s[y * LD + b * 9 + c] = v;
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < B * M; w += kNT) {
+    const int b = w / M;
+    const int y = w - b * M;
+    const int row = row0 + b;
+    if (row >= nrows) continue;
+    const cx<R>* t = tp + ((size_t)row * M + y) * 4;
+    const cx<R> tu = t[0], td = t[1], to = t[2], ts = t[3];
+    cx<R>* v = s + y * LD + b * 9;
+    const cx<R> sum = cadd<R>(cadd<R>(v[3], v[4]), v[5]);
+    const cx<R> dmo = csub<R>(td, to);
+    const cx<R> tos = cmul<R>(to, sum);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = cmul<R>(tu, v[c]);
+#pragma unroll
+    for (int c = 3; c < 6; ++c) v[c] = cadd<R>(cmul<R>(dmo, v[c]), tos);
+#pragma unroll
+    for (int c = 6; c < 9; ++c) v[c] = cmul<R>(ts, v[c]);
+  }
+  __syncthreads();
+  lds_fft<R, P, false, NTR, LD, kNT>(s, tw);
+  for (int w = threadIdx.x; w < B * P * 9; w += kNT) {
+    const int b = w / (P * 9);
+    const int rem = w - b * P * 9;
+    const int y = rem / 9;
+    const int c = rem - y * 9;
+    const int row = row0 + b;
+    if (row < nrows) buf1[(size_t)row * P * 9 + rem] = s[y * LD + b * 9 + c];
+  }
+}
+
+// ── pass 2: forward FFT along x of the M non-zero rows ─────────────────────
+template <typename R, int P, int Q>
+__global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
+                                              cx<R>* __restrict__ buf2, int M,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  constexpr int NTR = 9 * Q;
+  constexpr int LD = lds_ld<NTR>::value;
+  __shared__ cx<R> s[P * LD];
+  if (done != nullptr && *done) return;
+  constexpr int nck = P / Q;
+  const int z = blockIdx.x / nck;
+  const int ky0 = (blockIdx.x - z * nck) * Q;
+  for (int w = threadIdx.x; w < P * NTR; w += kNT) {
+    const int xr = w / NTR;
+    const int e = w - xr * NTR;
+    cx<R> v = czero<R>();
+    if (xr < M) v = buf1[(((size_t)z * M + xr) * P + ky0) * 9 + e];
+    s[xr * LD + e] = v;
+  }
+  __syncthreads();
+  lds_fft<R, P, false, NTR, LD, kNT>(s, tw);
+  for (int w = threadIdx.x; w < P * NTR; w += kNT) {
+    const int kx = w / NTR;
+    const int e = w - kx * NTR;
+    buf2[(((size_t)z * P + kx) * P + ky0) * 9 + e] = s[kx * LD + e];
+  }
+}
+
+// ── pass 3: z FFT, spectral 9x9 multiply, inverse z FFT (in place) ─────────
+template <typename R, int PZ, int Q>
+__global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
+                                               const cx<R>* __restrict__ table, int Nz,
+                                               int Pxy, const cx<R>* __restrict__ tw,
+                                               const int* __restrict__ done) {
+  constexpr int NTR = 9 * Q;
+  constexpr int LD = lds_ld<NTR>::value;
+  constexpr int NC = PZ / 2 + 1;
+  __shared__ cx<R> s[PZ * LD];
+  if (done != nullptr && *done) return;
+  const int nck = Pxy / Q;
+  const int kx = blockIdx.x / nck;
+  const int ky0 = (blockIdx.x - kx * nck) * Q;
+  const size_t plane = (size_t)Pxy * Pxy * 9;
+  const size_t col0 = ((size_t)kx * Pxy + ky0) * 9;
+  for (int w = threadIdx.x; w < PZ * NTR; w += kNT) {
+    const int z = w / NTR;
+    const int e = w - z * NTR;
+    cx<R> v = czero<R>();
+    if (z < Nz) v = buf2[(size_t)z * plane + col0 + e];
+    s[z * LD + e] = v;
+  }
+  __syncthreads();
+  lds_fft<R, PZ, false, NTR, LD, kNT>(s, tw);
+  const int half = Pxy / 2;
+  for (int w = threadIdx.x; w < PZ * Q; w += kNT) {
+    const int q = w / PZ;
+    const int kz = w - q * PZ;
+    const int ky = ky0 + q;
+    const int a = kx <= half ? kx : Pxy - kx;
+    const int b = ky <= half ? ky : Pxy - ky;
+    const int c = kz <= PZ / 2 ? kz : PZ - kz;
+    const unsigned flip = (kz > PZ / 2 ? kFlipZ : 0u) ^ (kx > half ? kFlipX : 0u) ^
+                          (ky > half ? kFlipY : 0u);
+    const bool sw = a < b;
+    const int u = sw ? b : a;
+    const int vv = sw ? a : b;
+    const int uv = (u * (u + 1)) / 2 + vv;
+    cx<R>* sp = s + kz * LD + q * 9;
+    cx<R> t[9], o[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t[i] = flip_if<R>(sp[i], flip, i);
+    if (sw) swap_xy<R>(t);
+    kmul45<R>(table + (size_t)uv * kNE * NC, NC, c, t, o);
+    if (sw) swap_xy<R>(o);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sp[i] = flip_if<R>(o[i], flip, i);
+  }
+  __syncthreads();
+  lds_fft<R, PZ, true, NTR, LD, kNT>(s, tw);
+  for (int w = threadIdx.x; w < Nz * NTR; w += kNT) {
+    const int z = w / NTR;
+    const int e = w - z * NTR;
+    buf2[(size_t)z * plane + col0 + e] = s[z * LD + e];
+  }
+}
+
+// ── pass 4: inverse FFT along x, keep x < M ────────────────────────────────
+template <typename R, int P, int Q>
+__global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
+                                              cx<R>* __restrict__ buf1, int M,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  constexpr int NTR = 9 * Q;
+  constexpr int LD = lds_ld<NTR>::value;
+  __shared__ cx<R> s[P * LD];
+  if (done != nullptr && *done) return;
+  constexpr int nck = P / Q;
+  const int z = blockIdx.x / nck;
+  const int ky0 = (blockIdx.x - z * nck) * Q;
+  for (int w = threadIdx.x; w < P * NTR; w += kNT) {
+    const int kx = w / NTR;
+    const int e = w - kx * NTR;
+    s[kx * LD + e] = buf2[(((size_t)z * P + kx) * P + ky0) * 9 + e];
+  }
+  __syncthreads();
+  lds_fft<R, P, true, NTR, LD, kNT>(s, tw);
+  for (int w = threadIdx.x; w < M * NTR; w += kNT) {
+    const int xr = w / NTR;
+    const int e = w - xr * NTR;
+    buf1[(((size_t)z * M + xr) * P + ky0) * 9 + e] = s[xr * LD + e];
+  }
+}
+
+// ── pass 5: inverse FFT along y, keep y < M, y = x - acc (or b - x + acc) ──
+template <typename R, int P, int B, bool RESID>
+__global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
+                                              const cx<R>* __restrict__ x,
+                                              const cx<R>* __restrict__ bvec,
+                                              cx<R>* __restrict__ out, int M, int nrows,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  constexpr int NTR = 9 * B;
+  constexpr int LD = lds_ld<NTR>::value;
+  __shared__ cx<R> s[P * LD];
+  if (done != nullptr && *done) return;
+  const int row0 = blockIdx.x * B;
+  for (int w = threadIdx.x; w < B * P * 9; w += kNT) {
+    const int b = w / (P * 9);
+    const int rem = w - b * P * 9;
+    const int y = rem / 9;
+    const int c = rem - y * 9;
+    const int row = row0 + b;
+    cx<R> v = czero<R>();
+    if (row < nrows) v = buf1[(size_t)row * P * 9 + rem];
+    s[y * LD + b * 9 + c] = v;
+  }
+  __syncthreads();
+  lds_fft<R, P, true, NTR, LD, kNT>(s, tw);
+  for (int w = threadIdx.x; w < B * M * 9; w += kNT) {
+    const int b = w / (M * 9);
+    const int rem = w - b * M * 9;
+    const int y = rem / 9;
+    const int c = rem - y * 9;
+    const int row = row0 + b;
+    if (row >= nrows) continue;
+    const size_t g = (size_t)row * M * 9 + rem;
+    const cx<R> acc = s[y * LD + b * 9 + c];
+    if constexpr (RESID) {
+      out[g] = cadd<R>(csub<R>(bvec[g], x[g]), acc);
+    } else {
+      out[g] = csub<R>(x[g], acc);
+    }
+  }
+}
+
+// ── host dispatch ───────────────────────────────────────────────────────────
+
+template <typename R> struct PlanBufs;
+template <> struct PlanBufs<double> {
+  static const double2* table(const Plan& p) { return p.table128; }
+  static const double2* tp(const Plan& p) { return p.tp128; }
+  static const double2* tw(const Plan& p) { return p.tw128; }
+};
+template <> struct PlanBufs<float> {
+  static const float2* table(const Plan& p) { return p.table64; }
+  static const float2* tp(const Plan& p) { return p.tp64; }
+  static const float2* tw(const Plan& p) { return p.tw64; }
+};
+
+// Rows per workgroup of the y passes and columns per workgroup of the x/z
+// passes, chosen so that the LDS tile stays <= ~80 KB.
+template <typename R, int P> struct YRows { static constexpr int value = (P * 18 * sizeof(cx<R>) <= 80 * 1024) ? 2 : 1; };
+template <typename R, int P> struct XCols { static constexpr int value = (P >= 2 && P * 18 * sizeof(cx<R>) <= 80 * 1024) ? 2 : 1; };
+template <typename R, int PZ> struct ZCols {
+  static constexpr int c4 = (PZ * 36 * sizeof(cx<R>) <= 80 * 1024) ? 4 : 2;
+  static constexpr int value = c4;
+};
+
+template <typename R, int P>
+static int launch_xy_fwd(Plan& p, const cx<R>* x, cx<R>* b1, cx<R>* b2, hipStream_t st, const int* done) {
+  constexpr int B = YRows<R, P>::value;
+  constexpr int Q = XCols<R, P>::value < P ? XCols<R, P>::value : 1;
+  const int nrows = p.Nz * p.M;
+  const cx<R>* tw = PlanBufs<R>::tw(p);
+  hipLaunchKernelGGL((k_fwd_y<R, P, B>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, x,
+                     PlanBufs<R>::tp(p), b1, p.M, nrows, tw, done);
+  MSC_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_fwd_x<R, P, Q>), dim3(p.Nz * (P / Q)), dim3(kNT), 0, st, b1, b2, p.M, tw,
+                     done);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename R, int P>
+static int launch_xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, cx<R>* b1,
+                         cx<R>* b2, hipStream_t st, const int* done) {
+  constexpr int B = YRows<R, P>::value;
+  constexpr int Q = XCols<R, P>::value < P ? XCols<R, P>::value : 1;
+  const int nrows = p.Nz * p.M;
+  const cx<R>* tw = PlanBufs<R>::tw(p);
+  hipLaunchKernelGGL((k_inv_x<R, P, Q>), dim3(p.Nz * (P / Q)), dim3(kNT), 0, st, b2, b1, p.M, tw,
+                     done);
+  MSC_LAUNCH_CHECK();
+  if (bvec != nullptr) {
+    hipLaunchKernelGGL((k_inv_y<R, P, B, true>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, b1,
+                       x, bvec, out, p.M, nrows, tw, done);
+  } else {
+    hipLaunchKernelGGL((k_inv_y<R, P, B, false>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st,
+                       b1, x, bvec, out, p.M, nrows, tw, done);
+  }
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+int timing_begin(Plan& p, hipStream_t st, size_t* slot) {
+  *slot = SIZE_MAX;
+  if (!p.timing) return kOk;
+  if (p.ev_used + 2 > p.ev_pool.size()) {
+    for (int i = 0; i < 256; ++i) {
+      hipEvent_t e;
+      MSC_HIP_CHECK(hipEventCreate(&e));
+      p.ev_pool.push_back(e);
+    }
+  }
+  MSC_HIP_CHECK(hipEventRecord(p.ev_pool[p.ev_used], st));
+  *slot = p.ev_used;
+  p.ev_used += 2;
+  return kOk;
+}
+
+int timing_end(Plan& p, hipStream_t st, size_t slot) {
+  if (slot == SIZE_MAX) return kOk;
+  MSC_HIP_CHECK(hipEventRecord(p.ev_pool[slot + 1], st));
+  return kOk;
+}
+
+template <typename R, int PZ>
+static int launch_z(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
+  size_t slot;
+  int rc = timing_begin(p, st, &slot);
+  if (rc) return rc;
+  constexpr int Q0 = ZCols<R, PZ>::value;
+  const int Q = (p.Pxy % Q0 == 0) ? Q0 : 1;
+  const cx<R>* tw = PlanBufs<R>::tw(p);
+  const cx<R>* tab = PlanBufs<R>::table(p);
+  if (Q == Q0) {
+    hipLaunchKernelGGL((k_z_mult<R, PZ, Q0>), dim3(p.Pxy * (p.Pxy / Q0)), dim3(kNT), 0, st, b2,
+                       tab, p.Nz, p.Pxy, tw, done);
+  } else {
+    hipLaunchKernelGGL((k_z_mult<R, PZ, 1>), dim3(p.Pxy * p.Pxy), dim3(kNT), 0, st, b2, tab, p.Nz,
+                       p.Pxy, tw, done);
+  }
+  MSC_LAUNCH_CHECK();
+  return timing_end(p, st, slot);
+}
+
+#define MSC_XY_CASES(F, ...)                 \
+  switch (p.Pxy) {                           \
+    case 2: return F<R, 2>(__VA_ARGS__);     \
+    case 4: return F<R, 4>(__VA_ARGS__);     \
+    case 8: return F<R, 8>(__VA_ARGS__);     \
+    case 16: return F<R, 16>(__VA_ARGS__);   \
+    case 32: return F<R, 32>(__VA_ARGS__);   \
+    case 64: return F<R, 64>(__VA_ARGS__);   \
+    case 128: return F<R, 128>(__VA_ARGS__); \
+    case 256: return F<R, 256>(__VA_ARGS__); \
+    case 512: return F<R, 512>(__VA_ARGS__); \
+    default: break;                          \
+  }
+
+#define MSC_Z_CASES(F, ...)                  \
+  switch (p.Pz) {                            \
+    case 2: return F<R, 2>(__VA_ARGS__);     \
+    case 4: return F<R, 4>(__VA_ARGS__);     \
+    case 8: return F<R, 8>(__VA_ARGS__);     \
+    case 16: return F<R, 16>(__VA_ARGS__);   \
+    case 32: return F<R, 32>(__VA_ARGS__);   \
+    case 64: return F<R, 64>(__VA_ARGS__);   \
+    case 128: return F<R, 128>(__VA_ARGS__); \
+    case 256: return F<R, 256>(__VA_ARGS__); \
+    default: break;                          \
+  }
+
+template <typename R>
+static int xy_fwd(Plan& p, const cx<R>* x, cx<R>* b1, cx<R>* b2, hipStream_t st, const int* done) {
+  MSC_XY_CASES(launch_xy_fwd, p, x, b1, b2, st, done);
+  set_error("unsupported lateral FFT length");
+  return kUnsupported;
+}
+template <typename R>
+static int xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, cx<R>* b1, cx<R>* b2,
+                  hipStream_t st, const int* done) {
+  MSC_XY_CASES(launch_xy_inv, p, x, bvec, out, b1, b2, st, done);
+  set_error("unsupported lateral FFT length");
+  return kUnsupported;
+}
+template <typename R>
+static int z_pass(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
+  MSC_Z_CASES(launch_z, p, b2, st, done);
+  set_error("unsupported vertical FFT length");
+  return kUnsupported;
+}
+
+template <typename R>
+static int apply(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, hipStream_t st,
+                 const int* done) {
+  if (!p.have_table || !p.have_tparams) {
+    set_error("plan needs build_g0 and set_tmatrix before the operator can be applied");
+    return kNotReady;
+  }
+  cx<R>* b1 = reinterpret_cast<cx<R>*>(p.buf1);
+  cx<R>* b2 = reinterpret_cast<cx<R>*>(p.buf2);
+  int rc = xy_fwd<R>(p, x, b1, b2, st, done);
+  if (rc) return rc;
+  rc = z_pass<R>(p, b2, st, done);
+  if (rc) return rc;
+  return xy_inv<R>(p, x, bvec, out, b1, b2, st, done);
+}
+
+template <typename R>
+int matvec(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done) {
+  return apply<R>(p, x, nullptr, y, st, done);
+}
+template <typename R>
+int residual(Plan& p, const cx<R>* b, const cx<R>* x, cx<R>* r, hipStream_t st, const int* done) {
+  return apply<R>(p, x, b, r, st, done);
+}
+
+template int matvec<double>(Plan&, const double2*, double2*, hipStream_t, const int*);
+template int matvec<float>(Plan&, const float2*, float2*, hipStream_t, const int*);
+template int residual<double>(Plan&, const double2*, const double2*, double2*, hipStream_t,
+                              const int*);
+template int residual<float>(Plan&, const float2*, const float2*, float2*, hipStream_t,
+                             const int*);
+
+// ── generic strided batched FFT (table build, tests) ───────────────────────
+// data [outer][N][inner]; a workgroup transforms NCOL consecutive inner
+// indices of one outer slice.
+template <typename R, int N, int NCOL, bool INV>
+__global__ __launch_bounds__(kNT) void k_fft_axis(const cx<R>* __restrict__ in,
+                                                 cx<R>* __restrict__ out, size_t inner,
+                                                 const cx<R>* __restrict__ tw) {
+  constexpr int LD = lds_ld<NCOL>::value;
+  __shared__ cx<R> s[N * LD];
+  const size_t nchunk = (inner + NCOL - 1) / NCOL;
+  const size_t o = blockIdx.x / nchunk;
+  const size_t i0 = (blockIdx.x - o * nchunk) * NCOL;
+  const size_t base = o * (size_t)N * inner + i0;
+  for (int w = threadIdx.x; w < N * NCOL; w += kNT) {
+    const int r = w / NCOL;
+    const int e = w - r * NCOL;
+    cx<R> v = czero<R>();
+    if (i0 + e < inner) v = in[base + (size_t)r * inner + e];
+    s[r * LD + e] = v;
+  }
+  __syncthreads();
+  lds_fft<R, N, INV, NCOL, LD, kNT>(s, tw);
+  for (int w = threadIdx.x; w < N * NCOL; w += kNT) {
+    const int r = w / NCOL;
+    const int e = w - r * NCOL;
+    if (i0 + e < inner) out[base + (size_t)r * inner + e] = s[r * LD + e];
+  }
+}
+
+template <typename R, int N>
+static int launch_fft_axis(const cx<R>* in, cx<R>* out, int outer, size_t inner, bool inv,
+                           const cx<R>* tw, hipStream_t st) {
+  constexpr int NCOL = (N <= 64) ? 16 : ((N * 16 * sizeof(cx<R>) <= 80 * 1024) ? 16 : 8);
+  const size_t nchunk = (inner + NCOL - 1) / NCOL;
+  const size_t nblk = (size_t)outer * nchunk;
+  MSC_REQUIRE(nblk < (1ull << 31), "FFT batch too large");
+  if (inv) {
+    hipLaunchKernelGGL((k_fft_axis<R, N, NCOL, true>), dim3((unsigned)nblk), dim3(kNT), 0, st, in,
+                       out, inner, tw);
+  } else {
+    hipLaunchKernelGGL((k_fft_axis<R, N, NCOL, false>), dim3((unsigned)nblk), dim3(kNT), 0, st, in,
+                       out, inner, tw);
+  }
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename R>
+int fft_axis(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, bool inverse,
+             const cx<R>* tw, hipStream_t st) {
+  switch (N) {
+    case 1: {
+      if (in != out) {
+        MSC_HIP_CHECK(hipMemcpyAsync(out, in, sizeof(cx<R>) * (size_t)outer * inner,
+                                     hipMemcpyDeviceToDevice, st));
+      }
+      return kOk;
+    }
+    case 2: return launch_fft_axis<R, 2>(in, out, outer, inner, inverse, tw, st);
+    case 4: return launch_fft_axis<R, 4>(in, out, outer, inner, inverse, tw, st);
+    case 8: return launch_fft_axis<R, 8>(in, out, outer, inner, inverse, tw, st);
+    case 16: return launch_fft_axis<R, 16>(in, out, outer, inner, inverse, tw, st);
+    case 32: return launch_fft_axis<R, 32>(in, out, outer, inner, inverse, tw, st);
+    case 64: return launch_fft_axis<R, 64>(in, out, outer, inner, inverse, tw, st);
+    case 128: return launch_fft_axis<R, 128>(in, out, outer, inner, inverse, tw, st);
+    case 256: return launch_fft_axis<R, 256>(in, out, outer, inner, inverse, tw, st);
+    case 512: return launch_fft_axis<R, 512>(in, out, outer, inner, inverse, tw, st);
+    case 1024: return launch_fft_axis<R, 1024>(in, out, outer, inner, inverse, tw, st);
+    default: break;
+  }
+  set_error("fft_axis: length must be a power of two <= 1024");
+  return kUnsupported;
+}
+
+template int fft_axis<double>(const double2*, double2*, int, int, size_t, bool, const double2*,
+                              hipStream_t);
+template int fft_axis<float>(const float2*, float2*, int, int, size_t, bool, const float2*,
+                             hipStream_t);
+
+}  // namespace msc

@@ -1,0 +1,74 @@
+// Plan object of the slab Foldy-Lax library: geometry, padded FFT grid, the
+// compressed spectral G0 table, per-voxel T0 parameters and all workspaces.
+//
+// HBM layout (n_vox = N_z*M*M voxels, component index innermost):
+//   field vectors      (N_z, M, M, 9)            complex, C-contiguous
+//   T0 parameters      (N_z, M, M, 4)            complex: t_u, t_d, t_o, t_s
+//   buf1 (y-spectra)   (N_z, M, Pxy, 9)          complex
+//   buf2 (xy-spectra)  (N_z, Pxy, Pxy, 9)        complex
+//   G0 table           (NUV, 45, NC)             complex, NC = Pz/2+1,
+//                      NUV = (Pxy/2+1)(Pxy/2+2)/2 canonical (u >= v) columns
+// The table holds the 45 independent entries of the 9x9 spectral propagator
+// on the octant 0 <= kz <= Pz/2, 0 <= v <= u <= Pxy/2, pre-scaled by 1/P.
+#pragma once
+
+#include <vector>
+
+#include "msc_common.hpp"
+
+namespace msc {
+
+constexpr int kNE = 45;  // independent entries of the 9x9 spectral propagator
+
+struct Krylov;  // GMRES workspace (msc_gmres.hip)
+
+struct Plan {
+  int M = 0, Nz = 0;
+  double a = 0.0;
+  bool periodic = false;
+  int Pz = 0, Pxy = 0;
+  size_t nvox = 0, n = 0;  // voxels, complex unknowns (9 per voxel)
+  int NC = 0, NUV = 0;
+
+  double omega = 0.0, alpha = 0.0, beta = 0.0, rho = 0.0;
+  bool have_table = false, have_tparams = false;
+
+  double2* table128 = nullptr;
+  float2* table64 = nullptr;
+  double2* tp128 = nullptr;
+  float2* tp64 = nullptr;
+  void* buf1 = nullptr;  // (Nz, M, Pxy, 9) complex128 bytes
+  void* buf2 = nullptr;  // (Nz, Pxy, Pxy, 9) complex128 bytes
+  double2* tw128 = nullptr;
+  float2* tw64 = nullptr;
+
+  int device = 0;
+
+  // HIP-event timing of the block-multiply pass (pass 3), one pair per launch
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+};
+
+// Event pair around one pass-3 launch when p.timing is set.
+int timing_begin(Plan& p, hipStream_t st, size_t* slot);
+int timing_end(Plan& p, hipStream_t st, size_t slot);
+
+// Matvec y = (I - G0 T0) x in precision R on the caller's stream.  `done`
+// (device int, may be null) turns every launch into a no-op when set.
+template <typename R>
+int matvec(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done);
+
+// Residual r = b - (I - G0 T0) x (fused into the last FFT pass).
+template <typename R>
+int residual(Plan& p, const cx<R>* b, const cx<R>* x, cx<R>* r, hipStream_t st, const int* done);
+
+// Generic strided batched FFT used by the table build:
+// data viewed as [outer][N][inner], transform along N for every inner index.
+template <typename R>
+int fft_axis(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, bool inverse,
+             const cx<R>* tw, hipStream_t st);
+
+int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st);
+
+}  // namespace msc

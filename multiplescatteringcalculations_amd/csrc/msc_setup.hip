@@ -1,0 +1,452 @@
+// Operator assembly on the device: spectral G0 table, per-voxel T0, incident
+// field and the reductions behind the R_PP / far-field outputs (gfx950).
+//
+// * G0 table: replaces _build_slab_kernels (cubic_scattering/slab_scattering.py:
+//   267-351) and _build_slab_kernels_gpu (slab_scattering_gpu.py:34-92).  The
+//   9x9 Kupradze propagator [[G, C], [H, S]] (resonance_tmatrix.py:162-428) is
+//   evaluated in FP64 at every lattice offset of the padded circular grid,
+//   its 45 independent entries are 3-D FFT'd, and the octant that the lattice
+//   reflections and the x<->y swap leave independent is kept (scaled by 1/P).
+// * T0: the closed-form Rayleigh cube T-matrix reduced to 4 complex numbers
+//   per voxel (effective_contrasts.py:360-451, resonance_tmatrix.py:461-487).
+// * psi0: plane-wave incident field (slab_scattering.py:426-478).
+#include <cmath>
+
+#include "msc_fft.hpp"
+#include "msc_plan.hpp"
+#include "msc_setup.hpp"
+
+namespace msc {
+
+namespace {
+
+struct Rad {
+  double2 phi, psi, phi1, psi1, phi2, psi2;
+};
+
+__device__ __forceinline__ double2 zc() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 cmd(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 csc(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cad(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+
+// c e^{ikr} r^p and its first two r-derivatives accumulated into f, f1, f2
+// (resonance_tmatrix.py:224-271).
+__device__ __forceinline__ void rterm(double2 c, double2 e, double k, double r, int p, double2& f,
+                                      double2& f1, double2& f2) {
+  const double rp = pow(r, (double)p);
+  const double2 ce = cmd(c, e);
+  const double2 ikr = make_double2(0.0, k * r);
+  f = cad(f, csc(ce, rp));
+  // (ikr + p)
+  f1 = cad(f1, csc(cmd(ce, make_double2((double)p, k * r)), rp / r));
+  // (ikr)^2 + 2p ikr + p(p-1) = (-(kr)^2 + p(p-1)) + i 2p kr
+  const double2 q = make_double2(-(k * r) * (k * r) + (double)(p * (p - 1)), 2.0 * p * ikr.y);
+  f2 = cad(f2, csc(cmd(ce, q), rp / (r * r)));
+}
+
+__device__ Rad radial(double r, double kP, double kS) {
+  double sP, cP, sS, cS;
+  sincos(kP * r, &sP, &cP);
+  sincos(kS * r, &sS, &cS);
+  const double2 eP = make_double2(cP, sP), eS = make_double2(cS, sS);
+  Rad o;
+  o.phi = o.phi1 = o.phi2 = zc();
+  o.psi = o.psi1 = o.psi2 = zc();
+  rterm(make_double2(kS * kS, 0.0), eS, kS, r, -1, o.phi, o.phi1, o.phi2);
+  rterm(make_double2(0.0, kS), eS, kS, r, -2, o.phi, o.phi1, o.phi2);
+  rterm(make_double2(-1.0, 0.0), eS, kS, r, -3, o.phi, o.phi1, o.phi2);
+  rterm(make_double2(0.0, -kP), eP, kP, r, -2, o.phi, o.phi1, o.phi2);
+  rterm(make_double2(1.0, 0.0), eP, kP, r, -3, o.phi, o.phi1, o.phi2);
+  rterm(make_double2(-kS * kS, 0.0), eS, kS, r, -1, o.psi, o.psi1, o.psi2);
+  rterm(make_double2(0.0, -3.0 * kS), eS, kS, r, -2, o.psi, o.psi1, o.psi2);
+  rterm(make_double2(3.0, 0.0), eS, kS, r, -3, o.psi, o.psi1, o.psi2);
+  rterm(make_double2(kP * kP, 0.0), eP, kP, r, -1, o.psi, o.psi1, o.psi2);
+  rterm(make_double2(0.0, 3.0 * kP), eP, kP, r, -2, o.psi, o.psi1, o.psi2);
+  rterm(make_double2(-3.0, 0.0), eP, kP, r, -3, o.psi, o.psi1, o.psi2);
+  return o;
+}
+
+__device__ __forceinline__ double dl(int i, int j) { return i == j ? 1.0 : 0.0; }
+
+// The 45 independent entries of [[G, C], [H, S]] (without the common factor
+// P = 1/(4 pi rho w^2)), added into acc:
+//   G_ij        (i <= j), 6 entries
+//   C[i][al]    3 x 6 (engineering halving of off-diagonal al)
+//   S'[al][be]  = S[al][be] / W_be, al <= be, 21 entries
+__device__ void add_prop45(double rz, double rx, double ry, double kP, double kS, double pre,
+                           double2* acc) {
+  const double r = sqrt(rz * rz + rx * rx + ry * ry);
+  const double g[3] = {rz / r, rx / r, ry / r};
+  const Rad R = radial(r, kP, kS);
+  const double2 c1 = R.phi1;
+  const double2 c2 = cad(R.psi1, csc(R.psi, -2.0 / r));
+  const double2 c3 = csc(R.psi, 1.0 / r);
+  const double2 t1 = csc(R.phi1, 1.0 / r);
+  const double2 t2 = cad(R.phi2, csc(R.phi1, -1.0 / r));
+  const double2 t3 = cad(csc(R.psi1, 1.0 / r), csc(R.psi, -2.0 / (r * r)));
+  const double2 t4 = csc(R.psi, 1.0 / (r * r));
+  const double2 t7 = cad(cad(R.psi2, csc(R.psi1, -5.0 / r)), csc(R.psi, 8.0 / (r * r)));
+
+  auto Gd = [&](int i, int j, int k) -> double2 {
+    double2 v = csc(c1, g[k] * dl(i, j));
+    v = cad(v, csc(c2, g[i] * g[j] * g[k]));
+    v = cad(v, csc(c3, dl(i, k) * g[j] + dl(j, k) * g[i]));
+    return v;
+  };
+  auto Gdd = [&](int i, int j, int k, int l) -> double2 {
+    double2 v = csc(t1, dl(i, j) * dl(k, l));
+    v = cad(v, csc(t2, dl(i, j) * g[k] * g[l]));
+    v = cad(v, csc(t3, g[i] * g[j] * dl(k, l) + dl(i, l) * g[j] * g[k] + dl(j, l) * g[i] * g[k] +
+                           dl(i, k) * g[j] * g[l] + dl(j, k) * g[i] * g[l]));
+    v = cad(v, csc(t4, dl(i, k) * dl(j, l) + dl(j, k) * dl(i, l)));
+    v = cad(v, csc(t7, g[i] * g[j] * g[k] * g[l]));
+    return v;
+  };
+  const int VP[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+  int e = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      const double2 v = cad(csc(R.phi, dl(i, j)), csc(R.psi, g[i] * g[j]));
+      acc[e] = cad(acc[e], csc(v, pre));
+      ++e;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int al = 0; al < 6; ++al) {
+      const int p = VP[al][0], q = VP[al][1];
+      const double2 v = p == q ? Gd(i, p, p) : csc(cad(Gd(i, p, q), Gd(i, q, p)), 0.5);
+      acc[e] = cad(acc[e], csc(v, pre));
+      ++e;
+    }
+  }
+#pragma unroll
+  for (int al = 0; al < 6; ++al) {
+#pragma unroll
+    for (int be = al; be < 6; ++be) {
+      const int p = VP[al][0], q = VP[al][1], m = VP[be][0], nn = VP[be][1];
+      double2 v = Gdd(p, m, nn, q);
+      if (m != nn) v = cad(v, Gdd(p, nn, m, q));
+      if (p != q) {
+        v = cad(v, Gdd(q, m, nn, p));
+        if (m != nn) v = cad(v, Gdd(q, nn, m, p));
+      }
+      acc[e] = cad(acc[e], csc(v, pre));
+      ++e;
+    }
+  }
+}
+
+// Spatial kernel on the circular grid (Pz, Pxy, Pxy): offset d stored at
+// d mod P.  Aperiodic: |dz| < Nz, |dx|,|dy| < M.  Periodic: Pxy = M and the
+// lateral offsets dx in {j, j - M} (|dx| < M) are summed (fold without Bloch
+// phase, slab_scattering.py:337-346).  The self term is zero.
+__global__ __launch_bounds__(128) void k_g0_spatial(double2* __restrict__ work, int Pz, int Pxy,
+                                                    int Nz, int M, int periodic, double d,
+                                                    double kP, double kS, double pre) {
+  const size_t total = (size_t)Pz * Pxy * Pxy;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int iy = (int)(idx % Pxy);
+    const int ix = (int)((idx / Pxy) % Pxy);
+    const int iz = (int)(idx / ((size_t)Pxy * Pxy));
+    double2 acc[kNE];
+#pragma unroll
+    for (int e = 0; e < kNE; ++e) acc[e] = zc();
+    const int dz = iz <= Pz / 2 ? iz : iz - Pz;
+    if (dz > -Nz && dz < Nz) {
+      int xs[2], ys[2], nx = 0, ny = 0;
+      if (periodic) {
+        xs[nx++] = ix;
+        if (ix >= 1) xs[nx++] = ix - M;
+        ys[ny++] = iy;
+        if (iy >= 1) ys[ny++] = iy - M;
+      } else {
+        const int dx = ix <= Pxy / 2 ? ix : ix - Pxy;
+        const int dy = iy <= Pxy / 2 ? iy : iy - Pxy;
+        if (dx > -M && dx < M) xs[nx++] = dx;
+        if (dy > -M && dy < M) ys[ny++] = dy;
+      }
+      for (int a = 0; a < nx; ++a) {
+        for (int b = 0; b < ny; ++b) {
+          if (dz == 0 && xs[a] == 0 && ys[b] == 0) continue;
+          add_prop45(dz * d, xs[a] * d, ys[b] * d, kP, kS, pre, acc);
+        }
+      }
+    }
+    double2* o = work + idx * kNE;
+#pragma unroll
+    for (int e = 0; e < kNE; ++e) o[e] = acc[e];
+  }
+}
+
+__device__ __forceinline__ void uv_split(int uv, int& u, int& v) {
+  int t = (int)((sqrt(8.0 * uv + 1.0) - 1.0) * 0.5);
+  while ((t + 1) * (t + 2) / 2 <= uv) ++t;
+  while (t * (t + 1) / 2 > uv) --t;
+  u = t;
+  v = uv - t * (t + 1) / 2;
+}
+
+// table[(uv*45 + e)*NC + c] = K^(kz=c, kx=u, ky=v)[e] / P
+__global__ __launch_bounds__(256) void k_table_compress(const double2* __restrict__ work, int Pz,
+                                                        int Pxy, int NC, int NUV, double invP,
+                                                        double2* __restrict__ t128,
+                                                        float2* __restrict__ t64) {
+  const size_t total = (size_t)NUV * kNE * NC;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % NC);
+    const int e = (int)((idx / NC) % kNE);
+    const int uv = (int)(idx / ((size_t)NC * kNE));
+    int u, v;
+    uv_split(uv, u, v);
+    const double2 s = work[(((size_t)c * Pxy + u) * Pxy + v) * kNE + e];
+    const double2 o = make_double2(s.x * invP, s.y * invP);
+    t128[idx] = o;
+    t64[idx] = make_float2((float)o.x, (float)o.y);
+  }
+}
+
+// Per-voxel T0 parameters from (Dlambda, Dmu, Drho) and the frequency
+// constants (Gamma0, A, B, C):
+//   T1 = Dl (A + 4B + C) + 2 B Dm,  T2 = (A + B) Dm,  T3 = 2 C Dm
+//   (the contraction of effective_contrasts.py:365-402 written out),
+//   amplifications (405-420), effective contrasts (423-451), V = (2a)^3:
+//   t_u = w^2 V Drho*, t_d = V(Dl* + 2 Dmu*_diag), t_o = V Dl*, t_s = 2 V Dmu*_off.
+__global__ __launch_bounds__(256) void k_tparams(const double* __restrict__ Dl,
+                                                 const double* __restrict__ Dm,
+                                                 const double* __restrict__ Dr, size_t nvox,
+                                                 double2 G0, double2 A, double2 B, double2 C,
+                                                 double omega, double V,
+                                                 double2* __restrict__ t128,
+                                                 float2* __restrict__ t64) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double dl = Dl[i], dm = Dm[i], dr = Dr[i];
+    const double2 AB = cad(A, B);
+    const double2 sum1 = cad(cad(A, csc(B, 4.0)), C);
+    const double2 T1 = cad(csc(sum1, dl), csc(B, 2.0 * dm));
+    const double2 T2 = csc(AB, dm);
+    const double2 T3 = csc(C, 2.0 * dm);
+    auto inv1m = [](double2 z) {  // 1 / (1 - z)
+      const double2 q = make_double2(1.0 - z.x, -z.y);
+      const double den = q.x * q.x + q.y * q.y;
+      return make_double2(q.x / den, -q.y / den);
+    };
+    const double2 amp_u = inv1m(csc(G0, omega * omega * dr));
+    const double2 amp_t = inv1m(cad(cad(csc(T1, 3.0), csc(T2, 2.0)), T3));
+    const double2 amp_o = inv1m(csc(T2, 2.0));
+    const double2 amp_d = inv1m(cad(csc(T2, 2.0), T3));
+    const double2 drs = csc(amp_u, dr);
+    const double2 dmo = csc(amp_o, dm);
+    const double2 dmd = csc(amp_d, dm);
+    const double2 dls = cad(csc(amp_t, dl + 2.0 / 3.0 * dm), csc(amp_d, -2.0 / 3.0 * dm));
+    const double2 tu = csc(drs, omega * omega * V);
+    const double2 td = csc(cad(dls, csc(dmd, 2.0)), V);
+    const double2 to = csc(dls, V);
+    const double2 ts = csc(dmo, 2.0 * V);
+    double2* o = t128 + i * 4;
+    float2* f = t64 + i * 4;
+    o[0] = tu;
+    o[1] = td;
+    o[2] = to;
+    o[3] = ts;
+    f[0] = make_float2((float)tu.x, (float)tu.y);
+    f[1] = make_float2((float)td.x, (float)td.y);
+    f[2] = make_float2((float)to.x, (float)to.y);
+    f[3] = make_float2((float)ts.x, (float)ts.y);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_narrow4(const double2* __restrict__ t128, size_t n,
+                                                 float2* __restrict__ t64) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    t64[i] = make_float2((float)t128[i].x, (float)t128[i].y);
+  }
+}
+
+// psi0[l,i,j,:] = inc[:] exp(i k khat . x_c), x_c = ((l+1/2)d, (i-(M-1)/2)d, (j-(M-1)/2)d)
+struct Inc9 {
+  double2 v[9];
+};
+
+__global__ __launch_bounds__(256) void k_incident(double2* __restrict__ psi0, int M, int Nz,
+                                                  double d, double kz, double kx, double ky,
+                                                  Inc9 inc) {
+  const size_t nvox = (size_t)Nz * M * M;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(v % M);
+    const int i = (int)((v / M) % M);
+    const int l = (int)(v / ((size_t)M * M));
+    const double z = (l + 0.5) * d;
+    const double x = (i - (M - 1) / 2.0) * d;
+    const double y = (j - (M - 1) / 2.0) * d;
+    double s, c;
+    sincos(kz * z + kx * x + ky * y, &s, &c);
+    const double2 ph = make_double2(c, s);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) psi0[v * 9 + k] = cmd(inc.v[k], ph);
+  }
+}
+
+__device__ __forceinline__ void tapply9(const double2* t, const double2* p, double2* tau) {
+  const double2 sum = cad(cad(p[3], p[4]), p[5]);
+  const double2 dmo = make_double2(t[1].x - t[2].x, t[1].y - t[2].y);
+  const double2 tos = cmd(t[2], sum);
+  for (int c = 0; c < 3; ++c) tau[c] = cmd(t[0], p[c]);
+  for (int c = 3; c < 6; ++c) tau[c] = cad(cmd(dmo, p[c]), tos);
+  for (int c = 6; c < 9; ++c) tau[c] = cmd(t[3], p[c]);
+}
+
+// Sums of tau = T0 psi: per layer (Nz x 9) and, when kP/kS given, the phased
+// sums sum_v exp(-i k r_hat . x_v) tau_v for k in {kP, kS} (2 x 9).  One
+// workgroup per layer; the phased sums are per-layer partials.
+__global__ __launch_bounds__(256) void k_source_sums(const double2* __restrict__ psi,
+                                                     const double2* __restrict__ tp, int M, int Nz,
+                                                     double d, double kP, double kS, double rz,
+                                                     double rx, double ry,
+                                                     double2* __restrict__ layer_sum,
+                                                     double2* __restrict__ phased) {
+  __shared__ double2 red[27][4];
+  const int l = blockIdx.x;
+  double2 s[27];
+  for (int k = 0; k < 27; ++k) s[k] = zc();
+  const size_t base = (size_t)l * M * M;
+  for (int v = threadIdx.x; v < M * M; v += blockDim.x) {
+    const size_t g = base + v;
+    double2 tau[9];
+    tapply9(tp + g * 4, psi + g * 9, tau);
+    const int i = v / M, j = v - (v / M) * M;
+    const double z = (l + 0.5) * d;
+    const double x = (i - (M - 1) / 2.0) * d;
+    const double y = (j - (M - 1) / 2.0) * d;
+    const double rdot = rz * z + rx * x + ry * y;
+    double sp, cp, ss, cs;
+    sincos(-kP * rdot, &sp, &cp);
+    sincos(-kS * rdot, &ss, &cs);
+    const double2 eP = make_double2(cp, sp), eS = make_double2(cs, ss);
+    for (int k = 0; k < 9; ++k) {
+      s[k] = cad(s[k], tau[k]);
+      s[9 + k] = cad(s[9 + k], cmd(eP, tau[k]));
+      s[18 + k] = cad(s[18 + k], cmd(eS, tau[k]));
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int k = 0; k < 27; ++k) {
+    double2 v = s[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      v.x += __shfl_down(v.x, off, 64);
+      v.y += __shfl_down(v.y, off, 64);
+    }
+    if (lane == 0) red[k][wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 27) {
+    double2 t = zc();
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) t = cad(t, red[threadIdx.x][w]);
+    if (threadIdx.x < 9) {
+      layer_sum[(size_t)l * 9 + threadIdx.x] = t;
+    } else {
+      phased[(size_t)l * 18 + threadIdx.x - 9] = t;
+    }
+  }
+}
+
+int grid1d(size_t n, int bs) { return (int)std::min<size_t>((n + bs - 1) / bs, 65536); }
+
+}  // namespace
+
+// ── host entry points ────────────────────────────────────────────────────────
+
+int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st) {
+  const size_t P = (size_t)p.Pz * p.Pxy * p.Pxy;
+  double2* work = nullptr;
+  MSC_HIP_CHECK(hipMalloc(&work, P * kNE * sizeof(double2)));
+  const double kP = omega / alpha, kS = omega / beta;
+  const double pre = 1.0 / (4.0 * M_PI * rho * omega * omega);
+  hipLaunchKernelGGL(k_g0_spatial, dim3(grid1d(P, 128)), dim3(128), 0, st, work, p.Pz, p.Pxy,
+                     p.Nz, p.M, p.periodic ? 1 : 0, 2.0 * p.a, kP, kS, pre);
+  int rc = kOk;
+  const int P2 = p.Pxy;
+  // along y: [Pz*Pxy][Pxy][45]; along x: [Pz][Pxy][Pxy*45]; along z: [1][Pz][Pxy*Pxy*45]
+  rc = fft_axis<double>(work, work, p.Pz * P2, P2, kNE, false, p.tw128, st);
+  if (!rc) rc = fft_axis<double>(work, work, p.Pz, P2, (size_t)P2 * kNE, false, p.tw128, st);
+  if (!rc) rc = fft_axis<double>(work, work, 1, p.Pz, (size_t)P2 * P2 * kNE, false, p.tw128, st);
+  if (!rc) {
+    const size_t tot = (size_t)p.NUV * kNE * p.NC;
+    hipLaunchKernelGGL(k_table_compress, dim3(grid1d(tot, 256)), dim3(256), 0, st, work, p.Pz,
+                       p.Pxy, p.NC, p.NUV, 1.0 / (double)P, p.table128, p.table64);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error(std::string("table compress launch: ") + hipGetErrorString(e));
+      rc = kHip;
+    }
+  }
+  hipError_t e1 = hipStreamSynchronize(st);
+  (void)hipFree(work);
+  if (rc) return rc;
+  if (e1 != hipSuccess) {
+    set_error(std::string("G0 table build: ") + hipGetErrorString(e1));
+    return kHip;
+  }
+  p.omega = omega;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.rho = rho;
+  p.have_table = true;
+  return kOk;
+}
+
+int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, const double* Dr,
+                               double omega, const double* coef8, hipStream_t st) {
+  const double2 G0 = make_double2(coef8[0], coef8[1]);
+  const double2 A = make_double2(coef8[2], coef8[3]);
+  const double2 B = make_double2(coef8[4], coef8[5]);
+  const double2 C = make_double2(coef8[6], coef8[7]);
+  const double V = 8.0 * p.a * p.a * p.a;
+  hipLaunchKernelGGL(k_tparams, dim3(grid1d(p.nvox, 256)), dim3(256), 0, st, Dl, Dm, Dr, p.nvox,
+                     G0, A, B, C, omega, V, p.tp128, p.tp64);
+  MSC_LAUNCH_CHECK();
+  p.have_tparams = true;
+  return kOk;
+}
+
+int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st) {
+  MSC_HIP_CHECK(hipMemcpyAsync(p.tp128, tp, p.nvox * 4 * sizeof(double2), hipMemcpyDeviceToDevice,
+                               st));
+  hipLaunchKernelGGL(k_narrow4, dim3(grid1d(p.nvox * 4, 256)), dim3(256), 0, st, p.tp128,
+                     p.nvox * 4, p.tp64);
+  MSC_LAUNCH_CHECK();
+  p.have_tparams = true;
+  return kOk;
+}
+
+int incident_field(Plan& p, double k_mag, const double* khat3, const double* inc18, double2* psi0,
+                   hipStream_t st) {
+  Inc9 inc;
+  for (int k = 0; k < 9; ++k) inc.v[k] = make_double2(inc18[2 * k], inc18[2 * k + 1]);
+  hipLaunchKernelGGL(k_incident, dim3(grid1d(p.nvox, 256)), dim3(256), 0, st, psi0, p.M, p.Nz,
+                     2.0 * p.a, k_mag * khat3[0], k_mag * khat3[1], k_mag * khat3[2], inc);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+int source_sums(Plan& p, const double2* psi, double kP, double kS, const double* rhat3,
+                double2* layer_sum, double2* phased, hipStream_t st) {
+  hipLaunchKernelGGL(k_source_sums, dim3(p.Nz), dim3(256), 0, st, psi, p.tp128, p.M, p.Nz,
+                     2.0 * p.a, kP, kS, rhat3[0], rhat3[1], rhat3[2], layer_sum, phased);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace msc

@@ -1,0 +1,16 @@
+// Device-side operator assembly entry points (implemented in msc_setup.hip).
+#pragma once
+
+#include "msc_plan.hpp"
+
+namespace msc {
+
+int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, const double* Dr,
+                               double omega, const double* coef8, hipStream_t st);
+int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st);
+int incident_field(Plan& p, double k_mag, const double* khat3, const double* inc18, double2* psi0,
+                   hipStream_t st);
+int source_sums(Plan& p, const double2* psi, double kP, double kS, const double* rhat3,
+                double2* layer_sum, double2* phased, hipStream_t st);
+
+}  // namespace msc

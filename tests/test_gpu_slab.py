@@ -1,0 +1,302 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the complex128 oracle.
+
+Tolerances: complex128 operator and solve 1e-10 relative (written per test);
+complex64 operator 1e-5; mixed-precision solve 1e-6 relative to the FP64 one,
+with the same FP64 residual target.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import slab as osl
+from oracle import tmatrix as otm
+
+pytestmark = pytest.mark.gpu
+
+REF_T = otm.Medium(5000.0, 3000.0, 2500.0)
+OMEGA_T = 150.0  # ka = 0.05 with a = 1 (reference test constants)
+REF_C = otm.Medium(3000.0, 1732.0, 2500.0)
+
+
+def _mods():
+    import multiplescatteringcalculations_amd as msc
+    from multiplescatteringcalculations_amd import slab_scattering as ss
+
+    return msc, ss
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _iid(M, Nz, ref, phi, c, seed):
+    rng = np.random.default_rng(seed)
+    shape = (Nz, M, M)
+    mask = rng.random(shape) < phi
+    lam = ref.rho * (ref.alpha**2 - 2 * ref.beta**2)
+    mu = ref.rho * ref.beta**2
+    Dl = np.where(mask, rng.uniform(-c, c, shape) * lam, 0.0)
+    Dm = np.where(mask, rng.uniform(-c, c, shape) * mu, 0.0)
+    Dr = np.where(mask, rng.uniform(-c, c, shape) * ref.rho, 0.0)
+    return Dl, Dm, Dr
+
+
+def _plan(M, Nz, a, ref, omega, Dl, Dm, Dr, periodic=False, pad=None):
+    msc, ss = _mods()
+    g = msc.SlabGeometry(M=M, N_z=Nz, a=a)
+    r = msc.ReferenceMedium(ref.alpha, ref.beta, ref.rho)
+    mat = msc.SlabMaterial(Dlambda=Dl, Dmu=Dm, Drho=Dr, ref=r)
+    plan = ss.SlabPlan(g, periodic=periodic, pad=pad)
+    plan.build_g0(omega, r)
+    plan.set_material(mat, omega)
+    return plan, g, r, mat
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_fft_axis_matches_numpy(hip_device, prec):
+    from multiplescatteringcalculations_amd import _lib
+
+    lib = _lib.load()
+    dt = torch.complex128 if prec == "c128" else torch.complex64
+    tol = 1e-13 if prec == "c128" else 3e-6
+    rng = np.random.default_rng(1)
+    for n in [2, 4, 8, 16, 32, 64, 128, 256, 512]:
+        for outer, inner in [(3, 1), (2, 9), (1, 45)]:
+            x = rng.standard_normal((outer, n, inner)) + 1j * rng.standard_normal((outer, n, inner))
+            t = torch.as_tensor(x).to(hip_device, dt)
+            out = torch.empty_like(t)
+            for inv in (0, 1):
+                _lib.check(lib.msc_fft_axis(_lib.ptr(t), _lib.ptr(out), outer, n, inner, inv,
+                                            0 if prec == "c128" else 1, _lib.stream_ptr()))
+                ref = np.fft.fft(x, axis=1) if not inv else np.fft.ifft(x, axis=1) * n
+                assert _rel(out.cpu().numpy(), ref) < tol, (n, outer, inner, inv)
+
+
+def test_tparams_match_oracle(hip_device):
+    M, Nz = 4, 3
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.6, 0.4, seed=3)
+    plan, g, r, mat = _plan(M, Nz, 1.0, REF_C, 346.4, Dl, Dm, Dr)
+    T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, 346.4, REF_C)
+    tp = plan.tparams().cpu().numpy()
+    assert _rel(tp, osl.tparams_from_T(T)) < 1e-12
+
+
+def test_cube_tmatrix_host_matches_oracle():
+    msc, _ = _mods()
+    r = msc.ReferenceMedium(5.0, 3.0, 2.5)
+    res = msc.compute_cube_tmatrix(2 * np.pi * 10, 0.01, r, msc.MaterialContrast(1.25, 0.75, 0.3))
+    o = otm.cube_tmatrix(2 * np.pi * 10, 0.01, otm.Medium(5.0, 3.0, 2.5), 1.25, 0.75, 0.3)
+    for k in ("Gamma0", "Ac", "Bc", "Cc", "T1c", "T2c", "T3c", "Drho_star", "Dlambda_star",
+              "Dmu_star_off", "Dmu_star_diag"):
+        assert abs(getattr(res, k) - getattr(o, k)) <= 1e-12 * abs(getattr(o, k)), k
+
+
+@pytest.mark.parametrize("wave,khat", [("P", [1.0, 0.0, 0.0]), ("P", [0.893, 0.45, 0.0]),
+                                       ("S", [0.8, 0.36, 0.48]), ("S", [1.0, 0.0, 0.0])])
+def test_incident_matches_oracle(hip_device, wave, khat):
+    _, ss = _mods()
+    M, Nz = 5, 3
+    g = ss.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    plan = ss.SlabPlan(g)
+    r = ss.ReferenceMedium(REF_C.alpha, REF_C.beta, REF_C.rho)
+    psi0 = plan.incident_field(692.8, r, khat, wave).cpu().numpy()
+    ref = osl.incident_field(M, Nz, 1.0, 692.8, REF_C, khat, wave)
+    assert _rel(psi0, ref) < 1e-13
+
+
+@pytest.mark.parametrize("M,Nz,contrast,pad", [
+    (2, 2, "uniform", None),
+    (3, 2, "iid", None),
+    (3, 2, "iid", (8, 16)),
+    (5, 3, "iid", None),
+    (13, 5, "iid", None),
+])
+def test_matvec_matches_oracle(hip_device, M, Nz, contrast, pad):
+    if contrast == "uniform":
+        Dl = np.full((Nz, M, M), 2e9)
+        Dm = np.full((Nz, M, M), 1e9)
+        Dr = np.full((Nz, M, M), 100.0)
+        ref, om = REF_T, OMEGA_T
+    else:
+        ref, om = REF_C, 692.8
+        Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.4, seed=M * 10 + Nz)
+    plan, *_ = _plan(M, Nz, 1.0, ref, om, Dl, Dm, Dr, pad=pad)
+    T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, om, ref)
+    kh = osl.slab_kernels(M, Nz, 1.0, om, ref)
+    rng = np.random.default_rng(123)
+    n = Nz * M * M * 9
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y_ref = osl.slab_matvec(x, T, kh, M, Nz)
+    y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    # compare the scattered part (y - x) too, which carries all of G0 T0
+    assert _rel(y, y_ref) < 1e-12
+    assert _rel(x - y, x - y_ref) < 1e-11
+    if M * M * Nz <= 18:
+        y_dir = osl.direct_matvec(x, T, M, Nz, 1.0, om, ref)
+        assert _rel(x - y, x - y_dir) < 1e-11
+    # complex64 operator
+    y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
+    assert _rel(x - y64, x - y_ref) < 2e-5
+
+
+@pytest.mark.parametrize("M,Nz", [(4, 2), (8, 3)])
+def test_matvec_periodic_matches_oracle(hip_device, M, Nz):
+    ref, om = REF_C, 692.8
+    Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=7)
+    plan, *_ = _plan(M, Nz, 1.0, ref, om, Dl, Dm, Dr, periodic=True)
+    T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, om, ref)
+    kh = osl.slab_kernels(M, Nz, 1.0, om, ref, periodic=True)
+    rng = np.random.default_rng(5)
+    n = Nz * M * M * 9
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y_ref = osl.slab_matvec(x, T, kh, M, Nz, periodic=True)
+    y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    assert _rel(x - y, x - y_ref) < 1e-11
+
+
+def test_matvec_identity_when_T_zero(hip_device):
+    M, Nz = 6, 3
+    z = np.zeros((Nz, M, M))
+    plan, *_ = _plan(M, Nz, 1.0, REF_T, OMEGA_T, z, z, z)
+    x = torch.randn(Nz * M * M * 9, dtype=torch.complex128, device=hip_device)
+    assert torch.equal(plan.matvec(x), x)
+
+
+def test_residual_is_fused_b_minus_Ax(hip_device):
+    M, Nz = 7, 4
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=11)
+    plan, *_ = _plan(M, Nz, 1.0, REF_C, 692.8, Dl, Dm, Dr)
+    n = Nz * M * M * 9
+    x = torch.randn(n, dtype=torch.complex128, device=hip_device)
+    b = torch.randn(n, dtype=torch.complex128, device=hip_device)
+    r = plan.residual(b, x)
+    assert torch.allclose(r, b - plan.matvec(x), rtol=0, atol=1e-13)
+
+
+def test_gmres_matches_dense_solve(hip_device):
+    M, Nz = 3, 2
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.7, 0.4, seed=21)
+    plan, g, r, mat = _plan(M, Nz, 1.0, REF_C, 692.8, Dl, Dm, Dr)
+    T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, 692.8, REF_C)
+    A = osl.dense_operator(T, M, Nz, 1.0, 692.8, REF_C)
+    psi0 = osl.incident_field(M, Nz, 1.0, 692.8, REF_C, [0.893, 0.45, 0.0], "P").ravel()
+    x_ref = np.linalg.solve(A, psi0)
+    for restart in (None, 5):
+        x, st = plan.gmres(torch.as_tensor(psi0), "born", tol=1e-13, max_iter=200, restart=restart)
+        assert st["converged"] and st["rel_residual"] < 1e-13
+        assert _rel(x.cpu().numpy(), x_ref) < 1e-10
+
+
+def test_solve_matches_oracle_and_outputs(hip_device):
+    """End-to-end: psi, R_PP (periodic formula) and far-field R vs the oracle."""
+    msc, ss = _mods()
+    M, Nz = 8, 4
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=0)
+    om = 0.4 * REF_C.beta / 1.0
+    p = 1.5e-4
+    kh = [np.sqrt(1 - (p * REF_C.alpha) ** 2), p * REF_C.alpha, 0.0]
+    x_o, psi0_o, T_o, _, res_o = osl.solve(M, Nz, 1.0, Dl, Dm, Dr, om, REF_C, kh, "P", tol=1e-13,
+                                           max_iter=400)
+    assert res_o < 1e-12
+    g = msc.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    r = msc.ReferenceMedium(REF_C.alpha, REF_C.beta, REF_C.rho)
+    mat = msc.SlabMaterial(Dlambda=Dl, Dmu=Dm, Drho=Dr, ref=r)
+    res = msc.compute_slab_scattering_gpu(g, mat, om, np.array(kh), "P", gmres_tol=1e-13,
+                                          max_iter=400, restart=30)
+    assert res.gmres_residual < 1e-13
+    assert _rel(res.psi0, psi0_o) < 1e-13
+    assert _rel(res.psi, x_o) < 1e-10
+    rpp = msc.slab_rpp_periodic(res, p=p)
+    rpp_o = osl.rpp_periodic(x_o, T_o, M, Nz, 1.0, om, REF_C, p=p)
+    assert abs(rpp - rpp_o) <= 1e-10 * abs(rpp_o)
+    ff = msc.slab_reflected_field(res)
+    ff_o = osl.reflected_field(x_o, T_o, M, Nz, 1.0, om, REF_C, kh)
+    for a, b in zip(ff, ff_o):
+        assert abs(a - b) <= 1e-10 * max(abs(b), 1e-300)
+    # explicit T_local path of the outputs (reference signature)
+    T_loc = msc.compute_slab_tmatrices(g, mat, om)
+    assert _rel(T_loc, T_o) < 1e-12
+    assert abs(msc.slab_rpp_periodic(res, T_loc, p=p) - rpp_o) <= 1e-10 * abs(rpp_o)
+
+
+def test_mixed_precision_solve(hip_device):
+    msc, ss = _mods()
+    M, Nz = 16, 8
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=4)
+    om = 0.4 * REF_C.beta
+    g = msc.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    r = msc.ReferenceMedium(REF_C.alpha, REF_C.beta, REF_C.rho)
+    mat = msc.SlabMaterial(Dlambda=Dl, Dmu=Dm, Drho=Dr, ref=r)
+    kh = np.array([0.893, 0.45, 0.0])
+    r64 = msc.compute_slab_scattering_gpu(g, mat, om, kh, gmres_tol=1e-12, restart=30)
+    rmx = msc.compute_slab_scattering_gpu(g, mat, om, kh, gmres_tol=1e-8, restart=30,
+                                          precision="mixed")
+    assert rmx.gmres_residual <= 1e-8 and r64.gmres_residual <= 1e-12
+    assert rmx.stats["matvecs_c64"] > 0
+    assert _rel(rmx.psi, r64.psi) < 1e-6
+    assert abs(msc.slab_rpp_periodic(rmx) - msc.slab_rpp_periodic(r64)) <= 1e-6 * abs(
+        msc.slab_rpp_periodic(r64))
+
+
+def test_zero_contrast_gives_incident_field(hip_device):
+    msc, _ = _mods()
+    g = msc.SlabGeometry(M=4, N_z=2, a=1.0)
+    r = msc.ReferenceMedium(5000.0, 3000.0, 2500.0)
+    mat = msc.uniform_slab_material(g, r, msc.MaterialContrast(0.0, 0.0, 0.0))
+    res = msc.compute_slab_scattering(g, mat, OMEGA_T, np.array([1.0, 0.0, 0.0]))
+    assert np.allclose(res.psi, res.psi0, atol=1e-12)
+    assert abs(msc.slab_rpp_periodic(res)) < 1e-20
+    R_PP, R_PS, _ = msc.slab_reflected_field(res)
+    assert abs(R_PP) < 1e-20 and abs(R_PS) < 1e-20
+
+
+def test_nonconvergence_warns(hip_device):
+    msc, _ = _mods()
+    g = msc.SlabGeometry(M=4, N_z=2, a=1.0)
+    r = msc.ReferenceMedium(5000.0, 3000.0, 2500.0)
+    mat = msc.uniform_slab_material(g, r, msc.MaterialContrast(2e9, 1e9, 100.0))
+    with pytest.warns(UserWarning, match="did not converge"):
+        res = msc.compute_slab_scattering_gpu(g, mat, OMEGA_T, np.array([1.0, 0.0, 0.0]),
+                                              gmres_tol=1e-15, max_iter=1)
+    assert res.n_gmres_iter == 1 and res.gmres_residual > 1e-15
+
+
+def test_torch_gmres_dense(hip_device):
+    msc, _ = _mods()
+    rng = np.random.default_rng(42)
+    n = 20
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A @ A.conj().T + 5 * np.eye(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    At = torch.as_tensor(A).to(hip_device)
+    bt = torch.as_tensor(b).to(hip_device)
+    x, it, rr = msc.torch_gmres(lambda v: At @ v, bt, tol=1e-12, maxiter=50)
+    assert rr < 1e-12
+    assert _rel(x.cpu().numpy(), np.linalg.solve(A, b)) < 1e-9
+    d = torch.tensor([1.0, 2.0, 3.0], dtype=torch.complex128, device=hip_device)
+    bb = torch.tensor([1.0, 4.0, 9.0], dtype=torch.complex128, device=hip_device)
+    _, it0, rr0 = msc.torch_gmres(lambda v: d * v, bb, x0=bb / d, tol=1e-10)
+    assert it0 == 0 and rr0 < 1e-10
+
+
+def test_full_size_operator_properties(hip_device):
+    """128 x 128 x 32 slab (Pxy 256, Pz 64): linearity, c64 vs c128, T=0 identity."""
+    _, ss = _mods()
+    M, Nz = 128, 32
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=0)
+    plan, *_ = _plan(M, Nz, 1.0, REF_C, 0.4 * REF_C.beta, Dl, Dm, Dr)
+    assert (plan.Pz, plan.Pxy) == (64, 256)
+    n = plan.n
+    gen = torch.Generator(device=hip_device).manual_seed(0)
+    x = torch.randn(n, dtype=torch.complex128, device=hip_device, generator=gen)
+    y = torch.randn(n, dtype=torch.complex128, device=hip_device, generator=gen)
+    a, b = 0.7 - 0.2j, -1.3 + 0.5j
+    lhs = plan.matvec(a * x + b * y)
+    rhs = a * plan.matvec(x) + b * plan.matvec(y)
+    assert float((lhs - rhs).abs().max() / rhs.abs().max()) < 1e-13
+    y128 = plan.matvec(x)
+    y64 = plan.matvec(x.to(torch.complex64)).to(torch.complex128)
+    assert float(((x - y64) - (x - y128)).norm() / (x - y128).norm()) < 1e-5
