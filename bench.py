@@ -1,0 +1,198 @@
+"""Benchmark: seconds per solve to a 1e-8 FP64 relative residual on the
+128 x 128 x 32 slab (padded FFT grid 64 x 256 x 256), one MI355X.
+
+Workload (synthetic, seeded): background Vp 3000 m/s, Vs 1732 m/s, rho 2500
+kg/m^3; cube half-width a = 1 m; Bernoulli(0.3) inclusions with i.i.d.
+relative contrasts U[-0.2, 0.2] in lambda, mu, rho; ka = k_S a = 0.4; oblique
+P plane wave with horizontal slowness p = 0.15 s/km.  One "step" is one full
+GMRES(30) solve of (I - G0 T0) psi = psi0 from the Born guess to
+||b - A psi|| / ||b|| <= 1e-8 in FP64, with the operator (G0 table, T0) and
+psi0 already resident in HBM.
+
+The headline (``value``) is the mixed-precision solve (complex64 operator and
+Krylov basis inside FP64 iterative refinement); the plain FP64 GMRES(30)
+solve of the same system is timed the same way and reported beside it.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+M, NZ, A = 128, 32, 1.0
+VP, VS, RHO = 3000.0, 1732.0, 2500.0
+PHI, CONTRAST, KA, P_SLOW = 0.3, 0.2, 0.4, 1.5e-4
+TOL, RESTART = 1e-8, 30
+
+
+def setup(device):
+    import multiplescatteringcalculations_amd as msc
+    from multiplescatteringcalculations_amd.slab_scattering import SlabPlan
+
+    ref = msc.ReferenceMedium(VP, VS, RHO)
+    g = msc.SlabGeometry(M=M, N_z=NZ, a=A)
+    mat = msc.random_iid_slab_material(g, ref, PHI, CONTRAST, seed=0)
+    omega = KA * VS / A
+    khat = np.array([np.sqrt(1.0 - (P_SLOW * VP) ** 2), P_SLOW * VP, 0.0])
+    plan = SlabPlan(g, device=device)
+    plan.build_g0(omega, ref)
+    plan.set_material(mat, omega)
+    psi0 = plan.incident_field(omega, ref, khat, "P").reshape(-1)
+    return plan, psi0, mat, omega
+
+
+def time_solves(plan, psi0, precision, steps, warmup):
+    import torch
+
+    st = None
+    for _ in range(warmup):
+        _, st = plan.gmres(psi0, "born", tol=TOL, max_iter=600, restart=RESTART, precision=precision)
+    torch.cuda.synchronize()
+    times = []
+    plan.timing(True)
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, st = plan.gmres(psi0, "born", tol=TOL, max_iter=600, restart=RESTART, precision=precision)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"  {precision}: {times[-1] * 1e3:.2f} ms  iters={st['iterations']} "
+            f"cycles={st['cycles']} rel_res={st['rel_residual']:.2e}")
+    tim = plan.timing()
+    plan.timing(False)
+    return times, st, x, tim
+
+
+def cpu_baseline(mat, omega, n_matvec: int):
+    """The oracle's restatement of the reference CPU matvec (slab_scattering.py:
+    359-418: per-layer 2-D FFTs and the N_z^2 layer-pair 9x9 loop) timed on one
+    matvec of this workload and scaled by the plain solve's operator count.
+    The spectral table is a synthetic complex array of the reference's shape
+    (2N_z-1, 2M-1, 2M-1, 9, 9): the matvec's time does not depend on its values,
+    and building the real one takes minutes of Python loops."""
+    from oracle import slab as osl
+    from oracle import tmatrix as otm
+
+    rng = np.random.default_rng(1)
+    H = 2 * M - 1
+    kh = np.full((2 * NZ - 1, H, H, 9, 9), 1e-12 + 1e-13j, dtype=np.complex128)
+    ref = otm.Medium(VP, VS, RHO)
+    # per-voxel T0 through the oracle's closed form (one evaluation per inclusion)
+    T = np.zeros((NZ, M, M, 9, 9), dtype=complex)
+    idx = np.argwhere(mat.Drho != 0)
+    for (l, i, j) in idx[:64]:
+        T[l, i, j] = otm.tmatrix_9x9(otm.cube_tmatrix(omega, A, ref, mat.Dlambda[l, i, j],
+                                                      mat.Dmu[l, i, j], mat.Drho[l, i, j]), omega, A)
+    x = (rng.standard_normal(NZ * M * M * 9) + 1j * rng.standard_normal(NZ * M * M * 9))
+    t0 = time.perf_counter()
+    osl.slab_matvec(x, T, kh, M, NZ)
+    t_mv = time.perf_counter() - t0
+    return t_mv, t_mv * n_matvec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: independent (frequency, slowness, slab) solves shard "
+              "across GPUs with no communication, so run one process per GPU; --gpus > 1 is "
+              "not measured by this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        log("no HIP device")
+        sys.exit(1)
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    plan, psi0, mat, omega = setup(dev)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    log(f"setup (plan, G0 table, T0, psi0): {t_setup:.2f} s")
+
+    log("plain FP64 GMRES(30):")
+    t64, st64, x64, tim64 = time_solves(plan, psi0, "fp64", args.steps, args.warmup)
+    log("mixed (complex64 cycles in FP64 refinement):")
+    tmx, stmx, xmx, timmx = time_solves(plan, psi0, "mixed", args.steps, args.warmup)
+    diff = float((xmx - x64).abs().max() / x64.abs().max())
+
+    # roofline of the block-multiply pass (pass 3) in the headline (complex64) solves
+    n64, ms64 = timmx["c64"]
+    nuv, nc = plan.NUV, plan.NC
+    s = 8  # bytes per complex64
+    spectra = NZ * plan.Pxy * plan.Pxy * 9 * s  # read and written once
+    table = nuv * 45 * nc * s
+    bytes_pass3 = 2 * spectra + table
+    avg_ms = ms64 / max(n64, 1)
+    achieved = bytes_pass3 / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "kernel": "k_z_mult<float,64,4> (z FFT + 9x9 block multiply + inverse z FFT)",
+                "launches": n64, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_pass3}
+
+    mean_mx = float(np.mean(tmx))
+    mean_64 = float(np.mean(t64))
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle restatement of the reference CPU matvec, 1 matvec) ...")
+        n_mv = st64["matvecs_c128"]
+        t_mv, t_solve = cpu_baseline(mat, omega, n_mv)
+        cpu = {"value": round(t_solve, 2), "unit": "s/solve", "cores": 1, "kind": "port",
+               "sample": (f"1 CPU matvec of the 128x128x32 slab ({t_mv:.2f} s: numpy 2-D FFTs + "
+                          f"N_z^2 layer-pair einsum loop, synthetic table) x {n_mv} operator "
+                          f"applications of the plain FP64 GPU solve; setup excluded; host "
+                          f"{platform.processor() or platform.machine()}, {os.cpu_count()} cpus "
+                          f"visible, 1 used")}
+        log(f"  cpu matvec {t_mv:.2f} s -> {t_solve:.1f} s/solve (extrapolated)")
+
+    out = {
+        "metric": "s/solve to 1e-8 residual, 128\u00d7128\u00d732 slab: learned+mixed vs plain FP64 GMRES",
+        "value": round(mean_mx, 6),
+        "unit": "s/solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mean_mx * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+c128",
+        "data": "synthetic (seeded random slab, Bernoulli(0.3) inclusions, U[-0.2,0.2] contrasts)",
+        "config": {"workload": "slab 128x128x32, pad 64x256x256, ka=0.4, p=0.15 s/km, GMRES(30), "
+                               "tol 1e-8 FP64; mixed precision (no learned preconditioner yet)",
+                   "M": M, "N_z": NZ, "restart": RESTART, "tol": TOL, "precision": "mixed"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": {
+            "fp64_s_per_solve": round(mean_64, 6),
+            "speedup_vs_fp64": round(mean_64 / mean_mx, 3),
+            "fp64_iterations": st64["iterations"], "mixed_iterations": stmx["iterations"],
+            "mixed_matvecs_c64": stmx["matvecs_c64"], "mixed_matvecs_c128": stmx["matvecs_c128"],
+            "fp64_rel_residual": st64["rel_residual"], "mixed_rel_residual": stmx["rel_residual"],
+            "mixed_vs_fp64_max_rel_diff": diff, "setup_s": round(t_setup, 3),
+            "pass3_c128_avg_us": round(tim64["c128"][1] / max(tim64["c128"][0], 1) * 1e3, 2),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

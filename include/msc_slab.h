@@ -197,10 +197,11 @@ int msc_arnoldi_solution(msc_arnoldi* ctx, const void* V, long long ldv, void* x
 /*
  * Kernel timing with HIP events recorded around every launch of the
  * block-multiply pass (pass 3) on the launch stream: enable resets the
- * counters; read returns (launches, total milliseconds).
+ * counters; read returns (complex64 launches, their total ms, complex128
+ * launches, their total ms).
  */
 int msc_timing_enable(msc_plan* plan, int enable);
-int msc_timing_read(msc_plan* plan, double* out2);
+int msc_timing_read(msc_plan* plan, double* out4);
 
 #ifdef __cplusplus
 }

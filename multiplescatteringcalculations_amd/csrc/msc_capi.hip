@@ -630,20 +630,27 @@ int msc_timing_enable(msc_plan* plan, int enable) {
   return kOk;
 }
 
-int msc_timing_read(msc_plan* plan, double* out2) {
-  MSC_REQUIRE(plan != nullptr && out2 != nullptr, "NULL argument");
-  double total = 0.0;
-  size_t nl = 0;
+int msc_timing_read(msc_plan* plan, double* out4) {
+  MSC_REQUIRE(plan != nullptr && out4 != nullptr, "NULL argument");
+  double ms64 = 0.0, ms128 = 0.0;
+  size_t n64 = 0, n128 = 0;
   const Plan& p = plan->p;
   for (size_t i = 0; i + 1 < p.ev_used; i += 2) {
     MSC_HIP_CHECK(hipEventSynchronize(p.ev_pool[i + 1]));
     float ms = 0.0f;
     MSC_HIP_CHECK(hipEventElapsedTime(&ms, p.ev_pool[i], p.ev_pool[i + 1]));
-    total += ms;
-    ++nl;
+    if (p.ev_prec[i / 2] == 8) {
+      ms64 += ms;
+      ++n64;
+    } else {
+      ms128 += ms;
+      ++n128;
+    }
   }
-  out2[0] = (double)nl;
-  out2[1] = total;
+  out4[0] = (double)n64;
+  out4[1] = ms64;
+  out4[2] = (double)n128;
+  out4[3] = ms128;
   return kOk;
 }
 

@@ -358,7 +358,7 @@ static int launch_xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out,
   return kOk;
 }
 
-int timing_begin(Plan& p, hipStream_t st, size_t* slot) {
+int timing_begin(Plan& p, hipStream_t st, size_t* slot, int elem_bytes) {
   *slot = SIZE_MAX;
   if (!p.timing) return kOk;
   if (p.ev_used + 2 > p.ev_pool.size()) {
@@ -367,7 +367,9 @@ int timing_begin(Plan& p, hipStream_t st, size_t* slot) {
       MSC_HIP_CHECK(hipEventCreate(&e));
       p.ev_pool.push_back(e);
     }
+    p.ev_prec.resize(p.ev_pool.size() / 2);
   }
+  p.ev_prec[p.ev_used / 2] = elem_bytes;
   MSC_HIP_CHECK(hipEventRecord(p.ev_pool[p.ev_used], st));
   *slot = p.ev_used;
   p.ev_used += 2;
@@ -383,7 +385,7 @@ int timing_end(Plan& p, hipStream_t st, size_t slot) {
 template <typename R, int PZ>
 static int launch_z(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
   size_t slot;
-  int rc = timing_begin(p, st, &slot);
+  int rc = timing_begin(p, st, &slot, (int)sizeof(cx<R>));
   if (rc) return rc;
   constexpr int Q0 = ZCols<R, PZ>::value;
   const int Q = (p.Pxy % Q0 == 0) ? Q0 : 1;

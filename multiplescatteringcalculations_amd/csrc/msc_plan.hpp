@@ -47,11 +47,12 @@ struct Plan {
   // HIP-event timing of the block-multiply pass (pass 3), one pair per launch
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_prec;  // 8 (complex64) or 16 (complex128) per event pair
   size_t ev_used = 0;
 };
 
 // Event pair around one pass-3 launch when p.timing is set.
-int timing_begin(Plan& p, hipStream_t st, size_t* slot);
+int timing_begin(Plan& p, hipStream_t st, size_t* slot, int elem_bytes);
 int timing_end(Plan& p, hipStream_t st, size_t slot);
 
 // Matvec y = (I - G0 T0) x in precision R on the caller's stream.  `done`

@@ -303,14 +303,15 @@ class SlabPlan:
                                                 _lib.ptr(ls), _lib.ptr(ph), self.stream))
         return ls.cpu().numpy(), ph.cpu().numpy()
 
-    def timing(self, enable: bool | None = None) -> tuple[int, float]:
-        """Enable/disable pass-3 event timing, or read (launches, total ms)."""
+    def timing(self, enable: bool | None = None) -> dict:
+        """Enable/disable (and reset) the HIP-event timing of the block-multiply
+        pass, or read it: {"c64": (launches, ms), "c128": (launches, ms)}."""
         if enable is not None:
             _lib.check(self.lib.msc_timing_enable(self._h, int(bool(enable))))
-            return (0, 0.0)
-        out = (ctypes.c_double * 2)()
+            return {}
+        out = (ctypes.c_double * 4)()
         _lib.check(self.lib.msc_timing_read(self._h, out))
-        return int(out[0]), float(out[1])
+        return {"c64": (int(out[0]), float(out[1])), "c128": (int(out[2]), float(out[3]))}
 
 
 # ═══════════════════════════════════════════════════════════════
