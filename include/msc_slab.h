@@ -95,6 +95,22 @@ int msc_plan_destroy(msc_plan* plan);
 int msc_plan_info(const msc_plan* plan, long long* out6);
 
 /*
+ * Tile autotuning: time every tile candidate of the five FFT passes in place
+ * for one precision and keep the fastest (needs msc_build_g0 and
+ * msc_set_tmatrix first).  msc_gmres tunes each precision it uses once per
+ * plan unless the environment sets MSC_AUTOTUNE=0.  tiles: out10 = for
+ * complex128 then complex64, (y-forward rows, x-forward columns, z columns,
+ * x-inverse columns, y-inverse rows) per workgroup.  No counterpart in the
+ * reference (its FFTs are library calls).
+ */
+int msc_plan_autotune(msc_plan* plan, int precision, void* stream);
+int msc_plan_tiles(const msc_plan* plan, int* out10);
+/* Set the tiles explicitly (same layout as msc_plan_tiles; marks both
+ * precisions tuned).  A tile larger than the LDS allows for the plan's FFT
+ * lengths falls back to the next smaller one. */
+int msc_plan_set_tiles(msc_plan* plan, const int* tiles10);
+
+/*
  * Frequency constants of the Rayleigh cube T-matrix on the host:
  * out8 = (Gamma0, A^c, B^c, C^c) as (re, im) pairs.
  * Replaces _compute_Gamma0_analytical and _compute_ABC_polynomial

@@ -31,6 +31,9 @@ SIGNATURES = {
     "msc_plan_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_int, c_int, c_int]),
     "msc_plan_destroy": (c_int, [c_void_p]),
     "msc_plan_info": (c_int, [c_void_p, POINTER(c_longlong)]),
+    "msc_plan_autotune": (c_int, [c_void_p, c_int, c_void_p]),
+    "msc_plan_tiles": (c_int, [c_void_p, POINTER(c_int)]),
+    "msc_plan_set_tiles": (c_int, [c_void_p, POINTER(c_int)]),
     "msc_cube_coefficients": (c_int, [c_double, c_double, c_double, c_double, c_double,
                                       POINTER(c_double)]),
     "msc_build_g0": (c_int, [c_void_p, c_double, c_double, c_double, c_double, c_int, c_int,

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -153,6 +154,10 @@ struct msc_plan {
   Krylov k128;
   Krylov k64;
   double2* r128 = nullptr;  // residual buffer
+  float2* b64 = nullptr;    // right-hand side narrowed to complex64
+  float2* r64 = nullptr;    // complex64 residual
+  hipStream_t cap = nullptr;  // private stream for HIP graph capture
+  bool use_graphs = true;
 };
 
 struct msc_arnoldi {
@@ -163,6 +168,11 @@ struct msc_arnoldi {
 namespace {
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+bool autotune_enabled() {
+  const char* v = std::getenv("MSC_AUTOTUNE");
+  return v == nullptr || std::strcmp(v, "0") != 0;
+}
 
 int alloc_plan_buffers(Plan& p) {
   const size_t b1 = (size_t)p.Nz * p.M * p.Pxy * 9;
@@ -206,25 +216,100 @@ int check_opts(const msc_gmres_opts* o) {
 
 namespace {
 
-// One cycle's Arnoldi steps; checks the device flag every `poll` steps.
+// Launch the work of Arnoldi step j (operator on B = G0 T0 and the
+// orthogonalisation) on stream s.
+template <typename R>
+int step_launches(msc_plan* pl, Krylov& k, int j, int orth, hipStream_t s) {
+  cx<R>* V = reinterpret_cast<cx<R>*>(k.V);
+  int rc = scattered<R>(pl->p, V + (size_t)j * k.ldv, V + (size_t)(j + 1) * k.ldv, s, &k.st->done);
+  if (rc) return rc;
+  return krylov_step<R>(k, j, orth, s);
+}
+
+// The instantiated graph of step j (captured once per Krylov buffer set).
+template <typename R>
+int step_graph(msc_plan* pl, Krylov& k, int j, int orth, hipGraphExec_t* out) {
+  if (k.graph_orth != orth) {
+    k.drop_graphs();
+    k.graph_orth = orth;
+  }
+  if ((int)k.graphs.size() <= j) k.graphs.resize(j + 1, nullptr);
+  if (k.graphs[j] == nullptr) {
+    if (pl->cap == nullptr) MSC_HIP_CHECK(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+    MSC_HIP_CHECK(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeRelaxed));
+    const int rc = step_launches<R>(pl, k, j, orth, pl->cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(pl->cap, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      return kHip;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+      return kHip;
+    }
+    k.graphs[j] = exec;
+  }
+  *out = k.graphs[j];
+  return kOk;
+}
+
+// One cycle's Arnoldi steps.  The host keeps at most two steps queued ahead
+// of the device and stops launching once the device has raised `done` (read
+// through a pinned, device-mapped flag, so no stream synchronisation is
+// needed inside the cycle).  Steps already queued when `done` rises are
+// no-ops.  Pass-3 event timing (msc_timing_enable) disables the graphs so the
+// events bracket the kernel itself.
 template <typename R>
 int run_cycle(msc_plan* pl, Krylov& k, int steps, int orth, hipStream_t st, int* taken) {
-  Plan& p = pl->p;
-  constexpr int poll = 6;
-  KState hs{};
-  for (int j = 0; j < steps; ++j) {
-    cx<R>* V = reinterpret_cast<cx<R>*>(k.V);
-    int rc = matvec<R>(p, V + (size_t)j * k.ldv, V + (size_t)(j + 1) * k.ldv, st, &k.st->done);
-    if (rc) return rc;
-    rc = krylov_step<R>(k, j, orth, st);
-    if (rc) return rc;
-    if ((j + 1) % poll == 0 && j + 1 < steps) {
-      rc = krylov_read_state(k, &hs, st);
-      if (rc) return rc;
-      if (hs.done) break;
+  const bool graphs = pl->use_graphs && !pl->p.timing;
+  volatile int* flag = k.done_flag;
+  hipEvent_t ev[2];
+  MSC_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  MSC_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int rc = kOk;
+  for (int j = 0; j < steps && rc == kOk; ++j) {
+    if (j >= 2) {
+      const hipError_t e = hipEventSynchronize(ev[j & 1]);
+      if (e != hipSuccess) {
+        set_error(std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+        rc = kHip;
+        break;
+      }
+      if (*flag) break;
+    }
+    if (graphs) {
+      hipGraphExec_t exec = nullptr;
+      rc = step_graph<R>(pl, k, j, orth, &exec);
+      if (rc) break;
+      const hipError_t e = hipGraphLaunch(exec, st);
+      if (e != hipSuccess) {
+        set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+        rc = kHip;
+        break;
+      }
+    } else {
+      rc = step_launches<R>(pl, k, j, orth, st);
+      if (rc) break;
+    }
+    const hipError_t e = hipEventRecord(ev[j & 1], st);
+    if (e != hipSuccess) {
+      set_error(std::string("hipEventRecord: ") + hipGetErrorString(e));
+      rc = kHip;
     }
   }
-  int rc = krylov_read_state(k, &hs, st);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc) return rc;
+  KState hs{};
+  rc = krylov_read_state(k, &hs, st);
   if (rc) return rc;
   *taken = hs.j;
   return kOk;
@@ -239,12 +324,31 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
   int rc = pl->k128.ensure(n, m, sizeof(double2));
   if (rc) return rc;
   Krylov& k = pl->k128;
-  if (o.mode == MSC_SOLVE_MIXED) {
+  k.shifted = true;
+  bool mixed = o.mode == MSC_SOLVE_MIXED;
+  if (mixed) {
     rc = pl->k64.ensure(n, m, sizeof(float2));
     if (rc) return rc;
+    pl->k64.shifted = true;
+    if (pl->b64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->b64, n * sizeof(float2)));
+    if (pl->r64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r64, n * sizeof(float2)));
   }
   if (pl->r128 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r128, n * sizeof(double2)));
   double2* r = pl->r128;
+  // tile autotuning once per plan and precision (MSC_AUTOTUNE=0 keeps the defaults)
+  if (autotune_enabled()) {
+    if (!p.tuned[0]) {
+      rc = autotune<double>(p, b, r, st);
+      if (rc) return rc;
+      k.drop_graphs();
+    }
+    if (mixed && !p.tuned[1]) {
+      rc = krylov_narrow(b, pl->b64, n, st);
+      if (!rc) rc = autotune<float>(p, pl->b64, pl->r64, st);
+      if (rc) return rc;
+      pl->k64.drop_graphs();
+    }
+  }
 
   // ||b||
   rc = krylov_norm<double>(k, b, &k.scal[0], st);
@@ -265,29 +369,52 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
     *s = out;
     return kOk;
   }
-  bool mixed = o.mode == MSC_SOLVE_MIXED;
   double prev_rel = INFINITY;
   const double eta_min = o.inner_tol > 0.0 ? o.inner_tol : 1e-4;
+  // Mixed form: the first residual needs no FP64 operator.  From x0 = 0 it is
+  // b itself; from the Born guess x0 = b it is b - A b = G0 T0 b, taken with
+  // the complex64 operator (its rounding only limits what the first complex64
+  // cycle could reach anyway).  Every later residual is an FP64 one, and
+  // convergence is only ever declared on an FP64 residual.
+  bool cheap = mixed && o.x0 != MSC_X0_GIVEN;
   for (;;) {
-    // true FP64 residual of the current iterate
-    rc = residual<double>(p, b, x, r, st, nullptr);
-    if (rc) return rc;
-    ++out.matvecs_c128;
     Krylov& kk = mixed ? pl->k64 : k;
-    rc = krylov_norm<double>(kk, r, &kk.st->rnorm, st);
-    if (rc) return rc;
+    const bool from64 = cheap && o.x0 == MSC_X0_BORN;
+    if (cheap) {
+      if (o.x0 == MSC_X0_ZERO) {
+        MSC_HIP_CHECK(hipMemcpyAsync(&kk.st->rnorm, &k.scal[0], sizeof(double),
+                                     hipMemcpyDeviceToDevice, st));
+      } else {
+        rc = krylov_narrow(b, pl->b64, n, st);
+        if (!rc) rc = scattered<float>(p, pl->b64, pl->r64, st, nullptr);
+        if (!rc) rc = krylov_norm<float>(kk, pl->r64, &kk.st->rnorm, st);
+        if (rc) return rc;
+        ++out.matvecs_c64;
+      }
+    } else {
+      rc = residual<double>(p, b, x, r, st, nullptr);
+      if (rc) return rc;
+      ++out.matvecs_c128;
+      rc = krylov_norm<double>(kk, r, &kk.st->rnorm, st);
+      if (rc) return rc;
+    }
     double rnorm = 0.0;
     MSC_HIP_CHECK(hipMemcpyAsync(&rnorm, &kk.st->rnorm, sizeof(double), hipMemcpyDeviceToHost, st));
     MSC_HIP_CHECK(hipStreamSynchronize(st));
     const double rel = rnorm / bnorm;
-    out.rel_residual = rel;
-    if (!(rel > o.tol) || out.iterations >= o.max_iter) break;
-    if (mixed && out.cycles > 0 && rel > 0.9 * prev_rel) {
-      // a complex64 cycle failed to make progress: finish with FP64 cycles
-      mixed = false;
-      out.fell_back = 1;
-      rc = krylov_norm<double>(k, r, &k.st->rnorm, st);
-      if (rc) return rc;
+    if (cheap) {
+      cheap = false;
+      if (!(rel > o.tol)) continue;  // confirm with an FP64 residual
+    } else {
+      out.rel_residual = rel;
+      if (!(rel > o.tol) || out.iterations >= o.max_iter) break;
+      if (mixed && out.cycles > 0 && rel > 0.9 * prev_rel) {
+        // a complex64 cycle failed to make progress: finish with FP64 cycles
+        mixed = false;
+        out.fell_back = 1;
+        rc = krylov_norm<double>(k, r, &k.st->rnorm, st);
+        if (rc) return rc;
+      }
     }
     prev_rel = rel;
     const int steps = std::min(m, o.max_iter - out.iterations);
@@ -297,7 +424,13 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
       const double need = 0.3 * o.tol / rel;
       const double eta = std::min(std::max(eta_min, need), 0.5);
       Krylov& k6 = pl->k64;
-      rc = krylov_cycle_init<double, float>(k6, r, eta, true, st);
+      *k6.done_flag = 0;
+      if (from64) {
+        rc = krylov_cycle_init<float, float>(k6, pl->r64, eta, true, st);
+      } else {
+        rc = krylov_cycle_init<double, float>(k6, o.x0 == MSC_X0_ZERO && out.cycles == 0 ? b : r,
+                                              eta, true, st);
+      }
       if (rc) return rc;
       rc = run_cycle<float>(pl, k6, steps, o.orth, st, &taken);
       if (rc) return rc;
@@ -305,6 +438,7 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
       if (rc) return rc;
       out.matvecs_c64 += taken;
     } else {
+      *k.done_flag = 0;
       rc = krylov_cycle_init<double, double>(k, r, o.tol * bnorm, false, st);
       if (rc) return rc;
       rc = run_cycle<double>(pl, k, steps, o.orth, st, &taken);
@@ -401,6 +535,9 @@ int msc_plan_destroy(msc_plan* plan) {
   (void)hipDeviceSynchronize();
   free_plan_buffers(plan->p);
   if (plan->r128) (void)hipFree(plan->r128);
+  if (plan->b64) (void)hipFree(plan->b64);
+  if (plan->cap) (void)hipStreamDestroy(plan->cap);
+  if (plan->r64) (void)hipFree(plan->r64);
   for (hipEvent_t e : plan->p.ev_pool) (void)hipEventDestroy(e);
   delete plan;
   return kOk;
@@ -415,6 +552,55 @@ int msc_plan_info(const msc_plan* plan, long long* out6) {
   out6[3] = p.Pxy;
   out6[4] = p.NUV;
   out6[5] = p.NC;
+  return kOk;
+}
+
+int msc_plan_autotune(msc_plan* plan, int precision, void* stream) {
+  MSC_REQUIRE(plan != nullptr, "plan must not be NULL");
+  MSC_REQUIRE(precision == MSC_C128 || precision == MSC_C64, "precision must be 0 or 1");
+  Plan& p = plan->p;
+  void* xb = nullptr;
+  void* ob = nullptr;
+  const size_t eb = precision == MSC_C128 ? sizeof(double2) : sizeof(float2);
+  MSC_HIP_CHECK(hipMalloc(&xb, p.n * eb));
+  MSC_HIP_CHECK(hipMalloc(&ob, p.n * eb));
+  MSC_HIP_CHECK(hipMemsetAsync(xb, 0, p.n * eb, S(stream)));
+  const int rc = precision == MSC_C128
+                     ? autotune<double>(p, reinterpret_cast<const double2*>(xb),
+                                        reinterpret_cast<double2*>(ob), S(stream))
+                     : autotune<float>(p, reinterpret_cast<const float2*>(xb),
+                                       reinterpret_cast<float2*>(ob), S(stream));
+  (void)hipStreamSynchronize(S(stream));
+  (void)hipFree(xb);
+  (void)hipFree(ob);
+  plan->k128.drop_graphs();
+  plan->k64.drop_graphs();
+  return rc;
+}
+
+int msc_plan_set_tiles(msc_plan* plan, const int* tiles10) {
+  MSC_REQUIRE(plan != nullptr && tiles10 != nullptr, "plan and tiles must not be NULL");
+  for (int i = 0; i < 10; ++i) {
+    const int v = tiles10[i];
+    const int pass = i % 5;
+    const bool ok = (pass == 0 || pass == 4) ? (v == 1 || v == 2)
+                    : (pass == 2) ? (v == 1 || v == 2 || v == 4 || v == 8)
+                                  : (v == 1 || v == 2 || v == 4);
+    MSC_REQUIRE(ok, "tile out of range (y passes 1-2, x passes 1/2/4, z pass 1/2/4/8)");
+  }
+  for (int pr = 0; pr < 2; ++pr) {
+    for (int i = 0; i < 5; ++i) plan->p.tile[pr][i] = tiles10[pr * 5 + i];
+    plan->p.tuned[pr] = true;
+  }
+  plan->k128.drop_graphs();
+  plan->k64.drop_graphs();
+  return kOk;
+}
+
+int msc_plan_tiles(const msc_plan* plan, int* out10) {
+  MSC_REQUIRE(plan != nullptr && out10 != nullptr, "plan and out must not be NULL");
+  for (int pr = 0; pr < 2; ++pr)
+    for (int i = 0; i < 5; ++i) out10[pr * 5 + i] = plan->p.tile[pr][i];
   return kOk;
 }
 

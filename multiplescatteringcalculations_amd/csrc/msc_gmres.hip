@@ -107,11 +107,90 @@ __global__ __launch_bounds__(kRT) void k_dots(const cx<R>* __restrict__ V, size_
   }
 }
 
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ double2 cm(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// Close Arnoldi step j with ||w|| = hn: form column j of the Hessenberg matrix
+// of the iterated operator, apply the previous Givens rotations, build the new
+// one (real c, complex s) and rotate g.  With `shifted`, the basis was built
+// from B = G0 T0 and the operator is A = I - B, so H_A = [I; 0] - H_B.
+__device__ void finish_step(KState* st, double2* __restrict__ H, int ldh,
+                            double2* __restrict__ cs, double2* __restrict__ g,
+                            double* __restrict__ hnext, int m, double hn, int shifted) {
+  const int j = st->j;
+  double2* col = H + (size_t)j * ldh;
+  hnext[j] = hn;
+  if (shifted) {
+    for (int i = 0; i < j; ++i) col[i] = make_double2(-col[i].x, -col[i].y);
+    col[j] = make_double2(1.0 - col[j].x, -col[j].y);
+    col[j + 1] = make_double2(-hn, 0.0);
+  } else {
+    col[j + 1] = make_double2(hn, 0.0);
+  }
+  for (int i = 0; i < j; ++i) {  // [c s; -conj(s) c]
+    const double c = cs[2 * i].x;
+    const double2 s = cs[2 * i + 1];
+    const double2 a = col[i], b = col[i + 1];
+    const double2 sb = cm(s, b);
+    const double2 csa = cm(cconj(s), a);
+    col[i] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
+    col[i + 1] = make_double2(c * b.x - csa.x, c * b.y - csa.y);
+  }
+  const double2 a = col[j], b = col[j + 1];
+  const double na = sqrt(cabs2(a)), nb = sqrt(cabs2(b));
+  const double d = sqrt(na * na + nb * nb);
+  double c;
+  double2 s;
+  if (d == 0.0) {
+    c = 1.0;
+    s = make_double2(0.0, 0.0);
+  } else if (na == 0.0) {
+    c = 0.0;
+    s = make_double2(b.x / nb, -b.y / nb);  // conj(b)/|b|
+  } else {
+    c = na / d;  // s = (a/|a|) conj(b) / d
+    const double2 u = make_double2(a.x / na, a.y / na);
+    s = cm(u, cconj(b));
+    s.x /= d;
+    s.y /= d;
+  }
+  cs[2 * j] = make_double2(c, 0.0);
+  cs[2 * j + 1] = s;
+  const double2 sb = cm(s, b);
+  col[j] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
+  col[j + 1] = make_double2(0.0, 0.0);
+  const double2 gj = g[j];
+  const double2 sg = cm(cconj(s), gj);
+  g[j] = make_double2(c * gj.x, c * gj.y);
+  g[j + 1] = make_double2(-sg.x, -sg.y);
+  const double res = sqrt(cabs2(g[j + 1]));
+  st->resid_est = res;
+  st->scale = hn < 1e-30 ? 0.0 : 1.0 / hn;
+  st->j = j + 1;
+  if (res <= st->tol_abs || hn < 1e-30 || j + 1 >= m) {
+    st->done = 1;
+    if (st->done_host != nullptr) {
+      *reinterpret_cast<volatile int*>(st->done_host) = 1;
+      __threadfence_system();
+    }
+  }
+}
+
 // Reduce the dot partials (fixed order) into hcur and add them to H[:, j].
+// Pass 0 also decides, from ||w||^2 - |h|^2 (exact for an orthonormal basis),
+// whether a second Gram-Schmidt pass is needed; if not it closes the step with
+// that norm, and k_update writes the normalised vector directly.
 __global__ __launch_bounds__(kRT) void k_reduce_dots(const double2* __restrict__ part, int nblk,
                                                      int nv, KState* st, double2* __restrict__ H,
                                                      int ldh, double2* __restrict__ hcur,
-                                                     int pass) {
+                                                     double2* __restrict__ cs,
+                                                     double2* __restrict__ g,
+                                                     double* __restrict__ hnext, int m, int pass,
+                                                     int orth_mode, int shifted) {
+  __shared__ double wnb;
   if (skip_step(st, pass)) return;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int ldp = nv + 1;
@@ -134,14 +213,29 @@ __global__ __launch_bounds__(kRT) void k_reduce_dots(const double2* __restrict__
           h.x += s.x;
           h.y += s.y;
         }
-      } else if (pass == 0) {
-        st->wn_before = s.x;
+      } else {
+        wnb = s.x;
       }
     }
   }
+  __syncthreads();
+  if (threadIdx.x != 0 || pass != 0) return;
+  double hh = 0.0;
+  for (int v = 0; v < nv; ++v) hh += cabs2(hcur[v]);
+  const double est = wnb - hh;
+  const bool again = orth_mode == kOrthCGS2 ||
+                     (orth_mode == kOrthAdaptive && !(est > 0.5 * wnb));
+  if (again) {
+    st->reorth = 1;
+    return;
+  }
+  st->reorth = 0;
+  finish_step(st, H, ldh, cs, g, hnext, m, sqrt(est > 0.0 ? est : 0.0), shifted);
 }
 
-// w -= sum_v hcur[v] V_v ; partial[blk] = sum |w|^2 after the update.
+// w -= sum_v hcur[v] V_v.  Without a second pass pending (pass 0, reorth == 0)
+// the result is scaled to unit norm in the same sweep; with one pending the
+// unscaled vector is kept.  Pass 1 writes per-workgroup ||w||^2 partials.
 template <typename R>
 __global__ __launch_bounds__(kRT) void k_update(const cx<R>* __restrict__ V, size_t ldv, int nv,
                                                 cx<R>* __restrict__ w, size_t n,
@@ -149,7 +243,9 @@ __global__ __launch_bounds__(kRT) void k_update(const cx<R>* __restrict__ V, siz
                                                 double* __restrict__ part, const KState* st,
                                                 int pass) {
   __shared__ double2 red[kRT / 64];
-  if (skip_step(st, pass)) return;
+  if (pass == 0 ? st->done != 0 : skip_step(st, pass)) return;
+  const bool finalise = pass == 0 && st->reorth == 0;
+  const double sc = finalise ? st->scale : 1.0;
   const size_t beg = (size_t)blockIdx.x * kChunk;
   const size_t end = beg + kChunk < n ? beg + kChunk : n;
   double2 nrm = make_double2(0.0, 0.0);
@@ -161,13 +257,15 @@ __global__ __launch_bounds__(kRT) void k_update(const cx<R>* __restrict__ V, siz
       a.x -= h.x * b.x - h.y * b.y;
       a.y -= h.x * b.y + h.y * b.x;
     }
-    const cx<R> o = from_c128<R>(a);
+    const cx<R> o = from_c128<R>(make_double2(a.x * sc, a.y * sc));
     w[t] = o;
     const double2 oo = to_c128(o);
     nrm.x += oo.x * oo.x + oo.y * oo.y;
   }
-  nrm = block_sum(nrm, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = nrm.x;
+  if (pass == 1) {
+    nrm = block_sum(nrm, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = nrm.x;
+  }
 }
 
 __device__ double reduce_scalar_partials(const double* part, int nblk, double* red) {
@@ -184,90 +282,26 @@ __device__ double reduce_scalar_partials(const double* part, int nblk, double* r
   return t;
 }
 
-__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
-__device__ __forceinline__ double2 cm(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-
-// Norm of the updated w; decide on a second CGS pass or finish the step:
-// H[j+1, j] = ||w||, apply previous rotations, new rotation, update g.
+// After a second Gram-Schmidt pass: exact norm, then close the step.
 __global__ __launch_bounds__(kRT) void k_norm_givens(const double* __restrict__ part, int nblk,
                                                      KState* st, double2* __restrict__ H, int ldh,
                                                      double2* __restrict__ cs,
                                                      double2* __restrict__ g,
-                                                     double* __restrict__ hnext, int m, int pass,
-                                                     int orth_mode) {
+                                                     double* __restrict__ hnext, int m,
+                                                     int shifted) {
   __shared__ double red[kRT / 64];
-  if (skip_step(st, pass)) return;
+  if (skip_step(st, 1)) return;
   const double wn = reduce_scalar_partials(part, nblk, red);
   if (threadIdx.x != 0) return;
-  if (pass == 0) {
-    const bool again = orth_mode == kOrthCGS2 ||
-                       (orth_mode == kOrthAdaptive && wn < 0.5 * st->wn_before);
-    if (again) {
-      st->reorth = 1;
-      return;
-    }
-  }
-  st->reorth = 0;
-  const int j = st->j;
-  double2* col = H + (size_t)j * ldh;
-  const double hn = sqrt(wn);
-  hnext[j] = hn;
-  col[j + 1] = make_double2(hn, 0.0);
-  // previous rotations: [c s; -conj(s) c], c real (stored in cs[i].x), s = sn
-  for (int i = 0; i < j; ++i) {
-    const double c = cs[2 * i].x;
-    const double2 s = cs[2 * i + 1];
-    const double2 a = col[i], b = col[i + 1];
-    const double2 sb = cm(s, b);
-    const double2 csa = cm(cconj(s), a);
-    col[i] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
-    col[i + 1] = make_double2(c * b.x - csa.x, c * b.y - csa.y);
-  }
-  const double2 a = col[j], b = col[j + 1];
-  const double na = sqrt(cabs2(a)), nb = sqrt(cabs2(b));
-  const double d = sqrt(na * na + nb * nb);
-  double c;
-  double2 s;
-  if (d == 0.0) {
-    c = 1.0;
-    s = make_double2(0.0, 0.0);
-  } else if (na == 0.0) {
-    c = 0.0;
-    s = make_double2(b.x / nb, -b.y / nb);  // conj(b)/|b|
-  } else {
-    c = na / d;
-    // s = (a/|a|) conj(b) / d
-    const double2 u = make_double2(a.x / na, a.y / na);
-    s = cm(u, cconj(b));
-    s.x /= d;
-    s.y /= d;
-  }
-  cs[2 * j] = make_double2(c, 0.0);
-  cs[2 * j + 1] = s;
-  const double2 sb = cm(s, b);
-  col[j] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
-  col[j + 1] = make_double2(0.0, 0.0);
-  const double2 gj = g[j];
-  const double2 sg = cm(cconj(s), gj);
-  g[j] = make_double2(c * gj.x, c * gj.y);
-  g[j + 1] = make_double2(-sg.x, -sg.y);
-  const double res = sqrt(cabs2(g[j + 1]));
-  st->resid_est = res;
-  st->j = j + 1;
-  if (res <= st->tol_abs || hn < 1e-30 || j + 1 >= m) st->done = 1;
+  finish_step(st, H, ldh, cs, g, hnext, m, sqrt(wn), shifted);
 }
 
-// V[j+1] = w / ||w|| (zero on breakdown), w stored in place.
+// V[j+1] = w / ||w|| after a second pass (zero on breakdown).
 template <typename R>
 __global__ __launch_bounds__(kRT) void k_scale_next(cx<R>* __restrict__ w, size_t n,
-                                                    const double* __restrict__ hnext,
-                                                    const KState* st, int jexpect) {
-  if (st->done) return;
-  const double hn = hnext[jexpect];
-  const double inv = hn < 1e-30 ? 0.0 : 1.0 / hn;
+                                                    const KState* st) {
+  if (skip_step(st, 1)) return;
+  const double inv = st->scale;
   for (size_t t = (size_t)blockIdx.x * kRT + threadIdx.x; t < n; t += (size_t)gridDim.x * kRT) {
     const double2 a = to_c128(w[t]);
     w[t] = from_c128<R>(make_double2(a.x * inv, a.y * inv));
@@ -369,10 +403,20 @@ __global__ __launch_bounds__(kRT) void k_cycle_init(const cx<RS>* __restrict__ r
       st->j = 0;
       st->reorth = 0;
       st->done = rn > 0.0 ? 0 : 1;
+      if (st->done && st->done_host != nullptr) {
+        *reinterpret_cast<volatile int*>(st->done_host) = 1;
+        __threadfence_system();
+      }
       st->tol_abs = tol_abs;
       st->resid_est = unit_rhs ? 1.0 : rn;
     }
   }
+}
+
+__global__ __launch_bounds__(kRT) void k_narrow(const double2* __restrict__ in,
+                                                float2* __restrict__ out, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * kRT + threadIdx.x; t < n; t += (size_t)gridDim.x * kRT)
+    out[t] = make_float2((float)in[t].x, (float)in[t].y);
 }
 
 // ── host side ───────────────────────────────────────────────────────────────
@@ -384,7 +428,17 @@ static int grid_for(size_t n) {
 
 Krylov::~Krylov() { release(); }
 
+void Krylov::drop_graphs() {
+  for (hipGraphExec_t e : graphs)
+    if (e) (void)hipGraphExecDestroy(e);
+  graphs.clear();
+  graph_orth = -1;
+}
+
 void Krylov::release() {
+  drop_graphs();
+  if (done_flag) (void)hipHostFree(done_flag);
+  done_flag = nullptr;
   if (V && owns_V) (void)hipFree(V);
   owns_V = false;
   if (st) (void)hipFree(st);
@@ -422,6 +476,11 @@ int Krylov::ensure_state(size_t n_, int m_, int elem_bytes_) {
   part = reinterpret_cast<double2*>(scal) + 2;
   hnext = reinterpret_cast<double*>(part + npart);
   MSC_HIP_CHECK(hipHostMalloc(&host, sizeof(KState), hipHostMallocDefault));
+  MSC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&done_flag), sizeof(int), hipHostMallocMapped));
+  *done_flag = 0;
+  int* dview = nullptr;
+  MSC_HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&dview), done_flag, 0));
+  MSC_HIP_CHECK(hipMemcpy(&st->done_host, &dview, sizeof(int*), hipMemcpyHostToDevice));
   return kOk;
 }
 
@@ -429,6 +488,7 @@ int Krylov::ensure(size_t n_, int m_, int elem_bytes_) {
   if (V != nullptr && owns_V && n_ == n && m_ == m && elem_bytes_ == elem_bytes) return kOk;
   int rc = ensure_state(n_, m_, elem_bytes_);
   if (rc) return rc;
+  drop_graphs();
   if (V != nullptr && owns_V) (void)hipFree(V);
   V = nullptr;
   MSC_HIP_CHECK(hipMalloc(&V, (size_t)(m + 1) * ldv * elem_bytes));
@@ -441,20 +501,22 @@ int krylov_step(Krylov& k, int j, int orth_mode, hipStream_t s) {
   const cx<R>* V = reinterpret_cast<const cx<R>*>(k.V);
   cx<R>* w = reinterpret_cast<cx<R>*>(k.V) + (size_t)(j + 1) * k.ldv;
   const int nv = j + 1;
+  const int sh = k.shifted ? 1 : 0;
+  double* dpart = reinterpret_cast<double*>(k.part);
   for (int pass = 0; pass < 2; ++pass) {
     hipLaunchKernelGGL((k_dots<R>), dim3(k.nblk), dim3(kRT), 0, s, V, k.ldv, nv, w, k.n, k.part,
                        k.st, pass);
     hipLaunchKernelGGL(k_reduce_dots, dim3(1), dim3(kRT), 0, s, k.part, k.nblk, nv, k.st, k.H,
-                       k.ldh, k.hcur, pass);
+                       k.ldh, k.hcur, k.cs, k.g, k.hnext, k.m, pass, orth_mode, sh);
     hipLaunchKernelGGL((k_update<R>), dim3(k.nblk), dim3(kRT), 0, s, V, k.ldv, nv, w, k.n,
-                       k.hcur, reinterpret_cast<double*>(k.part), k.st, pass);
-    hipLaunchKernelGGL(k_norm_givens, dim3(1), dim3(kRT), 0, s,
-                       reinterpret_cast<const double*>(k.part), k.nblk, k.st, k.H, k.ldh, k.cs,
-                       k.g, k.hnext, k.m, pass, orth_mode);
+                       k.hcur, dpart, k.st, pass);
     if (orth_mode == kOrthCGS1) break;
   }
-  hipLaunchKernelGGL((k_scale_next<R>), dim3(grid_for(k.n)), dim3(kRT), 0, s, w, k.n, k.hnext,
-                     k.st, j);
+  if (orth_mode != kOrthCGS1) {
+    hipLaunchKernelGGL(k_norm_givens, dim3(1), dim3(kRT), 0, s, dpart, k.nblk, k.st, k.H, k.ldh,
+                       k.cs, k.g, k.hnext, k.m, sh);
+    hipLaunchKernelGGL((k_scale_next<R>), dim3(grid_for(k.n)), dim3(kRT), 0, s, w, k.n, k.st);
+  }
   MSC_LAUNCH_CHECK();
   return kOk;
 }
@@ -481,6 +543,12 @@ int krylov_finish(Krylov& k, cx<RX>* x, const double* scale_dev, hipStream_t s) 
   hipLaunchKernelGGL(k_backsub, dim3(1), dim3(kRT), 0, s, k.st, k.H, k.ldh, k.g, k.y, k.tmp);
   hipLaunchKernelGGL((k_axpy_basis<R, RX>), dim3(grid_for(k.n)), dim3(kRT), 0, s,
                      reinterpret_cast<const cx<R>*>(k.V), k.ldv, k.y, k.st, x, k.n, scale_dev);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+int krylov_narrow(const double2* in, float2* out, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_narrow, dim3(grid_for(n)), dim3(kRT), 0, s, in, out, n);
   MSC_LAUNCH_CHECK();
   return kOk;
 }

@@ -1,6 +1,8 @@
 // GMRES workspace and building blocks (device-resident Arnoldi, FP64 state).
 #pragma once
 
+#include <vector>
+
 #include "msc_common.hpp"
 
 namespace msc {
@@ -17,6 +19,8 @@ struct KState {
   double tol_abs;    // stop when |g_{j+1}| <= tol_abs
   double wn_before;  // ||w||^2 before the first projection
   double resid_est;  // |g_{j+1}| after the last step
+  double scale;      // 1 / ||w|| of the last closed step (0 on breakdown)
+  int* done_host;    // device view of a pinned host flag mirroring `done`
 };
 
 struct Krylov {
@@ -34,7 +38,14 @@ struct Krylov {
   double2* part = nullptr;    // per-workgroup partials
   double* hnext = nullptr;    // ||w|| of each step (unrotated H[j+1, j])
   void* host = nullptr;       // pinned staging of KState
+  int* done_flag = nullptr;   // pinned, device-mapped mirror of KState::done
   bool owns_V = false;
+  bool shifted = false;  // basis built from B = G0 T0, operator A = I - B
+  // instantiated HIP graphs of whole Arnoldi steps (operator + orthogonalisation),
+  // indexed by step; valid while the buffers and `graph_orth` are unchanged
+  std::vector<hipGraphExec_t> graphs;
+  int graph_orth = -1;
+  void drop_graphs();
 
   ~Krylov();
   void release();
@@ -52,5 +63,6 @@ int krylov_cycle_init(Krylov& k, const cx<RS>* r, double tol_abs, bool unit_rhs,
 template <typename R, typename RX>
 int krylov_finish(Krylov& k, cx<RX>* x, const double* scale_dev, hipStream_t s);
 int krylov_read_state(Krylov& k, KState* out, hipStream_t s);
+int krylov_narrow(const double2* in, float2* out, size_t n, hipStream_t s);
 
 }  // namespace msc

@@ -44,36 +44,30 @@ __device__ __forceinline__ void swap_xy(cx<R>* t) {
   q = t[7]; t[7] = t[8]; t[8] = q;
 }
 
-// o = K t for the canonical spectral propagator K = [[G, C], [H, S]] stored
-// as 45 entries (G sym 6, C 3x6, upper triangle of S' = S W^-1 21) with
-// H = W^-1 C^T and S = S' W, W = diag(1,1,1,1/2,1/2,1/2).
+// o = K t for the canonical spectral propagator K = [[G, C], [H, S]] held as
+// its 45 independent entries K[] (G sym 6, C 3x6 row-major, upper triangle
+// of S' = S W^-1 21), with H = W^-1 C^T and S = S' W, W = diag(1,1,1,1/2,1/2,1/2).
 template <typename R>
-__device__ __forceinline__ void kmul45(const cx<R>* __restrict__ tab, int NC, int c,
-                                       const cx<R>* t, cx<R>* o) {
-#define KE(e) tab[(e) * NC + c]
-  {
-    const cx<R> g00 = KE(0), g01 = KE(1), g02 = KE(2), g11 = KE(3), g12 = KE(4), g22 = KE(5);
-    o[0] = cmul<R>(g00, t[0]);
-    cfma<R>(o[0], g01, t[1]);
-    cfma<R>(o[0], g02, t[2]);
-    o[1] = cmul<R>(g01, t[0]);
-    cfma<R>(o[1], g11, t[1]);
-    cfma<R>(o[1], g12, t[2]);
-    o[2] = cmul<R>(g02, t[0]);
-    cfma<R>(o[2], g12, t[1]);
-    cfma<R>(o[2], g22, t[2]);
-  }
+__device__ __forceinline__ void kmul45(const cx<R>* K, const cx<R>* t, cx<R>* o) {
+  o[0] = cmul<R>(K[0], t[0]);
+  cfma<R>(o[0], K[1], t[1]);
+  cfma<R>(o[0], K[2], t[2]);
+  o[1] = cmul<R>(K[1], t[0]);
+  cfma<R>(o[1], K[3], t[1]);
+  cfma<R>(o[1], K[4], t[2]);
+  o[2] = cmul<R>(K[2], t[0]);
+  cfma<R>(o[2], K[4], t[1]);
+  cfma<R>(o[2], K[5], t[2]);
 #pragma unroll
   for (int al = 0; al < 6; ++al) {
-    const cx<R> c0 = KE(6 + al), c1 = KE(12 + al), c2 = KE(18 + al);
-    const R wi = al < 3 ? R(1) : R(2);
+    const cx<R> c0 = K[6 + al], c1 = K[12 + al], c2 = K[18 + al];
     cfma<R>(o[0], c0, t[3 + al]);
     cfma<R>(o[1], c1, t[3 + al]);
     cfma<R>(o[2], c2, t[3 + al]);
     cx<R> h = cmul<R>(c0, t[0]);
     cfma<R>(h, c1, t[1]);
     cfma<R>(h, c2, t[2]);
-    o[3 + al] = cscale<R>(h, wi);
+    o[3 + al] = al < 3 ? h : cscale<R>(h, R(2));
   }
   cx<R> tw[6];
 #pragma unroll
@@ -82,12 +76,11 @@ __device__ __forceinline__ void kmul45(const cx<R>* __restrict__ tab, int NC, in
   for (int al = 0; al < 6; ++al) {
 #pragma unroll
     for (int be = al; be < 6; ++be) {
-      const cx<R> s = KE(24 + al * 6 - (al * (al - 1)) / 2 + (be - al));
-      cfma<R>(o[3 + al], s, tw[be]);
-      if (be != al) cfma<R>(o[3 + be], s, tw[al]);
+      const cx<R> sv = K[24 + al * 6 - (al * (al - 1)) / 2 + (be - al)];
+      cfma<R>(o[3 + al], sv, tw[be]);
+      if (be != al) cfma<R>(o[3 + be], sv, tw[al]);
     }
   }
-#undef KE
 }
 
 // ── pass 1: T0 gather + zero pad + forward FFT along y ─────────────────────
@@ -108,9 +101,9 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
     const int y = rem / 9;
     const int c = rem - y * 9;
     const int row = row0 + b;
-    cx<R> v = czero<R>();
-    if (y < M && row < nrows) v = x[(size_t)row * M * 9 + rem];
-    s[y * LD + b * 9 + c] = v;
+    const bool ok = y < M && row < nrows;
+    const cx<R> v = x[ok ? (size_t)row * M * 9 + rem : 0];
+    s[y * LD + b * 9 + c] = ok ? v : czero<R>();
   }
   __syncthreads();
   for (int w = threadIdx.x; w < B * M; w += kNT) {
@@ -159,9 +152,9 @@ __global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
   for (int w = threadIdx.x; w < P * NTR; w += kNT) {
     const int xr = w / NTR;
     const int e = w - xr * NTR;
-    cx<R> v = czero<R>();
-    if (xr < M) v = buf1[(((size_t)z * M + xr) * P + ky0) * 9 + e];
-    s[xr * LD + e] = v;
+    const bool ok = xr < M;
+    const cx<R> v = buf1[ok ? (((size_t)z * M + xr) * P + ky0) * 9 + e : 0];
+    s[xr * LD + e] = ok ? v : czero<R>();
   }
   __syncthreads();
   lds_fft<R, P, false, NTR, LD, kNT>(s, tw);
@@ -191,9 +184,9 @@ __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
   for (int w = threadIdx.x; w < PZ * NTR; w += kNT) {
     const int z = w / NTR;
     const int e = w - z * NTR;
-    cx<R> v = czero<R>();
-    if (z < Nz) v = buf2[(size_t)z * plane + col0 + e];
-    s[z * LD + e] = v;
+    const bool ok = z < Nz;
+    const cx<R> v = buf2[ok ? (size_t)z * plane + col0 + e : col0];
+    s[z * LD + e] = ok ? v : czero<R>();
   }
   __syncthreads();
   lds_fft<R, PZ, false, NTR, LD, kNT>(s, tw);
@@ -216,7 +209,10 @@ __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
 #pragma unroll
     for (int i = 0; i < 9; ++i) t[i] = flip_if<R>(sp[i], flip, i);
     if (sw) swap_xy<R>(t);
-    kmul45<R>(table + (size_t)uv * kNE * NC, NC, c, t, o);
+    cx<R> K[kNE];
+#pragma unroll
+    for (int e = 0; e < kNE; ++e) K[e] = table[((size_t)uv * kNE + e) * NC + c];
+    kmul45<R>(K, t, o);
     if (sw) swap_xy<R>(o);
 #pragma unroll
     for (int i = 0; i < 9; ++i) sp[i] = flip_if<R>(o[i], flip, i);
@@ -258,7 +254,10 @@ __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
 }
 
 // ── pass 5: inverse FFT along y, keep y < M, y = x - acc (or b - x + acc) ──
-template <typename R, int P, int B, bool RESID>
+// ── pass 5: inverse FFT along y, keep y < M and form the output ────────────
+// MODE 0: out = x - acc (the operator), 1: out = b - x + acc (residual),
+// 2: out = acc (the scattered part G0 T0 x).
+template <typename R, int P, int B, int MODE>
 __global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
                                               const cx<R>* __restrict__ x,
                                               const cx<R>* __restrict__ bvec,
@@ -276,9 +275,9 @@ __global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
     const int y = rem / 9;
     const int c = rem - y * 9;
     const int row = row0 + b;
-    cx<R> v = czero<R>();
-    if (row < nrows) v = buf1[(size_t)row * P * 9 + rem];
-    s[y * LD + b * 9 + c] = v;
+    const bool ok = row < nrows;
+    const cx<R> v = buf1[ok ? (size_t)row * P * 9 + rem : 0];
+    s[y * LD + b * 9 + c] = ok ? v : czero<R>();
   }
   __syncthreads();
   lds_fft<R, P, true, NTR, LD, kNT>(s, tw);
@@ -291,8 +290,10 @@ __global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
     if (row >= nrows) continue;
     const size_t g = (size_t)row * M * 9 + rem;
     const cx<R> acc = s[y * LD + b * 9 + c];
-    if constexpr (RESID) {
+    if constexpr (MODE == 1) {
       out[g] = cadd<R>(csub<R>(bvec[g], x[g]), acc);
+    } else if constexpr (MODE == 2) {
+      out[g] = acc;
     } else {
       out[g] = csub<R>(x[g], acc);
     }
@@ -313,46 +314,118 @@ template <> struct PlanBufs<float> {
   static const float2* tw(const Plan& p) { return p.tw64; }
 };
 
-// Rows per workgroup of the y passes and columns per workgroup of the x/z
-// passes, chosen so that the LDS tile stays <= ~80 KB.
-template <typename R, int P> struct YRows { static constexpr int value = (P * 18 * sizeof(cx<R>) <= 80 * 1024) ? 2 : 1; };
-template <typename R, int P> struct XCols { static constexpr int value = (P >= 2 && P * 18 * sizeof(cx<R>) <= 80 * 1024) ? 2 : 1; };
-template <typename R, int PZ> struct ZCols {
-  static constexpr int c4 = (PZ * 36 * sizeof(cx<R>) <= 80 * 1024) ? 4 : 2;
-  static constexpr int value = c4;
-};
+// Tiles: rows per workgroup of the y passes (B), columns per workgroup of the
+// x passes and of the z pass (Q).  Each plan starts from the defaults below
+// (measured best at the 128x128x32 slab, P = 256, Pz = 64) and autotune()
+// re-picks them per precision by timing every candidate in place.
+constexpr int kLdsMax = 160 * 1024;
+template <typename R, int N, int PER, int T>
+constexpr bool lds_fits() {
+  return N * (PER * T + 1) * (int)sizeof(cx<R>) <= kLdsMax;
+}
+
+int default_tile(int prec, int pass) {
+  // pass: 0 y-forward B, 1 x-forward Q, 2 z Q, 3 x-inverse Q, 4 y-inverse B
+  static const int t64[5] = {1, 2, 4, 2, 1};
+  static const int t128[5] = {1, 1, 4, 1, 1};
+  return prec == 1 ? t64[pass] : t128[pass];
+}
+
+template <typename R> constexpr int prec_of() { return sizeof(cx<R>) == 8 ? 1 : 0; }
+
+template <typename R, int P, int B>
+static void fwd_y_b(Plan& p, const cx<R>* x, cx<R>* b1, hipStream_t st, const int* done) {
+  const int nrows = p.Nz * p.M;
+  hipLaunchKernelGGL((k_fwd_y<R, P, B>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, x,
+                     PlanBufs<R>::tp(p), b1, p.M, nrows, PlanBufs<R>::tw(p), done);
+}
+template <typename R, int P, int B, int MODE>
+static void inv_y_b(Plan& p, const cx<R>* b1, const cx<R>* x, const cx<R>* bvec, cx<R>* out,
+                    hipStream_t st, const int* done) {
+  const int nrows = p.Nz * p.M;
+  hipLaunchKernelGGL((k_inv_y<R, P, B, MODE>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, b1, x,
+                     bvec, out, p.M, nrows, PlanBufs<R>::tw(p), done);
+}
 
 template <typename R, int P>
-static int launch_xy_fwd(Plan& p, const cx<R>* x, cx<R>* b1, cx<R>* b2, hipStream_t st, const int* done) {
-  constexpr int B = YRows<R, P>::value;
-  constexpr int Q = XCols<R, P>::value < P ? XCols<R, P>::value : 1;
-  const int nrows = p.Nz * p.M;
-  const cx<R>* tw = PlanBufs<R>::tw(p);
-  hipLaunchKernelGGL((k_fwd_y<R, P, B>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, x,
-                     PlanBufs<R>::tp(p), b1, p.M, nrows, tw, done);
-  MSC_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_fwd_x<R, P, Q>), dim3(p.Nz * (P / Q)), dim3(kNT), 0, st, b1, b2, p.M, tw,
-                     done);
+static int pass_fwd_y(Plan& p, const cx<R>* x, cx<R>* b1, hipStream_t st, const int* done) {
+  switch (p.tile[prec_of<R>()][0]) {
+    case 2:
+      if constexpr (lds_fits<R, P, 9, 2>()) { fwd_y_b<R, P, 2>(p, x, b1, st, done); break; }
+      [[fallthrough]];
+    default:
+      fwd_y_b<R, P, 1>(p, x, b1, st, done);
+  }
   MSC_LAUNCH_CHECK();
   return kOk;
 }
 
 template <typename R, int P>
-static int launch_xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, cx<R>* b1,
-                         cx<R>* b2, hipStream_t st, const int* done) {
-  constexpr int B = YRows<R, P>::value;
-  constexpr int Q = XCols<R, P>::value < P ? XCols<R, P>::value : 1;
-  const int nrows = p.Nz * p.M;
+static int pass_fwd_x(Plan& p, const cx<R>* b1, cx<R>* b2, hipStream_t st, const int* done) {
   const cx<R>* tw = PlanBufs<R>::tw(p);
-  hipLaunchKernelGGL((k_inv_x<R, P, Q>), dim3(p.Nz * (P / Q)), dim3(kNT), 0, st, b2, b1, p.M, tw,
-                     done);
+  switch (p.tile[prec_of<R>()][1]) {
+    case 4:
+      if constexpr (P >= 4 && lds_fits<R, P, 9, 4>()) {
+        hipLaunchKernelGGL((k_fwd_x<R, P, 4>), dim3(p.Nz * (P / 4)), dim3(kNT), 0, st, b1, b2, p.M,
+                           tw, done);
+        break;
+      }
+      [[fallthrough]];
+    case 2:
+      if constexpr (lds_fits<R, P, 9, 2>()) {
+        hipLaunchKernelGGL((k_fwd_x<R, P, 2>), dim3(p.Nz * (P / 2)), dim3(kNT), 0, st, b1, b2, p.M,
+                           tw, done);
+        break;
+      }
+      [[fallthrough]];
+    default:
+      hipLaunchKernelGGL((k_fwd_x<R, P, 1>), dim3(p.Nz * P), dim3(kNT), 0, st, b1, b2, p.M, tw,
+                         done);
+  }
   MSC_LAUNCH_CHECK();
-  if (bvec != nullptr) {
-    hipLaunchKernelGGL((k_inv_y<R, P, B, true>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, b1,
-                       x, bvec, out, p.M, nrows, tw, done);
+  return kOk;
+}
+
+template <typename R, int P>
+static int pass_inv_x(Plan& p, const cx<R>* b2, cx<R>* b1, hipStream_t st, const int* done) {
+  const cx<R>* tw = PlanBufs<R>::tw(p);
+  switch (p.tile[prec_of<R>()][3]) {
+    case 4:
+      if constexpr (P >= 4 && lds_fits<R, P, 9, 4>()) {
+        hipLaunchKernelGGL((k_inv_x<R, P, 4>), dim3(p.Nz * (P / 4)), dim3(kNT), 0, st, b2, b1, p.M,
+                           tw, done);
+        break;
+      }
+      [[fallthrough]];
+    case 2:
+      if constexpr (lds_fits<R, P, 9, 2>()) {
+        hipLaunchKernelGGL((k_inv_x<R, P, 2>), dim3(p.Nz * (P / 2)), dim3(kNT), 0, st, b2, b1, p.M,
+                           tw, done);
+        break;
+      }
+      [[fallthrough]];
+    default:
+      hipLaunchKernelGGL((k_inv_x<R, P, 1>), dim3(p.Nz * P), dim3(kNT), 0, st, b2, b1, p.M, tw,
+                         done);
+  }
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename R, int P>
+static int pass_inv_y(Plan& p, const cx<R>* b1, const cx<R>* x, const cx<R>* bvec, cx<R>* out,
+                      int mode, hipStream_t st, const int* done) {
+  const bool two = p.tile[prec_of<R>()][4] == 2 && lds_fits<R, P, 9, 2>();
+  if (two) {
+    if constexpr (lds_fits<R, P, 9, 2>()) {
+      if (mode == 1) inv_y_b<R, P, 2, 1>(p, b1, x, bvec, out, st, done);
+      else if (mode == 2) inv_y_b<R, P, 2, 2>(p, b1, x, bvec, out, st, done);
+      else inv_y_b<R, P, 2, 0>(p, b1, x, bvec, out, st, done);
+    }
   } else {
-    hipLaunchKernelGGL((k_inv_y<R, P, B, false>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st,
-                       b1, x, bvec, out, p.M, nrows, tw, done);
+    if (mode == 1) inv_y_b<R, P, 1, 1>(p, b1, x, bvec, out, st, done);
+    else if (mode == 2) inv_y_b<R, P, 1, 2>(p, b1, x, bvec, out, st, done);
+    else inv_y_b<R, P, 1, 0>(p, b1, x, bvec, out, st, done);
   }
   MSC_LAUNCH_CHECK();
   return kOk;
@@ -382,24 +455,63 @@ int timing_end(Plan& p, hipStream_t st, size_t slot) {
   return kOk;
 }
 
+template <typename R, int PZ, int Q>
+static void z_q(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
+  hipLaunchKernelGGL((k_z_mult<R, PZ, Q>), dim3(p.Pxy * (p.Pxy / Q)), dim3(kNT), 0, st, b2,
+                     PlanBufs<R>::table(p), p.Nz, p.Pxy, PlanBufs<R>::tw(p), done);
+}
+
 template <typename R, int PZ>
 static int launch_z(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
   size_t slot;
   int rc = timing_begin(p, st, &slot, (int)sizeof(cx<R>));
   if (rc) return rc;
-  constexpr int Q0 = ZCols<R, PZ>::value;
-  const int Q = (p.Pxy % Q0 == 0) ? Q0 : 1;
-  const cx<R>* tw = PlanBufs<R>::tw(p);
-  const cx<R>* tab = PlanBufs<R>::table(p);
-  if (Q == Q0) {
-    hipLaunchKernelGGL((k_z_mult<R, PZ, Q0>), dim3(p.Pxy * (p.Pxy / Q0)), dim3(kNT), 0, st, b2,
-                       tab, p.Nz, p.Pxy, tw, done);
-  } else {
-    hipLaunchKernelGGL((k_z_mult<R, PZ, 1>), dim3(p.Pxy * p.Pxy), dim3(kNT), 0, st, b2, tab, p.Nz,
-                       p.Pxy, tw, done);
+  int q = p.tile[prec_of<R>()][2];
+  while (q > 1 && p.Pxy % q != 0) q >>= 1;
+  switch (q) {
+    case 8:
+      if constexpr (lds_fits<R, PZ, 9, 8>()) { z_q<R, PZ, 8>(p, b2, st, done); break; }
+      [[fallthrough]];
+    case 4:
+      if constexpr (lds_fits<R, PZ, 9, 4>()) { z_q<R, PZ, 4>(p, b2, st, done); break; }
+      [[fallthrough]];
+    case 2:
+      if constexpr (lds_fits<R, PZ, 9, 2>()) { z_q<R, PZ, 2>(p, b2, st, done); break; }
+      [[fallthrough]];
+    default:
+      z_q<R, PZ, 1>(p, b2, st, done);
   }
   MSC_LAUNCH_CHECK();
   return timing_end(p, st, slot);
+}
+
+template <typename R, int P>
+static int launch_xy_fwd(Plan& p, const cx<R>* x, cx<R>* b1, cx<R>* b2, hipStream_t st,
+                         const int* done) {
+  int rc = pass_fwd_y<R, P>(p, x, b1, st, done);
+  if (rc) return rc;
+  return pass_fwd_x<R, P>(p, b1, b2, st, done);
+}
+
+template <typename R, int P>
+static int launch_xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, cx<R>* b1,
+                         cx<R>* b2, int mode, hipStream_t st, const int* done) {
+  int rc = pass_inv_x<R, P>(p, b2, b1, st, done);
+  if (rc) return rc;
+  return pass_inv_y<R, P>(p, b1, x, bvec, out, mode, st, done);
+}
+
+// One pass alone (autotuning): 0 y-forward, 1 x-forward, 2 z, 3 x-inverse, 4 y-inverse.
+template <typename R, int P>
+static int launch_one_xy(Plan& p, int pass, const cx<R>* x, cx<R>* out, hipStream_t st) {
+  cx<R>* b1 = reinterpret_cast<cx<R>*>(p.buf1);
+  cx<R>* b2 = reinterpret_cast<cx<R>*>(p.buf2);
+  switch (pass) {
+    case 0: return pass_fwd_y<R, P>(p, x, b1, st, nullptr);
+    case 1: return pass_fwd_x<R, P>(p, b1, b2, st, nullptr);
+    case 3: return pass_inv_x<R, P>(p, b2, b1, st, nullptr);
+    default: return pass_inv_y<R, P>(p, b1, x, nullptr, out, 2, st, nullptr);
+  }
 }
 
 #define MSC_XY_CASES(F, ...)                 \
@@ -437,8 +549,14 @@ static int xy_fwd(Plan& p, const cx<R>* x, cx<R>* b1, cx<R>* b2, hipStream_t st,
 }
 template <typename R>
 static int xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, cx<R>* b1, cx<R>* b2,
-                  hipStream_t st, const int* done) {
-  MSC_XY_CASES(launch_xy_inv, p, x, bvec, out, b1, b2, st, done);
+                  int mode, hipStream_t st, const int* done) {
+  MSC_XY_CASES(launch_xy_inv, p, x, bvec, out, b1, b2, mode, st, done);
+  set_error("unsupported lateral FFT length");
+  return kUnsupported;
+}
+template <typename R>
+static int one_xy(Plan& p, int pass, const cx<R>* x, cx<R>* out, hipStream_t st) {
+  MSC_XY_CASES(launch_one_xy, p, pass, x, out, st);
   set_error("unsupported lateral FFT length");
   return kUnsupported;
 }
@@ -450,8 +568,8 @@ static int z_pass(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
 }
 
 template <typename R>
-static int apply(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, hipStream_t st,
-                 const int* done) {
+static int apply(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, int mode,
+                 hipStream_t st, const int* done) {
   if (!p.have_table || !p.have_tparams) {
     set_error("plan needs build_g0 and set_tmatrix before the operator can be applied");
     return kNotReady;
@@ -462,17 +580,78 @@ static int apply(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, hipStre
   if (rc) return rc;
   rc = z_pass<R>(p, b2, st, done);
   if (rc) return rc;
-  return xy_inv<R>(p, x, bvec, out, b1, b2, st, done);
+  return xy_inv<R>(p, x, bvec, out, b1, b2, mode, st, done);
 }
 
 template <typename R>
 int matvec(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done) {
-  return apply<R>(p, x, nullptr, y, st, done);
+  return apply<R>(p, x, nullptr, y, 0, st, done);
 }
 template <typename R>
 int residual(Plan& p, const cx<R>* b, const cx<R>* x, cx<R>* r, hipStream_t st, const int* done) {
-  return apply<R>(p, x, b, r, st, done);
+  return apply<R>(p, x, b, r, 1, st, done);
 }
+template <typename R>
+int scattered(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done) {
+  return apply<R>(p, x, nullptr, y, 2, st, done);
+}
+
+// Pick the tile of every pass for precision R by timing each candidate in
+// place on the plan's workspaces (inputs: x, outputs: out, both n complex).
+template <typename R>
+int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st) {
+  if (!p.have_table || !p.have_tparams) {
+    set_error("autotune needs build_g0 and set_tmatrix first");
+    return kNotReady;
+  }
+  const int pr = prec_of<R>();
+  const bool saved_timing = p.timing;
+  p.timing = false;
+  hipEvent_t e0, e1;
+  MSC_HIP_CHECK(hipEventCreate(&e0));
+  MSC_HIP_CHECK(hipEventCreate(&e1));
+  static const int cand_y[] = {1, 2};
+  static const int cand_x[] = {1, 2, 4};
+  static const int cand_z[] = {2, 4, 8};
+  static const int order[] = {2, 1, 3, 0, 4};
+  int rc = apply<R>(p, x, nullptr, out, 2, st, nullptr);  // warm every buffer
+  for (int pass : order) {
+    if (rc) break;
+    const int* cand = pass == 2 ? cand_z : (pass == 0 || pass == 4 ? cand_y : cand_x);
+    const int nc = pass == 2 ? 3 : (pass == 0 || pass == 4 ? 2 : 3);
+    int best = p.tile[pr][pass];
+    float best_ms = 1e30f;
+    for (int i = 0; i < nc && !rc; ++i) {
+      p.tile[pr][pass] = cand[i];
+      auto run = [&]() {
+        return pass == 2 ? z_pass<R>(p, reinterpret_cast<cx<R>*>(p.buf2), st, nullptr)
+                         : one_xy<R>(p, pass, x, out, st);
+      };
+      rc = run();
+      if (rc) break;
+      MSC_HIP_CHECK(hipEventRecord(e0, st));
+      for (int rep = 0; rep < 3 && !rc; ++rep) rc = run();
+      MSC_HIP_CHECK(hipEventRecord(e1, st));
+      MSC_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      MSC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cand[i];
+      }
+    }
+    p.tile[pr][pass] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p.timing = saved_timing;
+  if (rc) return rc;
+  p.tuned[pr] = true;
+  return kOk;
+}
+
+template int autotune<double>(Plan&, const double2*, double2*, hipStream_t);
+template int autotune<float>(Plan&, const float2*, float2*, hipStream_t);
 
 template int matvec<double>(Plan&, const double2*, double2*, hipStream_t, const int*);
 template int matvec<float>(Plan&, const float2*, float2*, hipStream_t, const int*);
@@ -480,6 +659,8 @@ template int residual<double>(Plan&, const double2*, const double2*, double2*, h
                               const int*);
 template int residual<float>(Plan&, const float2*, const float2*, float2*, hipStream_t,
                              const int*);
+template int scattered<double>(Plan&, const double2*, double2*, hipStream_t, const int*);
+template int scattered<float>(Plan&, const float2*, float2*, hipStream_t, const int*);
 
 // ── generic strided batched FFT (table build, tests) ───────────────────────
 // data [outer][N][inner]; a workgroup transforms NCOL consecutive inner

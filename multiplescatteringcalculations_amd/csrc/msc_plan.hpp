@@ -31,6 +31,10 @@ struct Plan {
   int NC = 0, NUV = 0;
 
   double omega = 0.0, alpha = 0.0, beta = 0.0, rho = 0.0;
+  // tiles per precision (0 complex128, 1 complex64) and pass (y-forward B,
+  // x-forward Q, z Q, x-inverse Q, y-inverse B); see autotune()
+  int tile[2][5] = {{1, 1, 4, 1, 1}, {1, 2, 4, 2, 1}};
+  bool tuned[2] = {false, false};
   bool have_table = false, have_tparams = false;
 
   double2* table128 = nullptr;
@@ -63,6 +67,15 @@ int matvec(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done);
 // Residual r = b - (I - G0 T0) x (fused into the last FFT pass).
 template <typename R>
 int residual(Plan& p, const cx<R>* b, const cx<R>* x, cx<R>* r, hipStream_t st, const int* done);
+
+// Scattered part y = G0 T0 x = x - (I - G0 T0) x (the operator GMRES iterates on).
+template <typename R>
+int scattered(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done);
+
+// Time every tile candidate of every pass for precision R in place (x, out:
+// n-element scratch vectors) and keep the fastest; sets p.tuned[prec].
+template <typename R>
+int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st);
 
 // Generic strided batched FFT used by the table build:
 // data viewed as [outer][N][inner], transform along N for every inner index.

@@ -303,6 +303,23 @@ class SlabPlan:
                                                 _lib.ptr(ls), _lib.ptr(ph), self.stream))
         return ls.cpu().numpy(), ph.cpu().numpy()
 
+    def autotune(self, precision: str = "c64") -> None:
+        """Time every tile candidate of the FFT passes in place and keep the fastest."""
+        prec = {"c128": _lib.MSC_C128, "c64": _lib.MSC_C64}[precision]
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.msc_plan_autotune(self._h, prec, self.stream))
+
+    def tiles(self) -> dict:
+        """Tiles per precision: (y-fwd rows, x-fwd cols, z cols, x-inv cols, y-inv rows)."""
+        out = (ctypes.c_int * 10)()
+        _lib.check(self.lib.msc_plan_tiles(self._h, out))
+        return {"c128": tuple(out[0:5]), "c64": tuple(out[5:10])}
+
+    def set_tiles(self, c128: tuple, c64: tuple) -> None:
+        """Fix the tiles (5 per precision, order as in ``tiles()``)."""
+        arr = (ctypes.c_int * 10)(*[int(v) for v in tuple(c128) + tuple(c64)])
+        _lib.check(self.lib.msc_plan_set_tiles(self._h, arr))
+
     def timing(self, enable: bool | None = None) -> dict:
         """Enable/disable (and reset) the HIP-event timing of the block-multiply
         pass, or read it: {"c64": (launches, ms), "c128": (launches, ms)}."""

@@ -300,3 +300,45 @@ def test_full_size_operator_properties(hip_device):
     y128 = plan.matvec(x)
     y64 = plan.matvec(x.to(torch.complex64)).to(torch.complex128)
     assert float(((x - y64) - (x - y128)).norm() / (x - y128).norm()) < 1e-5
+
+
+@pytest.mark.parametrize("tiles", [(1, 1, 1, 1, 1), (2, 2, 2, 2, 2), (1, 4, 4, 4, 1), (2, 1, 8, 2, 1)])
+def test_every_tile_variant_matches_oracle(hip_device, tiles):
+    """Each tile instantiation of the five passes gives the same operator."""
+    M, Nz = 9, 4
+    ref, om = REF_C, 692.8
+    Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=17)
+    plan, *_ = _plan(M, Nz, 1.0, ref, om, Dl, Dm, Dr)
+    plan.set_tiles(tiles, tiles)
+    assert plan.tiles() == {"c128": tiles, "c64": tiles}
+    T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, om, ref)
+    kh = osl.slab_kernels(M, Nz, 1.0, om, ref)
+    rng = np.random.default_rng(8)
+    n = Nz * M * M * 9
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y_ref = osl.slab_matvec(x, T, kh, M, Nz)
+    y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    assert _rel(x - y, x - y_ref) < 1e-11
+    y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
+    assert _rel(x - y64, x - y_ref) < 2e-5
+
+
+def test_autotune_and_graph_solves_are_reproducible(hip_device):
+    """Autotuned tiles stay valid, and repeated graph-replayed solves give
+    bit-identical results (deterministic reductions, replay-safe graphs)."""
+    msc, ss = _mods()
+    M, Nz = 16, 6
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=5)
+    plan, g, r, mat = _plan(M, Nz, 1.0, REF_C, 0.4 * REF_C.beta, Dl, Dm, Dr)
+    plan.autotune("c128")
+    plan.autotune("c64")
+    t = plan.tiles()
+    assert all(v in (1, 2, 4, 8) for v in t["c128"] + t["c64"])
+    psi0 = plan.incident_field(0.4 * REF_C.beta, r, [0.893, 0.45, 0.0], "P")
+    outs = []
+    for prec in ("fp64", "mixed", "fp64", "mixed"):
+        x, st = plan.gmres(psi0, "born", tol=1e-10, max_iter=200, restart=30, precision=prec)
+        assert st["converged"], st
+        outs.append(x.cpu().numpy())
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
+    assert _rel(outs[1], outs[0]) < 1e-6
