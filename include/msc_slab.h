@@ -51,6 +51,7 @@ typedef enum msc_x0 {
 
 typedef struct msc_plan msc_plan;
 typedef struct msc_arnoldi msc_arnoldi;
+typedef struct msc_net msc_net; /* learned preconditioner (msc_precond.h)     */
 
 typedef struct msc_gmres_opts {
   int restart;       /* Krylov dimension per cycle (>= 1)                    */
@@ -61,6 +62,8 @@ typedef struct msc_gmres_opts {
   double inner_tol;  /* MIXED: relative reduction asked of each c64 cycle    */
   int orth;          /* 0 adaptive CGS (reorthogonalise when needed),       */
                      /* 1 CGS once, 2 CGS twice                              */
+  msc_net* precond;  /* NULL: GMRES; else flexible GMRES with this learned   */
+                     /* right preconditioner (z_j = P(v_j), x = x0 + Z y)    */
 } msc_gmres_opts;
 
 typedef struct msc_gmres_stats {
@@ -69,7 +72,10 @@ typedef struct msc_gmres_stats {
   int matvecs_c128;    /* complex128 operator applications                   */
   int matvecs_c64;     /* complex64 operator applications                    */
   int converged;       /* 1 if rel_residual <= tol                           */
-  int fell_back;       /* MIXED: 1 if it switched to FP64 cycles             */
+  int fell_back;       /* bit 0: MIXED switched to FP64 cycles; bit 1: the   */
+                       /* learned preconditioner stalled and plain GMRES    */
+                       /* cycles finished the solve                          */
+  int precond_applies; /* learned-preconditioner applications                */
   double rel_residual; /* final true FP64 ||b - A x|| / ||b||               */
   double bnorm;        /* ||b||                                              */
 } msc_gmres_stats;

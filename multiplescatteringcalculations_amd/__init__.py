@@ -26,6 +26,7 @@ from .slab_scattering import (
     slab_rpp_periodic,
     uniform_slab_material,
 )
+from .learned_precond import LearnedPreconditioner, harvest_pairs, merge_pairs
 from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
 from .solver_config import load_config, run_from_config
 from .torch_gmres import get_device, select_dtype, to_numpy, to_torch, torch_gmres
@@ -37,4 +38,5 @@ __all__ = [
     "random_iid_slab_material", "random_slab_material", "slab_reflected_field",
     "slab_rpp_periodic", "uniform_slab_material", "SlabOperator", "load_config",
     "run_from_config", "get_device", "select_dtype", "to_numpy", "to_torch", "torch_gmres",
+    "LearnedPreconditioner", "harvest_pairs", "merge_pairs",
 ]

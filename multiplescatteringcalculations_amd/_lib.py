@@ -59,18 +59,33 @@ SIGNATURES = {
     "msc_arnoldi_solution": (c_int, [c_void_p, c_void_p, c_longlong, c_void_p, c_void_p]),
     "msc_timing_enable": (c_int, [c_void_p, c_int]),
     "msc_timing_read": (c_int, [c_void_p, POINTER(c_double)]),
+    # learned preconditioner (include/msc_precond.h)
+    "msc_net_create": (c_int, [POINTER(c_void_p), c_int]),
+    "msc_net_destroy": (c_int, [c_void_p]),
+    "msc_net_num_params": (c_int, [c_void_p, POINTER(c_longlong)]),
+    "msc_net_set_params": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "msc_net_get_params": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "msc_net_input": (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p]),
+    "msc_net_forward": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "msc_net_gradients": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                  c_void_p, POINTER(c_double), c_void_p]),
+    "msc_net_train_step": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                   c_double, POINTER(c_double), c_void_p]),
+    "msc_gmres_basis": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "msc_net_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
 }
 
 
 class GmresOpts(ctypes.Structure):
     _fields_ = [("restart", c_int), ("max_iter", c_int), ("tol", c_double), ("x0", c_int),
-                ("mode", c_int), ("inner_tol", c_double), ("orth", c_int)]
+                ("mode", c_int), ("inner_tol", c_double), ("orth", c_int),
+                ("precond", c_void_p)]
 
 
 class GmresStats(ctypes.Structure):
     _fields_ = [("iterations", c_int), ("cycles", c_int), ("matvecs_c128", c_int),
                 ("matvecs_c64", c_int), ("converged", c_int), ("fell_back", c_int),
-                ("rel_residual", c_double), ("bnorm", c_double)]
+                ("precond_applies", c_int), ("rel_residual", c_double), ("bnorm", c_double)]
 
 
 _lib = None

@@ -10,8 +10,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/msc_slab.h"
+#include "../../include/msc_precond.h"
 #include "msc_gmres.hpp"
+#include "msc_net.hpp"
 #include "msc_plan.hpp"
 #include "msc_setup.hpp"
 
@@ -149,8 +150,13 @@ static void cube_coefficients(double omega, double a, double alpha, double beta,
 
 using namespace msc;
 
+struct msc_net {
+  Net n;
+};
+
 struct msc_plan {
   Plan p;
+  Net* net = nullptr;  // learned preconditioner of the running solve (flexible GMRES)
   Krylov k128;
   Krylov k64;
   double2* r128 = nullptr;  // residual buffer
@@ -158,6 +164,7 @@ struct msc_plan {
   float2* r64 = nullptr;    // complex64 residual
   hipStream_t cap = nullptr;  // private stream for HIP graph capture
   bool use_graphs = true;
+  float* scal = nullptr;      // device scalars of msc_net_input
 };
 
 struct msc_arnoldi {
@@ -184,6 +191,7 @@ int alloc_plan_buffers(Plan& p) {
   MSC_HIP_CHECK(hipMalloc(&p.table64, tab * sizeof(float2)));
   MSC_HIP_CHECK(hipMalloc(&p.tp128, p.nvox * 4 * sizeof(double2)));
   MSC_HIP_CHECK(hipMalloc(&p.tp64, p.nvox * 4 * sizeof(float2)));
+  MSC_HIP_CHECK(hipMalloc(&p.tmax, sizeof(unsigned int)));
   return kOk;
 }
 
@@ -194,6 +202,8 @@ void free_plan_buffers(Plan& p) {
   if (p.table64) (void)hipFree(p.table64);
   if (p.tp128) (void)hipFree(p.tp128);
   if (p.tp64) (void)hipFree(p.tp64);
+  if (p.tmax) (void)hipFree(p.tmax);
+  p.tmax = nullptr;
   p.buf1 = p.buf2 = nullptr;
   p.table128 = nullptr;
   p.table64 = nullptr;
@@ -218,10 +228,21 @@ namespace {
 
 // Launch the work of Arnoldi step j (operator on B = G0 T0 and the
 // orthogonalisation) on stream s.
+// Flexible GMRES (a learned preconditioner is set): z_j = P(v_j) (v_j has unit
+// norm, so the network's input scale is sqrt(n)), then V[j+1] = A z_j.
 template <typename R>
 int step_launches(msc_plan* pl, Krylov& k, int j, int orth, hipStream_t s) {
   cx<R>* V = reinterpret_cast<cx<R>*>(k.V);
-  int rc = scattered<R>(pl->p, V + (size_t)j * k.ldv, V + (size_t)(j + 1) * k.ldv, s, &k.st->done);
+  const cx<R>* vj = V + (size_t)j * k.ldv;
+  cx<R>* w = V + (size_t)(j + 1) * k.ldv;
+  int rc;
+  if (pl->net != nullptr) {
+    cx<R>* zj = reinterpret_cast<cx<R>*>(k.Z) + (size_t)j * k.ldv;
+    rc = net_apply<R>(*pl->net, pl->p, vj, zj, std::sqrt((double)pl->p.n), s, &k.st->done);
+    if (!rc) rc = matvec<R>(pl->p, zj, w, s, &k.st->done);
+  } else {
+    rc = scattered<R>(pl->p, vj, w, s, &k.st->done);
+  }
   if (rc) return rc;
   return krylov_step<R>(k, j, orth, s);
 }
@@ -229,9 +250,11 @@ int step_launches(msc_plan* pl, Krylov& k, int j, int orth, hipStream_t s) {
 // The instantiated graph of step j (captured once per Krylov buffer set).
 template <typename R>
 int step_graph(msc_plan* pl, Krylov& k, int j, int orth, hipGraphExec_t* out) {
-  if (k.graph_orth != orth) {
+  const unsigned long long key = pl->net != nullptr ? pl->net->gen : 0ull;
+  if (k.graph_orth != orth || k.graph_net != key) {
     k.drop_graphs();
     k.graph_orth = orth;
+    k.graph_net = key;
   }
   if ((int)k.graphs.size() <= j) k.graphs.resize(j + 1, nullptr);
   if (k.graphs[j] == nullptr) {
@@ -324,12 +347,33 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
   int rc = pl->k128.ensure(n, m, sizeof(double2));
   if (rc) return rc;
   Krylov& k = pl->k128;
-  k.shifted = true;
+  Net* net = o.precond != nullptr ? &o.precond->n : nullptr;
+  if (net != nullptr) {
+    MSC_REQUIRE(net->packed, "preconditioner parameters are not set");
+    MSC_REQUIRE(net->device == p.device, "preconditioner and plan are on different devices");
+    rc = net->ensure_grid(p.Nz, p.M, p.M);
+    if (rc) return rc;
+    rc = k.ensure_z();
+    if (rc) return rc;
+  }
+  // the network of the running solve; cleared on every exit
+  struct NetScope {
+    msc_plan* pl;
+    ~NetScope() { pl->net = nullptr; }
+  } scope{pl};
+  pl->net = net;
+  // plain GMRES builds its basis from B = G0 T0 (A v = v - B v); flexible GMRES
+  // needs A z_j itself
+  k.shifted = net == nullptr;
   bool mixed = o.mode == MSC_SOLVE_MIXED;
   if (mixed) {
     rc = pl->k64.ensure(n, m, sizeof(float2));
     if (rc) return rc;
-    pl->k64.shifted = true;
+    pl->k64.shifted = net == nullptr;
+    if (net != nullptr) {
+      rc = pl->k64.ensure_z();
+      if (rc) return rc;
+    }
     if (pl->b64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->b64, n * sizeof(float2)));
     if (pl->r64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r64, n * sizeof(float2)));
   }
@@ -408,12 +452,22 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
     } else {
       out.rel_residual = rel;
       if (!(rel > o.tol) || out.iterations >= o.max_iter) break;
-      if (mixed && out.cycles > 0 && rel > 0.9 * prev_rel) {
-        // a complex64 cycle failed to make progress: finish with FP64 cycles
-        mixed = false;
-        out.fell_back = 1;
-        rc = krylov_norm<double>(k, r, &k.st->rnorm, st);
-        if (rc) return rc;
+      if (out.cycles > 0 && rel > 0.9 * prev_rel) {
+        if (pl->net != nullptr) {
+          // the learned preconditioner stalled: plain GMRES cycles take over
+          pl->net = nullptr;
+          k.shifted = true;
+          pl->k64.shifted = true;
+          out.fell_back |= 2;
+        } else if (mixed) {
+          // a complex64 cycle failed to make progress: finish with FP64 cycles
+          mixed = false;
+          out.fell_back |= 1;
+        }
+        if (!mixed) {
+          rc = krylov_norm<double>(k, r, &k.st->rnorm, st);
+          if (rc) return rc;
+        }
       }
     }
     prev_rel = rel;
@@ -434,18 +488,20 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
       if (rc) return rc;
       rc = run_cycle<float>(pl, k6, steps, o.orth, st, &taken);
       if (rc) return rc;
-      rc = krylov_finish<float, double>(k6, x, &k6.st->rnorm, st);
+      rc = krylov_finish<float, double>(k6, x, &k6.st->rnorm, st, pl->net != nullptr);
       if (rc) return rc;
       out.matvecs_c64 += taken;
+      if (pl->net != nullptr) out.precond_applies += taken;
     } else {
       *k.done_flag = 0;
       rc = krylov_cycle_init<double, double>(k, r, o.tol * bnorm, false, st);
       if (rc) return rc;
       rc = run_cycle<double>(pl, k, steps, o.orth, st, &taken);
       if (rc) return rc;
-      rc = krylov_finish<double, double>(k, x, nullptr, st);
+      rc = krylov_finish<double, double>(k, x, nullptr, st, pl->net != nullptr);
       if (rc) return rc;
       out.matvecs_c128 += taken;
+      if (pl->net != nullptr) out.precond_applies += taken;
     }
     out.iterations += taken;
     ++out.cycles;
@@ -538,6 +594,7 @@ int msc_plan_destroy(msc_plan* plan) {
   if (plan->b64) (void)hipFree(plan->b64);
   if (plan->cap) (void)hipStreamDestroy(plan->cap);
   if (plan->r64) (void)hipFree(plan->r64);
+  if (plan->scal) (void)hipFree(plan->scal);
   for (hipEvent_t e : plan->p.ev_pool) (void)hipEventDestroy(e);
   delete plan;
   return kOk;
@@ -838,6 +895,142 @@ int msc_timing_read(msc_plan* plan, double* out4) {
   out4[2] = (double)n128;
   out4[3] = ms128;
   return kOk;
+}
+
+// ── learned preconditioner (msc_precond.h) ──────────────────────────────────
+
+int msc_net_create(msc_net** net, int hidden) {
+  MSC_REQUIRE(net != nullptr, "net must not be NULL");
+  *net = nullptr;
+  msc_net* h = new (std::nothrow) msc_net();
+  MSC_REQUIRE(h != nullptr, "out of host memory");
+  int rc = net_init(h->n, hidden);
+  if (!rc) rc = net_pack(h->n, nullptr);
+  if (!rc) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      set_error(std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+      rc = kHip;
+    }
+  }
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *net = h;
+  return kOk;
+}
+
+int msc_net_destroy(msc_net* net) {
+  if (net == nullptr) return kOk;
+  (void)hipDeviceSynchronize();
+  delete net;
+  return kOk;
+}
+
+int msc_net_num_params(const msc_net* net, long long* n) {
+  MSC_REQUIRE(net != nullptr && n != nullptr, "NULL argument");
+  *n = (long long)net->n.nparams;
+  return kOk;
+}
+
+int msc_net_set_params(msc_net* net, const float* params, void* stream) {
+  MSC_REQUIRE(net != nullptr && params != nullptr, "NULL argument");
+  MSC_HIP_CHECK(hipMemcpyAsync(net->n.params, params, net->n.nparams * sizeof(float),
+                               hipMemcpyDeviceToDevice, S(stream)));
+  return net_pack(net->n, S(stream));
+}
+
+int msc_net_get_params(const msc_net* net, float* params, void* stream) {
+  MSC_REQUIRE(net != nullptr && params != nullptr, "NULL argument");
+  MSC_HIP_CHECK(hipMemcpyAsync(params, net->n.params, net->n.nparams * sizeof(float),
+                               hipMemcpyDeviceToDevice, S(stream)));
+  return kOk;
+}
+
+int msc_net_input(msc_plan* plan, const void* r, int precision, double scale, void* out_bf16,
+                  void* stream) {
+  MSC_REQUIRE(plan != nullptr && r != nullptr && out_bf16 != nullptr, "NULL argument");
+  if (!plan->p.have_tparams) {
+    set_error("plan needs msc_set_tmatrix before msc_net_input");
+    return kNotReady;
+  }
+  if (plan->scal == nullptr) MSC_HIP_CHECK(hipMalloc(&plan->scal, 4 * sizeof(float)));
+  uint16_t* out = reinterpret_cast<uint16_t*>(out_bf16);
+  if (precision == MSC_C128)
+    return net_input<double>(plan->p, reinterpret_cast<const double2*>(r), scale, plan->scal,
+                             out, S(stream));
+  if (precision == MSC_C64)
+    return net_input<float>(plan->p, reinterpret_cast<const float2*>(r), scale, plan->scal, out,
+                            S(stream));
+  MSC_REQUIRE(false, "precision must be 0 (complex128) or 1 (complex64)");
+}
+
+int msc_net_forward(msc_net* net, int D, int H, int W, const void* in_bf16, float* out,
+                    void* stream) {
+  MSC_REQUIRE(net != nullptr && in_bf16 != nullptr && out != nullptr, "NULL argument");
+  MSC_REQUIRE(D >= 1 && H >= 1 && W >= 1, "grid dimensions must be >= 1");
+  return net_forward_out(net->n, D, H, W, reinterpret_cast<const uint16_t*>(in_bf16), out,
+                         S(stream));
+}
+
+int msc_net_gradients(msc_net* net, int D, int H, int W, int batch, const void* in_bf16,
+                      const float* target, float* grads, double* loss, void* stream) {
+  MSC_REQUIRE(net != nullptr && in_bf16 != nullptr && target != nullptr && grads != nullptr,
+              "NULL argument");
+  MSC_REQUIRE(D >= 1 && H >= 1 && W >= 1, "grid dimensions must be >= 1");
+  int rc = net_gradients(net->n, D, H, W, batch, reinterpret_cast<const uint16_t*>(in_bf16),
+                         target, loss, S(stream));
+  if (rc) return rc;
+  MSC_HIP_CHECK(hipMemcpyAsync(grads, net->n.grads, net->n.nparams * sizeof(float),
+                               hipMemcpyDeviceToDevice, S(stream)));
+  return kOk;
+}
+
+int msc_net_train_step(msc_net* net, int D, int H, int W, int batch, const void* in_bf16,
+                       const float* target, double lr, double* loss, void* stream) {
+  MSC_REQUIRE(net != nullptr && in_bf16 != nullptr && target != nullptr, "NULL argument");
+  MSC_REQUIRE(D >= 1 && H >= 1 && W >= 1, "grid dimensions must be >= 1");
+  MSC_REQUIRE(lr > 0.0 && std::isfinite(lr), "lr must be positive and finite");
+  int rc = net_gradients(net->n, D, H, W, batch, reinterpret_cast<const uint16_t*>(in_bf16),
+                         target, loss, S(stream));
+  if (rc) return rc;
+  return net_adam(net->n, lr, S(stream));
+}
+
+int msc_gmres_basis(msc_plan* plan, int precision, int j, void* out, void* stream) {
+  MSC_REQUIRE(plan != nullptr && out != nullptr, "NULL argument");
+  MSC_REQUIRE(precision == MSC_C128 || precision == MSC_C64,
+              "precision must be 0 (complex128) or 1 (complex64)");
+  const Krylov& k = precision == MSC_C128 ? plan->k128 : plan->k64;
+  if (k.V == nullptr || !k.owns_V) {
+    set_error("no GMRES basis of this precision: run msc_gmres first");
+    return kNotReady;
+  }
+  MSC_REQUIRE(j >= 0 && j <= k.m, "j must be in [0, restart]");
+  const size_t eb = (size_t)k.elem_bytes;
+  MSC_HIP_CHECK(hipMemcpyAsync(out, reinterpret_cast<const char*>(k.V) + (size_t)j * k.ldv * eb,
+                               k.n * eb, hipMemcpyDeviceToDevice, S(stream)));
+  return kOk;
+}
+
+int msc_net_apply(msc_net* net, msc_plan* plan, const void* r, void* z, int precision,
+                  void* stream) {
+  MSC_REQUIRE(net != nullptr && plan != nullptr && r != nullptr && z != nullptr,
+              "NULL argument");
+  MSC_REQUIRE(r != z, "r and z must not alias");
+  if (!plan->p.have_tparams) {
+    set_error("plan needs msc_set_tmatrix before msc_net_apply");
+    return kNotReady;
+  }
+  MSC_REQUIRE(net->n.device == plan->p.device, "network and plan are on different devices");
+  if (precision == MSC_C128)
+    return net_apply<double>(net->n, plan->p, reinterpret_cast<const double2*>(r),
+                             reinterpret_cast<double2*>(z), -1.0, S(stream));
+  if (precision == MSC_C64)
+    return net_apply<float>(net->n, plan->p, reinterpret_cast<const float2*>(r),
+                            reinterpret_cast<float2*>(z), -1.0, S(stream));
+  MSC_REQUIRE(false, "precision must be 0 (complex128) or 1 (complex64)");
 }
 
 }  // extern "C"

@@ -433,6 +433,7 @@ void Krylov::drop_graphs() {
     if (e) (void)hipGraphExecDestroy(e);
   graphs.clear();
   graph_orth = -1;
+  graph_net = 0;
 }
 
 void Krylov::release() {
@@ -441,6 +442,8 @@ void Krylov::release() {
   done_flag = nullptr;
   if (V && owns_V) (void)hipFree(V);
   owns_V = false;
+  if (Z) (void)hipFree(Z);
+  Z = nullptr;
   if (st) (void)hipFree(st);
   if (H) (void)hipFree(H);
   V = nullptr;
@@ -491,8 +494,17 @@ int Krylov::ensure(size_t n_, int m_, int elem_bytes_) {
   drop_graphs();
   if (V != nullptr && owns_V) (void)hipFree(V);
   V = nullptr;
+  if (Z != nullptr) (void)hipFree(Z);
+  Z = nullptr;
   MSC_HIP_CHECK(hipMalloc(&V, (size_t)(m + 1) * ldv * elem_bytes));
   owns_V = true;
+  return kOk;
+}
+
+int Krylov::ensure_z() {
+  if (Z != nullptr) return kOk;
+  MSC_REQUIRE(st != nullptr, "Krylov workspace not allocated");
+  MSC_HIP_CHECK(hipMalloc(&Z, (size_t)m * ldv * elem_bytes));
   return kOk;
 }
 
@@ -539,10 +551,12 @@ int krylov_cycle_init(Krylov& k, const cx<RS>* r, double tol_abs, bool unit_rhs,
 }
 
 template <typename R, typename RX>
-int krylov_finish(Krylov& k, cx<RX>* x, const double* scale_dev, hipStream_t s) {
+int krylov_finish(Krylov& k, cx<RX>* x, const double* scale_dev, hipStream_t s, bool flexible) {
+  MSC_REQUIRE(!flexible || k.Z != nullptr, "flexible GMRES without a Z basis");
   hipLaunchKernelGGL(k_backsub, dim3(1), dim3(kRT), 0, s, k.st, k.H, k.ldh, k.g, k.y, k.tmp);
   hipLaunchKernelGGL((k_axpy_basis<R, RX>), dim3(grid_for(k.n)), dim3(kRT), 0, s,
-                     reinterpret_cast<const cx<R>*>(k.V), k.ldv, k.y, k.st, x, k.n, scale_dev);
+                     reinterpret_cast<const cx<R>*>(flexible ? k.Z : k.V), k.ldv, k.y, k.st, x,
+                     k.n, scale_dev);
   MSC_LAUNCH_CHECK();
   return kOk;
 }
@@ -567,8 +581,8 @@ template int krylov_norm<float>(Krylov&, const float2*, double*, hipStream_t);
 template int krylov_cycle_init<double, double>(Krylov&, const double2*, double, bool, hipStream_t);
 template int krylov_cycle_init<double, float>(Krylov&, const double2*, double, bool, hipStream_t);
 template int krylov_cycle_init<float, float>(Krylov&, const float2*, double, bool, hipStream_t);
-template int krylov_finish<double, double>(Krylov&, double2*, const double*, hipStream_t);
-template int krylov_finish<float, double>(Krylov&, double2*, const double*, hipStream_t);
-template int krylov_finish<float, float>(Krylov&, float2*, const double*, hipStream_t);
+template int krylov_finish<double, double>(Krylov&, double2*, const double*, hipStream_t, bool);
+template int krylov_finish<float, double>(Krylov&, double2*, const double*, hipStream_t, bool);
+template int krylov_finish<float, float>(Krylov&, float2*, const double*, hipStream_t, bool);
 
 }  // namespace msc

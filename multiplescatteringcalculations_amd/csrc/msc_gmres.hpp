@@ -40,11 +40,13 @@ struct Krylov {
   void* host = nullptr;       // pinned staging of KState
   int* done_flag = nullptr;   // pinned, device-mapped mirror of KState::done
   bool owns_V = false;
+  void* Z = nullptr;          // flexible GMRES: m x ldv preconditioned vectors z_j = P(v_j)
   bool shifted = false;  // basis built from B = G0 T0, operator A = I - B
   // instantiated HIP graphs of whole Arnoldi steps (operator + orthogonalisation),
   // indexed by step; valid while the buffers and `graph_orth` are unchanged
   std::vector<hipGraphExec_t> graphs;
   int graph_orth = -1;
+  unsigned long long graph_net = 0;  // generation of the preconditioner the graphs hold (0: none)
   void drop_graphs();
 
   ~Krylov();
@@ -52,6 +54,8 @@ struct Krylov {
   // state only (caller-owned basis) / state plus an owned basis
   int ensure_state(size_t n, int m, int elem_bytes);
   int ensure(size_t n, int m, int elem_bytes);
+  // allocate Z (flexible GMRES) for the current n, m, precision
+  int ensure_z();
 };
 
 template <typename R>
@@ -60,8 +64,10 @@ template <typename R>
 int krylov_norm(Krylov& k, const cx<R>* r, double* out_dev, hipStream_t s);
 template <typename RS, typename R>
 int krylov_cycle_init(Krylov& k, const cx<RS>* r, double tol_abs, bool unit_rhs, hipStream_t s);
+// x += (scale) * sum_{v<j} y_v B_v with B = V (GMRES) or Z (flexible GMRES)
 template <typename R, typename RX>
-int krylov_finish(Krylov& k, cx<RX>* x, const double* scale_dev, hipStream_t s);
+int krylov_finish(Krylov& k, cx<RX>* x, const double* scale_dev, hipStream_t s,
+                  bool flexible = false);
 int krylov_read_state(Krylov& k, KState* out, hipStream_t s);
 int krylov_narrow(const double2* in, float2* out, size_t n, hipStream_t s);
 

@@ -41,6 +41,8 @@ struct Plan {
   float2* table64 = nullptr;
   double2* tp128 = nullptr;
   float2* tp64 = nullptr;
+  // max over voxels of |t_u|, |t_d|, |t_o|, |t_s| (device, float bits; set with T0)
+  unsigned int* tmax = nullptr;
   void* buf1 = nullptr;  // (Nz, M, Pxy, 9) complex128 bytes
   void* buf2 = nullptr;  // (Nz, Pxy, Pxy, 9) complex128 bytes
   double2* tw128 = nullptr;

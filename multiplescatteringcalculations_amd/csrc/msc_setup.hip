@@ -10,6 +10,7 @@
 // * T0: the closed-form Rayleigh cube T-matrix reduced to 4 complex numbers
 //   per voxel (effective_contrasts.py:360-451, resonance_tmatrix.py:461-487).
 // * psi0: plane-wave incident field (slab_scattering.py:426-478).
+#include <algorithm>
 #include <cmath>
 
 #include "msc_fft.hpp"
@@ -407,6 +408,29 @@ int build_table(Plan& p, double omega, double alpha, double beta, double rho, hi
   return kOk;
 }
 
+// *tmax = max |t| over the 4 parameters of every voxel, as float bits
+// (non-negative floats order like their bit patterns, so atomicMax works).
+__global__ __launch_bounds__(256) void k_tmax_clear(unsigned int* tmax) {
+  if (threadIdx.x == 0) *tmax = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_tmax(const double2* __restrict__ tp, size_t n,
+                                              unsigned int* __restrict__ tmax) {
+  float m = 0.f;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    m = fmaxf(m, (float)sqrt(tp[i].x * tp[i].x + tp[i].y * tp[i].y));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(tmax, __float_as_uint(m));
+}
+
+static int update_tmax(Plan& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_tmax_clear, dim3(1), dim3(64), 0, st, p.tmax);
+  hipLaunchKernelGGL(k_tmax, dim3(std::min(grid1d(p.nvox * 4, 256), 1024)), dim3(256), 0, st,
+                     p.tp128, p.nvox * 4, p.tmax);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
 int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, const double* Dr,
                                double omega, const double* coef8, hipStream_t st) {
   const double2 G0 = make_double2(coef8[0], coef8[1]);
@@ -418,7 +442,7 @@ int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, cons
                      G0, A, B, C, omega, V, p.tp128, p.tp64);
   MSC_LAUNCH_CHECK();
   p.have_tparams = true;
-  return kOk;
+  return update_tmax(p, st);
 }
 
 int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st) {
@@ -428,7 +452,7 @@ int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st) {
                      p.nvox * 4, p.tp64);
   MSC_LAUNCH_CHECK();
   p.have_tparams = true;
-  return kOk;
+  return update_tmax(p, st);
 }
 
 int incident_field(Plan& p, double k_mag, const double* khat3, const double* inc18, double2* psi0,

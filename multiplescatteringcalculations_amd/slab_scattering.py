@@ -255,14 +255,17 @@ class SlabPlan:
 
     def gmres(self, b: torch.Tensor, x0: str | torch.Tensor = "born", *, tol: float = 1e-6,
               max_iter: int = 500, restart: int | None = None, precision: str = "fp64",
-              inner_tol: float = 1e-4, orth: str = "adaptive") -> tuple[torch.Tensor, dict]:
+              inner_tol: float = 1e-4, orth: str = "adaptive",
+              precond=None) -> tuple[torch.Tensor, dict]:
         """GMRES(restart) on the device; returns (x, stats).
 
         ``precision`` "fp64" (complex128 throughout) or "mixed" (complex64
         cycles inside FP64 iterative refinement; the FP64 true residual judges
         every cycle and a stalled complex64 cycle falls back to FP64).
         ``restart=None`` keeps the reference's unrestarted GMRES (restart =
-        max_iter).
+        max_iter).  ``precond`` (a LearnedPreconditioner) switches to flexible
+        GMRES with that right preconditioner; a stalled flexible cycle hands
+        over to plain GMRES cycles.
         """
         b = _vec_c128(b, self.n, self.device)
         if isinstance(x0, torch.Tensor):
@@ -282,8 +285,11 @@ class SlabPlan:
         if orth_code is None:
             raise ValueError(f"orth must be 'adaptive', 'cgs' or 'cgs2', got '{orth}'")
         m = int(max_iter) if restart is None else int(restart)
+        if precond is not None and torch.device(precond.device) != torch.device(self.device):
+            raise ValueError("preconditioner and plan are on different devices")
         opts = _lib.GmresOpts(max(1, m), int(max_iter), float(tol), mode_x0,
-                              1 if precision == "mixed" else 0, float(inner_tol), orth_code)
+                              1 if precision == "mixed" else 0, float(inner_tol), orth_code,
+                              precond.handle if precond is not None else None)
         st = _lib.GmresStats()
         with torch.cuda.device(self.device):
             _lib.check(self.lib.msc_gmres(self._h, _lib.ptr(b), _lib.ptr(x), ctypes.byref(opts),
@@ -414,7 +420,7 @@ def _build_slab_incident_field(geometry: SlabGeometry, omega: float, ref: Refere
 
 def _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, initial_guess,
            device, *, volume_averaged, n_orders, periodic, restart, precision, inner_tol,
-           orth, pad, warn_label) -> SlabResult:
+           orth, pad, warn_label, precond=None) -> SlabResult:
     if initial_guess not in ("born", "zero"):
         raise ValueError(f"Invalid initial_guess '{initial_guess}'. Must be 'born' or 'zero'.")
     _incident_vector(omega, material.ref, k_hat, wave_type)
@@ -423,7 +429,7 @@ def _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, ini
     plan.set_material(material, omega)
     psi0 = plan.incident_field(omega, material.ref, k_hat, wave_type)
     x, st = plan.gmres(psi0, initial_guess, tol=gmres_tol, max_iter=max_iter, restart=restart,
-                       precision=precision, inner_tol=inner_tol, orth=orth)
+                       precision=precision, inner_tol=inner_tol, orth=orth, precond=precond)
     if not st["converged"]:
         warnings.warn(f"{warn_label} did not converge: rel_res={st['rel_residual']:.2e} "
                       f"after {st['iterations']} iterations", stacklevel=3)
@@ -452,17 +458,19 @@ def compute_slab_scattering(geometry: SlabGeometry, material: SlabMaterial, omeg
                             periodic: bool = False, restart: int | None = 20,
                             precision: str = "fp64", inner_tol: float = 1e-4,
                             orth: str = "adaptive", pad: tuple[int, int] | None = None,
-                            device: torch.device | None = None) -> SlabResult:
+                            device: torch.device | None = None,
+                            preconditioner=None) -> SlabResult:
     """Solve (I - G0 T0) psi = psi0 (slab_scattering.py:486-565) on the HIP device.
 
     Same signature and result as the reference CPU solver: GMRES(20) (SciPy's
     default restart) from the Born guess x0 = psi0, with the true residual
-    reported.  ``max_iter`` caps Arnoldi steps.
+    reported.  ``max_iter`` caps Arnoldi steps.  ``preconditioner`` (a
+    LearnedPreconditioner) selects flexible GMRES.
     """
     return _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, "born",
                   device, volume_averaged=volume_averaged, n_orders=n_orders, periodic=periodic,
                   restart=restart, precision=precision, inner_tol=inner_tol, orth=orth, pad=pad,
-                  warn_label="GMRES")
+                  warn_label="GMRES", precond=preconditioner)
 
 
 # ═══════════════════════════════════════════════════════════════

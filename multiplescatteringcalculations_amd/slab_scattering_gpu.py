@@ -27,7 +27,8 @@ def compute_slab_scattering_gpu(geometry: SlabGeometry, material: SlabMaterial, 
                                 periodic: bool = False, restart: int | None = None,
                                 precision: str | None = None, inner_tol: float = 1e-4,
                                 orth: str = "adaptive",
-                                pad: tuple[int, int] | None = None) -> SlabResult:
+                                pad: tuple[int, int] | None = None,
+                                preconditioner=None) -> SlabResult:
     """Solve (I - G0 T0) psi = psi0 with device GMRES (slab_scattering_gpu.py:158-266).
 
     Reference-compatible arguments: ``device`` must be a HIP device (``cuda``
@@ -35,7 +36,8 @@ def compute_slab_scattering_gpu(geometry: SlabGeometry, material: SlabMaterial, 
     complex128 (default) selects ``precision="fp64"``, complex64 selects the
     mixed form (complex64 operator and basis inside FP64 refinement, so the
     returned residual is still an FP64 one).  ``restart=None`` keeps the
-    reference's unrestarted GMRES.  Non-convergence warns, as the reference does.
+    reference's unrestarted GMRES.  ``preconditioner`` (a LearnedPreconditioner)
+    selects flexible GMRES.  Non-convergence warns, as the reference does.
     """
     if device is None:
         device = _lib.require_device()
@@ -54,7 +56,7 @@ def compute_slab_scattering_gpu(geometry: SlabGeometry, material: SlabMaterial, 
     return _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, initial_guess,
                   device, volume_averaged=volume_averaged, n_orders=n_orders, periodic=periodic,
                   restart=restart, precision=precision, inner_tol=inner_tol, orth=orth, pad=pad,
-                  warn_label="GPU GMRES")
+                  warn_label="GPU GMRES", precond=preconditioner)
 
 
 class SlabOperator:

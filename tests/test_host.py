@@ -1,5 +1,5 @@
 """CPU tests of the host layer: the C ABI library loads and exports every symbol
-include/msc_slab.h declares, its host-only entry points and argument checks,
+include/*.h declares, its host-only entry points and argument checks,
 data types, material generators and the YAML configuration (mirroring
 cubic_scattering/tests/test_slab_scattering.py:50-130 and
 test_solver_config.py:96-230).  No compute call needs a GPU here."""
@@ -18,7 +18,7 @@ ROOT = Path(__file__).resolve().parents[1]
 def test_library_exports_every_declared_symbol():
     from multiplescatteringcalculations_amd import _lib
 
-    header = (ROOT / "include" / "msc_slab.h").read_text()
+    header = "\n".join(h.read_text() for h in sorted((ROOT / "include").glob("*.h")))
     declared = set(re.findall(r"^(?:int|const char\*)\s+(msc_\w+)\(", header, re.M))
     assert declared, "no declarations parsed"
     assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
