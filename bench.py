@@ -9,9 +9,12 @@ GMRES(30) solve of (I - G0 T0) psi = psi0 from the Born guess to
 ||b - A psi|| / ||b|| <= 1e-8 in FP64, with the operator (G0 table, T0) and
 psi0 already resident in HBM.
 
-The headline (``value``) is the mixed-precision solve (complex64 operator and
-Krylov basis inside FP64 iterative refinement); the plain FP64 GMRES(30)
-solve of the same system is timed the same way and reported beside it.
+The headline (``value``) is the learned + mixed solve: flexible GMRES(30) with
+the learned convolutional preconditioner (trained on the GPU, before the timed
+region, on pairs harvested from plain solves of other seeded slabs of the same
+distribution) around complex64 cycles inside FP64 iterative refinement.  The
+plain FP64 GMRES(30) solve and the mixed solve without the network are timed
+the same way and reported beside it.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -36,6 +39,7 @@ M, NZ, A = 128, 32, 1.0
 VP, VS, RHO = 3000.0, 1732.0, 2500.0
 PHI, CONTRAST, KA, P_SLOW = 0.3, 0.2, 0.4, 1.5e-4
 TOL, RESTART = 1e-8, 30
+HIDDEN, TRAIN_SEEDS, TRAIN_STEPS, TRAIN_BATCH, TRAIN_LR = 16, 4, 100, 2, 2e-3
 
 
 def setup(device):
@@ -54,23 +58,64 @@ def setup(device):
     return plan, psi0, mat, omega
 
 
-def time_solves(plan, psi0, precision, steps, warmup):
+def train_preconditioner(plan, mat0, omega, device):
+    """Harvest training pairs from plain FP64 GMRES runs on seeded slabs 1..K of
+    the benchmark distribution (the timed slab is seed 0) and train the network
+    on the GPU.  The plan's G0 table is shared (same geometry and frequency);
+    the timed slab's T0 is restored afterwards."""
+    import torch
+
+    import multiplescatteringcalculations_amd as msc
+    from multiplescatteringcalculations_amd import learned_precond as lp
+
+    ref = msc.ReferenceMedium(VP, VS, RHO)
+    khat = np.array([np.sqrt(1.0 - (P_SLOW * VP) ** 2), P_SLOW * VP, 0.0])
+    t0 = time.perf_counter()
+    sets = []
+    for seed in range(1, TRAIN_SEEDS + 1):
+        m = msc.random_iid_slab_material(plan.geometry, ref, PHI, CONTRAST, seed=seed)
+        plan.set_material(m, omega)
+        b = plan.incident_field(omega, ref, khat, "P").reshape(-1)
+        sets.append(lp.harvest_pairs(plan, b, restart=RESTART, tol=TOL))
+    pairs = lp.merge_pairs(sets)
+    plan.set_material(mat0, omega)
+    torch.cuda.synchronize()
+    t_harvest = time.perf_counter() - t0
+    net = lp.LearnedPreconditioner(HIDDEN, device=device, seed=0)
+    t0 = time.perf_counter()
+    losses, before, after = lp.train(net, pairs, steps=TRAIN_STEPS, lr=TRAIN_LR,
+                                     batch=TRAIN_BATCH, holdout=4)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    info = {"pairs": len(pairs), "harvest_s": round(t_harvest, 3), "train_s": round(t_train, 3),
+            "train_steps": TRAIN_STEPS, "batch": TRAIN_BATCH, "hidden": HIDDEN,
+            "params": net.num_params, "loss_first": losses[0], "loss_last": losses[-1],
+            "holdout_loss_before": before, "holdout_loss_after": after}
+    log(f"learned preconditioner: {len(pairs)} pairs in {t_harvest:.2f} s, {TRAIN_STEPS} Adam "
+        f"steps in {t_train:.2f} s, held-out loss {before:.3e} -> {after:.3e}")
+    return net, info
+
+
+def time_solves(plan, psi0, precision, steps, warmup, precond=None, label=None):
     """K timed solves (HIP-graph Arnoldi steps), after W untimed ones."""
     import torch
 
+    label = label or precision
     st = None
     for _ in range(warmup):
-        _, st = plan.gmres(psi0, "born", tol=TOL, max_iter=600, restart=RESTART, precision=precision)
+        _, st = plan.gmres(psi0, "born", tol=TOL, max_iter=600, restart=RESTART, precision=precision,
+                           precond=precond)
     torch.cuda.synchronize()
     times = []
     for _ in range(steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        x, st = plan.gmres(psi0, "born", tol=TOL, max_iter=600, restart=RESTART, precision=precision)
+        x, st = plan.gmres(psi0, "born", tol=TOL, max_iter=600, restart=RESTART, precision=precision,
+                           precond=precond)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
-        log(f"  {precision}: {times[-1] * 1e3:.2f} ms  iters={st['iterations']} "
-            f"cycles={st['cycles']} rel_res={st['rel_residual']:.2e}")
+        log(f"  {label}: {times[-1] * 1e3:.2f} ms  iters={st['iterations']} "
+            f"cycles={st['cycles']} rel_res={st['rel_residual']:.2e} fell_back={st['fell_back']}")
     return times, st, x
 
 
@@ -141,11 +186,16 @@ def main():
     t_setup = time.perf_counter() - t0
     log(f"setup (plan, G0 table, T0, psi0): {t_setup:.2f} s")
 
+    net, learn_info = train_preconditioner(plan, mat, omega, dev)
     log("plain FP64 GMRES(30):")
     t64, st64, x64 = time_solves(plan, psi0, "fp64", args.steps, args.warmup)
     log("mixed (complex64 cycles in FP64 refinement):")
     tmx, stmx, xmx = time_solves(plan, psi0, "mixed", args.steps, args.warmup)
+    log("learned + mixed (flexible GMRES with the trained network):")
+    tln, stln, xln = time_solves(plan, psi0, "mixed", args.steps, args.warmup, precond=net,
+                                 label="learned+mixed")
     diff = float((xmx - x64).abs().max() / x64.abs().max())
+    diff_ln = float((xln - x64).abs().max() / x64.abs().max())
     log(f"tiles (y-fwd, x-fwd, z, x-inv, y-inv): {plan.tiles()}")
     timmx = time_pass3(plan, psi0, "mixed", args.steps)
     tim64 = time_pass3(plan, psi0, "fp64", args.steps)
@@ -167,6 +217,7 @@ def main():
 
     mean_mx = float(np.mean(tmx))
     mean_64 = float(np.mean(t64))
+    mean_ln = float(np.mean(tln))
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement of the reference CPU matvec, 1 matvec) ...")
@@ -182,25 +233,32 @@ def main():
 
     out = {
         "metric": "s/solve to 1e-8 residual, 128\u00d7128\u00d732 slab: learned+mixed vs plain FP64 GMRES",
-        "value": round(mean_mx, 6),
+        "value": round(mean_ln, 6),
         "unit": "s/solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mean_mx * 1e3, 3),
+        "ms_per_step": round(mean_ln * 1e3, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64+c128",
         "data": "synthetic (seeded random slab, Bernoulli(0.3) inclusions, U[-0.2,0.2] contrasts)",
-        "config": {"workload": "slab 128x128x32, pad 64x256x256, ka=0.4, p=0.15 s/km, GMRES(30), "
-                               "tol 1e-8 FP64; mixed precision (no learned preconditioner yet)",
-                   "M": M, "N_z": NZ, "restart": RESTART, "tol": TOL, "precision": "mixed"},
+        "config": {"workload": "slab 128x128x32, pad 64x256x256, ka=0.4, p=0.15 s/km, tol 1e-8 "
+                               "FP64; flexible GMRES(30) with the learned conv preconditioner "
+                               f"(hidden {HIDDEN}) around complex64 cycles in FP64 refinement",
+                   "M": M, "N_z": NZ, "restart": RESTART, "tol": TOL,
+                   "precision": "mixed", "preconditioner": f"learned conv3d h={HIDDEN}"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": {
             "fp64_s_per_solve": round(mean_64, 6),
-            "speedup_vs_fp64": round(mean_64 / mean_mx, 3),
+            "mixed_s_per_solve": round(mean_mx, 6),
+            "speedup_vs_fp64": round(mean_64 / mean_ln, 3),
+            "mixed_speedup_vs_fp64": round(mean_64 / mean_mx, 3),
+            "learned_iterations": stln["iterations"], "learned_fell_back": stln["fell_back"],
+            "learned_precond_applies": stln["precond_applies"],
+            "learned_vs_fp64_max_rel_diff": diff_ln, "learned": learn_info,
             "fp64_iterations": st64["iterations"], "mixed_iterations": stmx["iterations"],
             "mixed_matvecs_c64": stmx["matvecs_c64"], "mixed_matvecs_c128": stmx["matvecs_c128"],
             "fp64_rel_residual": st64["rel_residual"], "mixed_rel_residual": stmx["rel_residual"],

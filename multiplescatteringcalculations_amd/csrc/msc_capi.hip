@@ -164,7 +164,7 @@ struct msc_plan {
   float2* r64 = nullptr;    // complex64 residual
   hipStream_t cap = nullptr;  // private stream for HIP graph capture
   bool use_graphs = true;
-  float* scal = nullptr;      // device scalars of msc_net_input
+  double* scal = nullptr;     // device scalars of msc_net_input
 };
 
 struct msc_arnoldi {
@@ -955,7 +955,7 @@ int msc_net_input(msc_plan* plan, const void* r, int precision, double scale, vo
     set_error("plan needs msc_set_tmatrix before msc_net_input");
     return kNotReady;
   }
-  if (plan->scal == nullptr) MSC_HIP_CHECK(hipMalloc(&plan->scal, 4 * sizeof(float)));
+  if (plan->scal == nullptr) MSC_HIP_CHECK(hipMalloc(&plan->scal, 4 * sizeof(double)));
   uint16_t* out = reinterpret_cast<uint16_t*>(out_bf16);
   if (precision == MSC_C128)
     return net_input<double>(plan->p, reinterpret_cast<const double2*>(r), scale, plan->scal,

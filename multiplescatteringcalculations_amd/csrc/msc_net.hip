@@ -35,6 +35,14 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
+// The input scale lives on the device as (n, sum |r|^2): s = sqrt(n / sum).
+// A caller-given scale is stored as (s^2, 1); a measured one is accumulated by
+// k_norm_scale into the second slot.
+__device__ __forceinline__ float scale_of(const double* scal) {
+  const double sum = scal[1];
+  return sum > 0.0 ? (float)sqrt(scal[0] / sum) : 1.f;
+}
+
 enum ConvEpi : int { kEpiHidden = 0, kEpiLast = 1, kEpiBackward = 2 };
 
 // EPI kEpiHidden: out bf16 = lrelu(acc + bias); kEpiLast: out fp32 = acc + bias;
@@ -321,13 +329,14 @@ __global__ __launch_bounds__(kNetNT) void k_pack(const float* __restrict__ W,
 template <typename R>
 __global__ __launch_bounds__(kNetNT) void k_net_gather(const cx<R>* __restrict__ r,
                                                        const cx<R>* __restrict__ tp,
-                                                       const float* __restrict__ scal,
+                                                       const double* __restrict__ scal,
                                                        const unsigned int* __restrict__ tmax,
                                                        uint16_t* __restrict__ out, size_t nvox,
                                                        const int* __restrict__ done) {
   if (done != nullptr && *done) return;
   const float tm = __uint_as_float(*tmax);
-  const float s = tm > 0.f ? scal[0] / tm : 0.f;
+  const float s0 = scale_of(scal);
+  const float s = tm > 0.f ? s0 / tm : 0.f;
   for (size_t v = (size_t)blockIdx.x * kNetNT + threadIdx.x; v < nvox;
        v += (size_t)gridDim.x * kNetNT) {
     const cx<R>* t = tp + v * 4;
@@ -370,11 +379,11 @@ __global__ __launch_bounds__(kNetNT) void k_net_gather(const cx<R>* __restrict__
 template <typename R>
 __global__ __launch_bounds__(kNetNT) void k_net_scatter(const cx<R>* __restrict__ r,
                                                         const float* __restrict__ out,
-                                                        const float* __restrict__ scal,
+                                                        const double* __restrict__ scal,
                                                         cx<R>* __restrict__ z, size_t n,
                                                         const int* __restrict__ done) {
   if (done != nullptr && *done) return;
-  const R inv = R(1) / R(scal[0]);
+  const R inv = R(1) / R(scale_of(scal));
   for (size_t e = (size_t)blockIdx.x * kNetNT + threadIdx.x; e < n;
        e += (size_t)gridDim.x * kNetNT) {
     const size_t v = e / 9;
@@ -396,17 +405,20 @@ __global__ __launch_bounds__(kNetNT) void k_compact_out(const float* __restrict_
   }
 }
 
-__global__ void k_set_scale(float* scal, float s) {
-  if (threadIdx.x == 0) scal[0] = s;
+__global__ void k_set_scale(double* scal, double a, double b) {
+  if (threadIdx.x == 0) {
+    scal[0] = a;
+    scal[1] = b;
+  }
 }
 
-// scal[0] = sqrt(n) / ||r|| from per-block partial sums (one workgroup)
 template <typename R>
 __global__ __launch_bounds__(kNetNT) void k_norm_scale(const cx<R>* __restrict__ r, size_t n,
-                                                       float* __restrict__ scal) {
+                                                       double* __restrict__ scal) {
   __shared__ double red[kNetNT / 64];
   double acc = 0.0;
-  for (size_t e = threadIdx.x; e < n; e += kNetNT) {
+  for (size_t e = (size_t)blockIdx.x * kNetNT + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * kNetNT) {
     const double2 a = to_c128(r[e]);
     acc += a.x * a.x + a.y * a.y;
   }
@@ -416,7 +428,7 @@ __global__ __launch_bounds__(kNetNT) void k_norm_scale(const cx<R>* __restrict__
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int i = 0; i < kNetNT / 64; ++i) s += red[i];
-    scal[0] = s > 0.0 ? (float)sqrt((double)n / s) : 1.f;
+    atomicAdd(&scal[1], s);
   }
 }
 
@@ -511,7 +523,7 @@ int net_init(Net& net, int hidden) {
   MSC_HIP_CHECK(hipMalloc(&net.wtr, wp * sizeof(uint16_t)));
   MSC_HIP_CHECK(hipMalloc(&net.bpk, bp * sizeof(float)));
   MSC_HIP_CHECK(hipMalloc(&net.loss, 4 * sizeof(double)));
-  MSC_HIP_CHECK(hipMalloc(&net.scal, 4 * sizeof(float)));
+  MSC_HIP_CHECK(hipMalloc(&net.scal, 4 * sizeof(double)));
   return kOk;
 }
 
@@ -681,13 +693,15 @@ int net_adam(Net& net, double lr, hipStream_t s) {
 }
 
 template <typename R>
-int net_input(const Plan& p, const cx<R>* r, double scale, float* scal, uint16_t* out,
+int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_t* out,
               hipStream_t s, const int* done) {
   MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
   if (scale > 0.0) {
-    hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, (float)scale);
+    hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, scale * scale, 1.0);
   } else {
-    hipLaunchKernelGGL((k_norm_scale<R>), dim3(1), dim3(kNetNT), 0, s, r, p.n, scal);
+    hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, (double)p.n, 0.0);
+    hipLaunchKernelGGL((k_norm_scale<R>), dim3(grid_of(p.n, 2048)), dim3(kNetNT), 0, s, r, p.n,
+                       scal);
   }
   const cx<R>* tp;
   if constexpr (sizeof(R) == 8) {
@@ -716,9 +730,9 @@ int net_apply(Net& net, const Plan& p, const cx<R>* r, cx<R>* z, double scale, h
   return kOk;
 }
 
-template int net_input<double>(const Plan&, const double2*, double, float*, uint16_t*,
+template int net_input<double>(const Plan&, const double2*, double, double*, uint16_t*,
                                hipStream_t, const int*);
-template int net_input<float>(const Plan&, const float2*, double, float*, uint16_t*,
+template int net_input<float>(const Plan&, const float2*, double, double*, uint16_t*,
                               hipStream_t, const int*);
 template int net_apply<double>(Net&, const Plan&, const double2*, double2*, double, hipStream_t,
                                const int*);

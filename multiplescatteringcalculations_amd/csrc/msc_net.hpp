@@ -46,7 +46,7 @@ struct Net {
   float* outf = nullptr;           // last layer output, fp32 [vox][pout[3]]
   uint16_t* gbuf[2] = {};          // gradient ping-pong (bf16), sized for the widest layer
   double* loss = nullptr;          // device loss accumulator (+ partials)
-  float* scal = nullptr;           // device scalars (scale of the current apply)
+  double* scal = nullptr;          // device (n, sum |r|^2) of the current apply: s = sqrt(n/sum)
   bool packed = false;
   // changes whenever device buffers are (re)allocated; captured graphs that
   // reference this network are valid only for one generation
@@ -69,7 +69,7 @@ int net_gradients(Net& net, int D, int H, int W, int batch, const uint16_t* in,
 int net_adam(Net& net, double lr, hipStream_t s);
 // network input (s / t_max) T0 r as bf16 [v][32]; scale <= 0 selects s = sqrt(n) / ||r||
 template <typename R>
-int net_input(const Plan& p, const cx<R>* r, double scale, float* scal, uint16_t* out,
+int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_t* out,
               hipStream_t s, const int* done = nullptr);
 // z = r + N((s / t_max) T0 r) / s on the plan's grid (scale as for net_input); every
 // kernel returns at once while *done is set (stream-ordered early exit)
