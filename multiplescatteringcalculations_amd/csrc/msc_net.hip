@@ -9,13 +9,15 @@
 // One v_mfma_f32_16x16x16_bf16 multiplies a 16 (out channel) x 16 (in
 // channel) weight block by 16 (in channel) x 16 (voxel) activations.  Lane l
 // of a wave holds A[l%16][4(l/16)+0..3], B[4(l/16)+0..3][l%16] and
-// D[4(l/16)+0..3][l%16].  Each wave owns 64 consecutive voxels and every
-// output channel; the weight fragments of a tap are loaded once per tap and
-// reused over the wave's four voxel tiles.
+// D[4(l/16)+0..3][l%16].  A workgroup stages its input halo in LDS; each wave
+// owns 64 consecutive voxels of one row and every output channel, and loads
+// the weight fragments of a tap once for its four voxel tiles.  The last
+// layer of a preconditioner application writes z = r + N / s directly.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <new>
+#include <type_traits>
 
 #include "msc_net.hpp"
 
@@ -43,44 +45,90 @@ __device__ __forceinline__ float scale_of(const double* scal) {
   return sum > 0.0 ? (float)sqrt(scal[0] / sum) : 1.f;
 }
 
-enum ConvEpi : int { kEpiHidden = 0, kEpiLast = 1, kEpiBackward = 2 };
+enum ConvEpi : int {
+  kEpiHidden = 0,    // out bf16 = lrelu(acc + bias)
+  kEpiLast = 1,      // out fp32 = acc + bias
+  kEpiBackward = 2,  // out bf16 = acc * lrelu'(act)  (data gradient through act)
+  kEpiApply64 = 3,   // out complex64  z = r + (acc + bias) / s  (preconditioner output)
+  kEpiApply128 = 4   // out complex128 z = r + (acc + bias) / s
+};
 
-// EPI kEpiHidden: out bf16 = lrelu(acc + bias); kEpiLast: out fp32 = acc + bias;
-// kEpiBackward: out bf16 = acc * lrelu'(act) (data gradient through the
-// activation `act` of the layer input).  MIRROR negates the tap offsets (the
-// data gradient is a convolution with the mirrored, transposed kernel).
-template <int NI, int NO, int EPI, bool MIRROR>
-__global__ __launch_bounds__(kNetNT) void k_conv(const uint16_t* __restrict__ in,
-                                                 const uint16_t* __restrict__ w,
-                                                 const float* __restrict__ bias,
-                                                 const uint16_t* __restrict__ act,
-                                                 void* __restrict__ out, int D, int H, int W,
-                                                 const int* __restrict__ done) {
-  constexpr int NOT = NO / 16, NIB = NI / 16;
-  if (done != nullptr && *done) return;
-  const int lane = threadIdx.x & 63;
-  const int col = lane & 15, grp = lane >> 4;
-  const size_t nvox = (size_t)D * H * W;
-  const size_t v0 = ((size_t)blockIdx.x * (kNetNT / 64) + (threadIdx.x >> 6)) * (16 * kVT);
-  if (v0 >= nvox) return;
-  int vd[kVT], vh[kVT], vw[kVT];
-  bool vok[kVT];
-#pragma unroll
-  for (int t = 0; t < kVT; ++t) {
-    const size_t v = v0 + t * 16 + col;
-    vok[t] = v < nvox;
-    const size_t vv = vok[t] ? v : 0;
-    vw[t] = (int)(vv % W);
-    const size_t q = vv / W;
-    vh[t] = (int)(q % H);
-    vd[t] = (int)(q / H);
+struct ConvArgs {
+  const uint16_t* in;   // [D][H][W][NI] bf16
+  const uint16_t* w;    // [27][NO][NI] bf16
+  const float* bias;    // [NO]
+  const uint16_t* act;  // kEpiBackward: [D][H][W][NO] activation of the layer input
+  void* out;            // [D][H][W][NO] (or [D][H][W][9] complex for kEpiApply*)
+  const void* r;        // kEpiApply*: the preconditioned vector [D][H][W][9] complex
+  const double* scal;   // kEpiApply*: input scale (n, sum |r|^2)
+  const int* done;      // early exit while *done (may be NULL)
+  int D, H, W;
+};
+
+// Output tile of one workgroup: TH rows (x) of TW voxels (y) in one plane (z);
+// wave i computes row i as TW/16 tiles of 16 voxels.  The input halo
+// (3 planes, TH+2 rows, TW+2 columns, NI channels) is staged in LDS once, with
+// the voxel stride padded to NI+4 channels so that the 8-byte fragment reads of
+// 16 consecutive voxels fall on distinct banks; every tap then reads its B
+// fragments from LDS.  MIRROR negates the tap offsets (data gradient).
+constexpr int kTH = kNetNT / 64;  // rows per workgroup (one per wave)
+constexpr int kTW = 16 * kVT;     // voxels per row
+
+// Channels staged per voxel: the 32-channel network input and the last
+// layer's gradient carry only 18 non-zero channels, so 20 are staged and the
+// fragments of channels 20..31 are zero registers.  The LDS voxel stride is an
+// odd multiple of 4 elements (8 bytes), which puts the 8-byte reads of 16
+// consecutive voxels on 16 distinct bank pairs.
+constexpr int kIOStaged = 20;  // staged channels of an 18-channel (padded to 32) input
+
+template <int NS_>
+struct Halo {
+  static constexpr int NS = NS_;
+  static constexpr int LDV = ((NS / 4) % 2 == 1) ? NS : NS + 4;
+  static constexpr int ROWS = kTH + 2, COLS = kTW + 2;
+  static constexpr int ELEMS = 3 * ROWS * COLS * LDV;
+};
+
+// NS: channels of the input that can be non-zero (staged); NI: channels stored.
+template <int NI, int NO, int EPI, bool MIRROR, int NS = NI>
+__global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
+  static_assert(NS % 4 == 0 && NS <= NI, "staged channels must be a multiple of 4");
+  constexpr int NOT = NO / 16, NIB = NI / 16, LDV = Halo<NS>::LDV;
+  constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
+  __shared__ __align__(16) uint16_t sh[Halo<NS>::ELEMS];
+  if (a.done != nullptr && *a.done) return;
+  const int D = a.D, H = a.H, W = a.W;
+  const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
+  int b = blockIdx.x;
+  const int bw = b % nbw;
+  b /= nbw;
+  const int bh = b % nbh;
+  const int d = b / nbh;
+  const int h0 = bh * kTH, w0 = bw * kTW;
+  // stage the halo: 4-channel groups (8 bytes), zero outside the grid
+  constexpr int G4 = NS / 4;
+  constexpr int NSTAGE = 3 * ROWS * COLS * G4;
+  for (int e = threadIdx.x; e < NSTAGE; e += kNetNT) {
+    const int g = e % G4;
+    int q = e / G4;
+    const int c = q % COLS;
+    q /= COLS;
+    const int rr = q % ROWS;
+    const int pl = q / ROWS;
+    const int gd = d + pl - 1, gh = h0 + rr - 1, gw = w0 + c - 1;
+    v4s val = v4s{0, 0, 0, 0};
+    if (gd >= 0 && gd < D && gh >= 0 && gh < H && gw >= 0 && gw < W)
+      val = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI + g * 4);
+    *reinterpret_cast<v4s*>(&sh[((pl * ROWS + rr) * COLS + c) * LDV + g * 4]) = val;
   }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int col = lane & 15, grp = lane >> 4;
   v4f acc[NOT][kVT];
 #pragma unroll
   for (int o = 0; o < NOT; ++o)
 #pragma unroll
     for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
-  const v4s zero4 = v4s{0, 0, 0, 0};
   for (int tap = 0; tap < 27; ++tap) {
     int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
     if (MIRROR) {
@@ -88,62 +136,81 @@ __global__ __launch_bounds__(kNetNT) void k_conv(const uint16_t* __restrict__ in
       dx = -dx;
       dy = -dy;
     }
-    v4s a[NOT][NIB];
+    v4s wf[NOT][NIB];
 #pragma unroll
     for (int o = 0; o < NOT; ++o)
 #pragma unroll
       for (int ib = 0; ib < NIB; ++ib)
-        a[o][ib] = *reinterpret_cast<const v4s*>(w + ((size_t)tap * NO + o * 16 + col) * NI +
-                                                 ib * 16 + grp * 4);
+        wf[o][ib] = *reinterpret_cast<const v4s*>(a.w + ((size_t)tap * NO + o * 16 + col) * NI +
+                                                  ib * 16 + grp * 4);
+    const uint16_t* base =
+        sh + (((1 + dz) * ROWS + (1 + wave + dx)) * COLS + (1 + dy + col)) * LDV + grp * 4;
 #pragma unroll
     for (int t = 0; t < kVT; ++t) {
-      const int nd = vd[t] + dz, nh = vh[t] + dx, nw = vw[t] + dy;
-      const bool ok = vok[t] && nd >= 0 && nd < D && nh >= 0 && nh < H && nw >= 0 && nw < W;
-      const size_t nv = ok ? ((size_t)nd * H + nh) * W + nw : 0;
-      const uint16_t* src = in + nv * NI + grp * 4;
 #pragma unroll
       for (int ib = 0; ib < NIB; ++ib) {
-        const v4s b = ok ? *reinterpret_cast<const v4s*>(src + ib * 16) : zero4;
+        v4s bf = v4s{0, 0, 0, 0};
+        if (ib * 16 + 16 <= NS || ib * 16 + grp * 4 < NS)
+          bf = *reinterpret_cast<const v4s*>(base + t * 16 * LDV + ib * 16);
 #pragma unroll
         for (int o = 0; o < NOT; ++o)
-          acc[o][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[o][ib], b, acc[o][t], 0, 0, 0);
+          acc[o][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[o][ib], bf, acc[o][t], 0, 0, 0);
       }
     }
   }
+  const int h = h0 + wave;
+  if (h >= H) return;
+  float inv_s = 1.f;
+  if constexpr (EPI == kEpiApply64 || EPI == kEpiApply128) inv_s = 1.f / scale_of(a.scal);
 #pragma unroll
   for (int t = 0; t < kVT; ++t) {
-    if (!vok[t]) continue;
-    const size_t v = v0 + t * 16 + col;
+    const int w = w0 + t * 16 + col;
+    if (w >= W) continue;
+    const size_t v = ((size_t)d * H + h) * W + w;
 #pragma unroll
     for (int o = 0; o < NOT; ++o) {
       const int c0 = o * 16 + grp * 4;
       if constexpr (EPI == kEpiLast) {
         float4 r;
-        r.x = acc[o][t][0] + bias[c0 + 0];
-        r.y = acc[o][t][1] + bias[c0 + 1];
-        r.z = acc[o][t][2] + bias[c0 + 2];
-        r.w = acc[o][t][3] + bias[c0 + 3];
-        *reinterpret_cast<float4*>(reinterpret_cast<float*>(out) + v * NO + c0) = r;
+        r.x = acc[o][t][0] + a.bias[c0 + 0];
+        r.y = acc[o][t][1] + a.bias[c0 + 1];
+        r.z = acc[o][t][2] + a.bias[c0 + 2];
+        r.w = acc[o][t][3] + a.bias[c0 + 3];
+        *reinterpret_cast<float4*>(reinterpret_cast<float*>(a.out) + v * NO + c0) = r;
+      } else if constexpr (EPI == kEpiApply64 || EPI == kEpiApply128) {
+        // channels c0..c0+3 are (Re, Im) of components c0/2 and c0/2 + 1
+        using R = typename std::conditional<EPI == kEpiApply64, float, double>::type;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int comp = c0 / 2 + k;
+          if (comp < 9) {
+            const cx<R> rv = reinterpret_cast<const cx<R>*>(a.r)[v * 9 + comp];
+            const float re = acc[o][t][2 * k] + a.bias[c0 + 2 * k];
+            const float im = acc[o][t][2 * k + 1] + a.bias[c0 + 2 * k + 1];
+            reinterpret_cast<cx<R>*>(a.out)[v * 9 + comp] =
+                mk<R>(rv.x + R(inv_s * re), rv.y + R(inv_s * im));
+          }
+        }
       } else {
         float r[4];
         if constexpr (EPI == kEpiHidden) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const float z = acc[o][t][i] + bias[c0 + i];
+            const float z = acc[o][t][i] + a.bias[c0 + i];
             r[i] = z > 0.f ? z : kNetSlope * z;
           }
         } else {
-          const v4s av = *reinterpret_cast<const v4s*>(act + v * NO + c0);
+          const v4s av = *reinterpret_cast<const v4s*>(a.act + v * NO + c0);
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const float a = bf2f((uint16_t)av[i]);
-            r[i] = a > 0.f ? acc[o][t][i] : kNetSlope * acc[o][t][i];
+            const float x = bf2f((uint16_t)av[i]);
+            r[i] = x > 0.f ? acc[o][t][i] : kNetSlope * acc[o][t][i];
           }
         }
         v4s pk;
 #pragma unroll
         for (int i = 0; i < 4; ++i) pk[i] = (short)f2bf(r[i]);
-        *reinterpret_cast<v4s*>(reinterpret_cast<uint16_t*>(out) + v * NO + c0) = pk;
+        *reinterpret_cast<v4s*>(reinterpret_cast<uint16_t*>(a.out) + v * NO + c0) = pk;
       }
     }
   }
@@ -375,25 +442,6 @@ __global__ __launch_bounds__(kNetNT) void k_net_gather(const cx<R>* __restrict__
   }
 }
 
-// z = r + (out[.., 2c] + i out[.., 2c+1]) / s
-template <typename R>
-__global__ __launch_bounds__(kNetNT) void k_net_scatter(const cx<R>* __restrict__ r,
-                                                        const float* __restrict__ out,
-                                                        const double* __restrict__ scal,
-                                                        cx<R>* __restrict__ z, size_t n,
-                                                        const int* __restrict__ done) {
-  if (done != nullptr && *done) return;
-  const R inv = R(1) / R(scale_of(scal));
-  for (size_t e = (size_t)blockIdx.x * kNetNT + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * kNetNT) {
-    const size_t v = e / 9;
-    const int c = (int)(e - v * 9);
-    const float* o = out + v * kNetIOPad + 2 * c;
-    const cx<R> a = r[e];
-    z[e] = mk<R>(a.x + inv * R(o[0]), a.y + inv * R(o[1]));
-  }
-}
-
 // first kNetIO channels of [v][kNetIOPad] -> [v][kNetIO]
 __global__ __launch_bounds__(kNetNT) void k_compact_out(const float* __restrict__ in,
                                                         float* __restrict__ out, size_t nvox) {
@@ -436,11 +484,6 @@ __global__ __launch_bounds__(kNetNT) void k_norm_scale(const cx<R>* __restrict__
 
 static int grid_of(size_t work, int cap = 16384) {
   return (int)std::max<size_t>(1, std::min<size_t>((work + kNetNT - 1) / kNetNT, cap));
-}
-
-static int conv_blocks(size_t nvox) {
-  const size_t waves = (nvox + 16 * kVT - 1) / (16 * kVT);
-  return (int)((waves + kNetNT / 64 - 1) / (kNetNT / 64));
 }
 
 Net::~Net() {
@@ -542,30 +585,66 @@ int net_pack(Net& net, hipStream_t s) {
 
 namespace {
 
-// The four-layer forward pass from act[0] (or `in`) to outf, per hidden width.
+ConvArgs conv_args(const Net& net, const uint16_t* in, const uint16_t* w, const float* bias,
+                   const uint16_t* act, void* out, const int* done) {
+  ConvArgs a{};
+  a.in = in;
+  a.w = w;
+  a.bias = bias;
+  a.act = act;
+  a.out = out;
+  a.done = done;
+  a.D = net.D;
+  a.H = net.H;
+  a.W = net.W;
+  return a;
+}
+
+dim3 conv_grid(const Net& net) {
+  const int nbw = (net.W + kTW - 1) / kTW, nbh = (net.H + kTH - 1) / kTH;
+  return dim3((unsigned)(net.D * nbh * nbw));
+}
+
+// The four-layer pass per hidden width.  forward() ends in outf (fp32);
+// apply() ends in z = r + N/s (complex, R) with the scale of net.scal.
 template <int HID>
 struct NetOps {
+  static void hidden_layers(Net& net, const uint16_t* in, hipStream_t s, const int* done) {
+    const dim3 G = conv_grid(net);
+    hipLaunchKernelGGL((k_conv<kNetIOPad, HID, kEpiHidden, false, kIOStaged>), G, dim3(kNetNT), 0, s,
+                       conv_args(net, in, net.wpk + net.wpad_off[0], net.bpk + net.bpad_off[0],
+                                 nullptr, net.act[1], done));
+    hipLaunchKernelGGL((k_conv<HID, HID, kEpiHidden, false>), G, dim3(kNetNT), 0, s,
+                       conv_args(net, net.act[1], net.wpk + net.wpad_off[1],
+                                 net.bpk + net.bpad_off[1], nullptr, net.act[2], done));
+    hipLaunchKernelGGL((k_conv<HID, HID, kEpiHidden, false>), G, dim3(kNetNT), 0, s,
+                       conv_args(net, net.act[2], net.wpk + net.wpad_off[2],
+                                 net.bpk + net.bpad_off[2], nullptr, net.act[3], done));
+  }
+
   static void forward(Net& net, const uint16_t* in, hipStream_t s, const int* done) {
-    const int B = conv_blocks(net.nvox);
-    const int D = net.D, H = net.H, W = net.W;
-    hipLaunchKernelGGL((k_conv<kNetIOPad, HID, kEpiHidden, false>), dim3(B), dim3(kNetNT), 0, s,
-                       in, net.wpk + net.wpad_off[0], net.bpk + net.bpad_off[0], nullptr,
-                       net.act[1], D, H, W, done);
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiHidden, false>), dim3(B), dim3(kNetNT), 0, s,
-                       net.act[1], net.wpk + net.wpad_off[1], net.bpk + net.bpad_off[1], nullptr,
-                       net.act[2], D, H, W, done);
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiHidden, false>), dim3(B), dim3(kNetNT), 0, s,
-                       net.act[2], net.wpk + net.wpad_off[2], net.bpk + net.bpad_off[2], nullptr,
-                       net.act[3], D, H, W, done);
-    hipLaunchKernelGGL((k_conv<HID, kNetIOPad, kEpiLast, false>), dim3(B), dim3(kNetNT), 0, s,
-                       net.act[3], net.wpk + net.wpad_off[3], net.bpk + net.bpad_off[3], nullptr,
-                       net.outf, D, H, W, done);
+    hidden_layers(net, in, s, done);
+    hipLaunchKernelGGL((k_conv<HID, kNetIOPad, kEpiLast, false>), conv_grid(net), dim3(kNetNT), 0,
+                       s, conv_args(net, net.act[3], net.wpk + net.wpad_off[3],
+                                    net.bpk + net.bpad_off[3], nullptr, net.outf, done));
+  }
+
+  template <typename R>
+  static void apply(Net& net, const cx<R>* r, cx<R>* z, hipStream_t s, const int* done) {
+    hidden_layers(net, net.act[0], s, done);
+    ConvArgs a = conv_args(net, net.act[3], net.wpk + net.wpad_off[3], net.bpk + net.bpad_off[3],
+                           nullptr, z, done);
+    a.r = r;
+    a.scal = net.scal;
+    constexpr int EPI = sizeof(R) == 4 ? kEpiApply64 : kEpiApply128;
+    hipLaunchKernelGGL((k_conv<HID, kNetIOPad, EPI, false>), conv_grid(net), dim3(kNetNT), 0, s,
+                       a);
   }
 
   // Backward of the loss gradient held in gbuf[0] ([v][kNetIOPad]); `in` is
   // the network input.  Accumulates into net.grads.
   static void backward(Net& net, const uint16_t* in, hipStream_t s) {
-    const int B = conv_blocks(net.nvox);
+    const dim3 G = conv_grid(net);
     const int D = net.D, H = net.H, W = net.W;
     const size_t nv = net.nvox;
     const int vpb = 4096;
@@ -575,32 +654,36 @@ struct NetOps {
     uint16_t* g3 = net.gbuf[1];
     uint16_t* g2 = net.gbuf[0];
     uint16_t* g1 = net.gbuf[1];
-    float* G = net.grads;
+    float* Gr = net.grads;
     // layer 4 (HID -> IO)
     hipLaunchKernelGGL((k_conv_wgrad<HID, kNetIOPad>), wg, dim3(kNetNT), 0, s, net.act[3], g4,
-                       G + net.woff[3], net.cin[3], net.cout[3], D, H, W, vpb);
+                       Gr + net.woff[3], net.cin[3], net.cout[3], D, H, W, vpb);
     hipLaunchKernelGGL((k_bias_grad<kNetIOPad>), dim3(gb), dim3(kNetNT), 0, s, g4,
-                       G + net.boff[3], net.cout[3], nv);
-    hipLaunchKernelGGL((k_conv<kNetIOPad, HID, kEpiBackward, true>), dim3(B), dim3(kNetNT), 0, s,
-                       g4, net.wtr + net.wpad_off[3], nullptr, net.act[3], g3, D, H, W, nullptr);
+                       Gr + net.boff[3], net.cout[3], nv);
+    hipLaunchKernelGGL((k_conv<kNetIOPad, HID, kEpiBackward, true, kIOStaged>), G, dim3(kNetNT), 0,
+                       s,
+                       conv_args(net, g4, net.wtr + net.wpad_off[3], nullptr, net.act[3], g3,
+                                 nullptr));
     // layer 3
     hipLaunchKernelGGL((k_conv_wgrad<HID, HID>), wg, dim3(kNetNT), 0, s, net.act[2], g3,
-                       G + net.woff[2], net.cin[2], net.cout[2], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g3, G + net.boff[2],
+                       Gr + net.woff[2], net.cin[2], net.cout[2], D, H, W, vpb);
+    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g3, Gr + net.boff[2],
                        net.cout[2], nv);
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), dim3(B), dim3(kNetNT), 0, s, g3,
-                       net.wtr + net.wpad_off[2], nullptr, net.act[2], g2, D, H, W, nullptr);
+    hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), G, dim3(kNetNT), 0, s,
+                       conv_args(net, g3, net.wtr + net.wpad_off[2], nullptr, net.act[2], g2,
+                                 nullptr));
     // layer 2
     hipLaunchKernelGGL((k_conv_wgrad<HID, HID>), wg, dim3(kNetNT), 0, s, net.act[1], g2,
-                       G + net.woff[1], net.cin[1], net.cout[1], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g2, G + net.boff[1],
+                       Gr + net.woff[1], net.cin[1], net.cout[1], D, H, W, vpb);
+    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g2, Gr + net.boff[1],
                        net.cout[1], nv);
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), dim3(B), dim3(kNetNT), 0, s, g2,
-                       net.wtr + net.wpad_off[1], nullptr, net.act[1], g1, D, H, W, nullptr);
+    hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), G, dim3(kNetNT), 0, s,
+                       conv_args(net, g2, net.wtr + net.wpad_off[1], nullptr, net.act[1], g1,
+                                 nullptr));
     // layer 1 (IO -> HID): weights and bias only
     hipLaunchKernelGGL((k_conv_wgrad<kNetIOPad, HID>), wg, dim3(kNetNT), 0, s, in, g1,
-                       G + net.woff[0], net.cin[0], net.cout[0], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g1, G + net.boff[0],
+                       Gr + net.woff[0], net.cin[0], net.cout[0], D, H, W, vpb);
+    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g1, Gr + net.boff[0],
                        net.cout[0], nv);
   }
 };
@@ -624,6 +707,18 @@ struct Fwd {
 template <int HID>
 struct Bwd {
   static void run(Net& n, const uint16_t* in, hipStream_t s) { NetOps<HID>::backward(n, in, s); }
+};
+template <int HID>
+struct Apply64 {
+  static void run(Net& n, const float2* r, float2* z, hipStream_t s, const int* done) {
+    NetOps<HID>::template apply<float>(n, r, z, s, done);
+  }
+};
+template <int HID>
+struct Apply128 {
+  static void run(Net& n, const double2* r, double2* z, hipStream_t s, const int* done) {
+    NetOps<HID>::template apply<double>(n, r, z, s, done);
+  }
 };
 
 }  // namespace
@@ -723,9 +818,11 @@ int net_apply(Net& net, const Plan& p, const cx<R>* r, cx<R>* z, double scale, h
   if (rc) return rc;
   rc = net_input<R>(p, r, scale, net.scal, net.act[0], s, done);
   if (rc) return rc;
-  by_width<Fwd>(net.hidden, net, net.act[0], s, done);
-  hipLaunchKernelGGL((k_net_scatter<R>), dim3(grid_of(p.n)), dim3(kNetNT), 0, s, r, net.outf,
-                     net.scal, z, p.n, done);
+  if constexpr (sizeof(R) == 4) {
+    by_width<Apply64>(net.hidden, net, r, z, s, done);
+  } else {
+    by_width<Apply128>(net.hidden, net, r, z, s, done);
+  }
   MSC_LAUNCH_CHECK();
   return kOk;
 }
