@@ -44,41 +44,62 @@ __device__ __forceinline__ void swap_xy(cx<R>* t) {
   q = t[7]; t[7] = t[8]; t[8] = q;
 }
 
-// o = K t for the canonical spectral propagator K = [[G, C], [H, S]] held as
-// its 45 independent entries K[] (G sym 6, C 3x6 row-major, upper triangle
-// of S' = S W^-1 21), with H = W^-1 C^T and S = S' W, W = diag(1,1,1,1/2,1/2,1/2).
-template <typename R>
-__device__ __forceinline__ void kmul45(const cx<R>* K, const cx<R>* t, cx<R>* o) {
-  o[0] = cmul<R>(K[0], t[0]);
-  cfma<R>(o[0], K[1], t[1]);
-  cfma<R>(o[0], K[2], t[2]);
-  o[1] = cmul<R>(K[1], t[0]);
-  cfma<R>(o[1], K[3], t[1]);
-  cfma<R>(o[1], K[4], t[2]);
-  o[2] = cmul<R>(K[2], t[0]);
-  cfma<R>(o[2], K[4], t[1]);
-  cfma<R>(o[2], K[5], t[2]);
+// o_p = K t_p for NP vectors sharing one canonical spectral propagator
+// K = [[G, C], [H, S]], given by its 45 independent entries kp[e * stride]:
+// G symmetric (6), C 3x6 row-major (18), the upper triangle of S' = S W^-1 (21),
+// with H = W^-1 C^T, S = S' W and W = diag(1,1,1,1/2,1/2,1/2).  The entries are
+// read one at a time and applied at once to every product.  t_p are already
+// sign-flipped / swapped into the table's octant.
+template <typename R, int NP>
+__device__ __forceinline__ void kmul45_stream(const cx<R>* __restrict__ kp, int stride,
+                                              const cx<R> (&t)[NP][9], cx<R> (&o)[NP][9]) {
 #pragma unroll
-  for (int al = 0; al < 6; ++al) {
-    const cx<R> c0 = K[6 + al], c1 = K[12 + al], c2 = K[18 + al];
-    cfma<R>(o[0], c0, t[3 + al]);
-    cfma<R>(o[1], c1, t[3 + al]);
-    cfma<R>(o[2], c2, t[3 + al]);
-    cx<R> h = cmul<R>(c0, t[0]);
-    cfma<R>(h, c1, t[1]);
-    cfma<R>(h, c2, t[2]);
-    o[3 + al] = al < 3 ? h : cscale<R>(h, R(2));
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) o[p][i] = czero<R>();
+  // G (symmetric 3x3): entries (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const int i = e < 3 ? 0 : (e < 5 ? 1 : 2);
+    const int j = e < 3 ? e : (e < 5 ? e - 2 : 2);
+    const cx<R> kv = kp[e * stride];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      cfma<R>(o[p][i], kv, t[p][j]);
+      if (i != j) cfma<R>(o[p][j], kv, t[p][i]);
+    }
   }
-  cx<R> tw[6];
+  // C (3x6) and H = W^-1 C^T: o_u += C t_e, o_e += (1 or 2) C^T t_u
 #pragma unroll
-  for (int be = 0; be < 6; ++be) tw[be] = be < 3 ? t[3 + be] : cscale<R>(t[3 + be], R(0.5));
+  for (int e = 0; e < 18; ++e) {
+    const int i = e / 6, al = e % 6;
+    const cx<R> kv = kp[(6 + e) * stride];
 #pragma unroll
-  for (int al = 0; al < 6; ++al) {
+    for (int p = 0; p < NP; ++p) {
+      cfma<R>(o[p][i], kv, t[p][3 + al]);
+      cfma<R>(o[p][3 + al], al < 3 ? kv : cscale<R>(kv, R(2)), t[p][i]);
+    }
+  }
+  // S' (symmetric 6x6 upper triangle), S = S' W: o_e += S' (W t_e)
 #pragma unroll
-    for (int be = al; be < 6; ++be) {
-      const cx<R> sv = K[24 + al * 6 - (al * (al - 1)) / 2 + (be - al)];
-      cfma<R>(o[3 + al], sv, tw[be]);
-      if (be != al) cfma<R>(o[3 + be], sv, tw[al]);
+  for (int e = 0; e < 21; ++e) {
+    int al = 0, rem = e;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (rem >= 6 - k && al == k) {
+        rem -= 6 - k;
+        al = k + 1;
+      }
+    const int be = al + rem;
+    const cx<R> kv = kp[(24 + e) * stride];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const cx<R> twb = be < 3 ? t[p][3 + be] : cscale<R>(t[p][3 + be], R(0.5));
+      cfma<R>(o[p][3 + al], kv, twb);
+      if (be != al) {
+        const cx<R> twa = al < 3 ? t[p][3 + al] : cscale<R>(t[p][3 + al], R(0.5));
+        cfma<R>(o[p][3 + be], kv, twa);
+      }
     }
   }
 }
@@ -190,32 +211,32 @@ __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
   }
   __syncthreads();
   lds_fft<R, PZ, false, NTR, LD, kNT>(s, tw);
+  // Block multiply, one (column, kz) per item: the table entries are
+  // streamed (each loaded and applied at once) rather than held in registers.
   const int half = Pxy / 2;
+  constexpr int HZ = PZ / 2;
   for (int w = threadIdx.x; w < PZ * Q; w += kNT) {
     const int q = w / PZ;
     const int kz = w - q * PZ;
     const int ky = ky0 + q;
     const int a = kx <= half ? kx : Pxy - kx;
     const int b = ky <= half ? ky : Pxy - ky;
-    const int c = kz <= PZ / 2 ? kz : PZ - kz;
-    const unsigned flip = (kz > PZ / 2 ? kFlipZ : 0u) ^ (kx > half ? kFlipX : 0u) ^
+    const int c = kz <= HZ ? kz : PZ - kz;
+    const unsigned flip = (kz > HZ ? kFlipZ : 0u) ^ (kx > half ? kFlipX : 0u) ^
                           (ky > half ? kFlipY : 0u);
     const bool sw = a < b;
     const int u = sw ? b : a;
     const int vv = sw ? a : b;
-    const int uv = (u * (u + 1)) / 2 + vv;
+    const cx<R>* kp = table + (size_t)((u * (u + 1)) / 2 + vv) * kNE * NC + c;
     cx<R>* sp = s + kz * LD + q * 9;
-    cx<R> t[9], o[9];
+    cx<R> t[1][9], o[1][9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) t[i] = flip_if<R>(sp[i], flip, i);
-    if (sw) swap_xy<R>(t);
-    cx<R> K[kNE];
+    for (int i = 0; i < 9; ++i) t[0][i] = flip_if<R>(sp[i], flip, i);
+    if (sw) swap_xy<R>(t[0]);
+    kmul45_stream<R, 1>(kp, NC, t, o);
+    if (sw) swap_xy<R>(o[0]);
 #pragma unroll
-    for (int e = 0; e < kNE; ++e) K[e] = table[((size_t)uv * kNE + e) * NC + c];
-    kmul45<R>(K, t, o);
-    if (sw) swap_xy<R>(o);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) sp[i] = flip_if<R>(o[i], flip, i);
+    for (int i = 0; i < 9; ++i) sp[i] = flip_if<R>(o[0][i], flip, i);
   }
   __syncthreads();
   lds_fft<R, PZ, true, NTR, LD, kNT>(s, tw);
@@ -326,8 +347,8 @@ constexpr bool lds_fits() {
 
 int default_tile(int prec, int pass) {
   // pass: 0 y-forward B, 1 x-forward Q, 2 z Q, 3 x-inverse Q, 4 y-inverse B
-  static const int t64[5] = {1, 2, 4, 2, 1};
-  static const int t128[5] = {1, 1, 4, 1, 1};
+  static const int t64[5] = {1, 2, 8, 2, 1};
+  static const int t128[5] = {1, 1, 8, 1, 1};
   return prec == 1 ? t64[pass] : t128[pass];
 }
 
@@ -469,6 +490,9 @@ static int launch_z(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
   int q = p.tile[prec_of<R>()][2];
   while (q > 1 && p.Pxy % q != 0) q >>= 1;
   switch (q) {
+    case 16:
+      if constexpr (lds_fits<R, PZ, 9, 16>()) { z_q<R, PZ, 16>(p, b2, st, done); break; }
+      [[fallthrough]];
     case 8:
       if constexpr (lds_fits<R, PZ, 9, 8>()) { z_q<R, PZ, 8>(p, b2, st, done); break; }
       [[fallthrough]];

@@ -33,7 +33,7 @@ struct Plan {
   double omega = 0.0, alpha = 0.0, beta = 0.0, rho = 0.0;
   // tiles per precision (0 complex128, 1 complex64) and pass (y-forward B,
   // x-forward Q, z Q, x-inverse Q, y-inverse B); see autotune()
-  int tile[2][5] = {{1, 1, 4, 1, 1}, {1, 2, 4, 2, 1}};
+  int tile[2][5] = {{1, 1, 8, 1, 1}, {1, 2, 8, 2, 1}};
   bool tuned[2] = {false, false};
   bool have_table = false, have_tparams = false;
 
