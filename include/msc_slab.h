@@ -80,6 +80,30 @@ typedef struct msc_gmres_stats {
   double bnorm;        /* ||b||                                              */
 } msc_gmres_stats;
 
+/* One independent solve of a sweep (msc_sweep). */
+typedef struct msc_instance {
+  int M, N_z;               /* slab of M x M x N_z voxels                      */
+  int periodic;             /* 1: laterally periodic (M a power of two)        */
+  double a;                 /* cube half-width                                 */
+  double omega;             /* angular frequency                               */
+  double alpha, beta, rho;  /* background medium                               */
+  double k_mag;             /* incident wavenumber (omega / alpha for P)       */
+  double khat[3];           /* incident direction                              */
+  double inc18[18];         /* incident (polarisation, Voigt strain), re/im    */
+  const double* dlambda;    /* device (N_z, M, M) float64 contrasts            */
+  const double* dmu;
+  const double* drho;
+  void* psi;                /* out: device (N_z, M, M, 9) complex128           */
+  void* psi0;               /* out (may be NULL): the incident field, same     */
+  void* layer_sum;          /* out (may be NULL): device (N_z, 9) complex128   */
+  void* phased;             /* out (with layer_sum): device (N_z, 2, 9): sums  */
+                            /* of tau = T0 psi as msc_source_sums with kP, kS  */
+                            /* = omega/alpha, omega/beta and r_hat = -khat     */
+  msc_gmres_stats stats;    /* out                                             */
+  int status;               /* out: msc_status of this instance                */
+  char error[160];          /* out: message when status != MSC_OK              */
+} msc_instance;
+
 /* Last error message of the calling thread ("" if none). */
 const char* msc_last_error(void);
 
@@ -97,6 +121,38 @@ const char* msc_version(void);
  */
 int msc_plan_create(msc_plan** plan, int M, int N_z, double a, int periodic, int pz, int pxy);
 int msc_plan_destroy(msc_plan* plan);
+
+/*
+ * Re-target a plan to a smaller slab (M <= the M it was created with, N_z
+ * likewise, same FFT grid; a may change) without reallocating.  The G0 table
+ * and T0 must be set again.  No work on the plan may be in flight.  No
+ * reference counterpart: it lets msc_sweep reuse one plan per FFT grid across
+ * the instances of a sweep.
+ */
+int msc_plan_reshape(msc_plan* plan, int M, int N_z, double a);
+
+/*
+ * Run each Arnoldi step of msc_gmres as one instantiated HIP graph (enable = 1,
+ * the default) or as individual launches (0).  Graph capture must not overlap
+ * another host thread's allocation or synchronous copy calls, so callers that
+ * drive several plans from several threads turn graphs off (msc_sweep does).
+ * No reference counterpart.
+ */
+int msc_plan_use_graphs(msc_plan* plan, int enable);
+
+/*
+ * Solve `count` independent slabs (different frequencies, incidences, slab
+ * sizes and contrasts) with `workers` host threads.  Replaces the sequential
+ * per-frequency loop of compute_ocean_bottom_reflection
+ * (cubic_scattering/ocean_bottom.py:290-307).  Each worker owns a HIP stream
+ * and one plan per FFT grid (created at the largest slab of that grid and
+ * re-shaped per instance), rebuilds G0 only when (omega, medium, a, slab)
+ * change and autotunes tiles once per grid; solves on different workers
+ * overlap on the device.  Blocks until all instances are done.  Every
+ * instance records its own status and stats; the return value is the first
+ * failing instance's status.  opts->precond must be NULL.
+ */
+int msc_sweep(msc_instance* instances, int count, const msc_gmres_opts* opts, int workers);
 /* out[0..5] = M, N_z, Pz, Pxy, NUV (table columns), NC (table depth). */
 int msc_plan_info(const msc_plan* plan, long long* out6);
 

@@ -29,6 +29,7 @@ from .slab_scattering import (
 from .learned_precond import LearnedPreconditioner, harvest_pairs, merge_pairs
 from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
 from .solver_config import load_config, run_from_config
+from .sweep import SweepCase, compute_slab_sweep
 from .torch_gmres import get_device, select_dtype, to_numpy, to_torch, torch_gmres
 
 __all__ = [
@@ -38,5 +39,5 @@ __all__ = [
     "random_iid_slab_material", "random_slab_material", "slab_reflected_field",
     "slab_rpp_periodic", "uniform_slab_material", "SlabOperator", "load_config",
     "run_from_config", "get_device", "select_dtype", "to_numpy", "to_torch", "torch_gmres",
-    "LearnedPreconditioner", "harvest_pairs", "merge_pairs",
+    "LearnedPreconditioner", "harvest_pairs", "merge_pairs", "SweepCase", "compute_slab_sweep",
 ]

@@ -30,6 +30,9 @@ SIGNATURES = {
     "msc_version": (ctypes.c_char_p, []),
     "msc_plan_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_int, c_int, c_int]),
     "msc_plan_destroy": (c_int, [c_void_p]),
+    "msc_plan_reshape": (c_int, [c_void_p, c_int, c_int, c_double]),
+    "msc_plan_use_graphs": (c_int, [c_void_p, c_int]),
+    "msc_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int]),
     "msc_plan_info": (c_int, [c_void_p, POINTER(c_longlong)]),
     "msc_plan_autotune": (c_int, [c_void_p, c_int, c_void_p]),
     "msc_plan_tiles": (c_int, [c_void_p, POINTER(c_int)]),
@@ -86,6 +89,16 @@ class GmresStats(ctypes.Structure):
     _fields_ = [("iterations", c_int), ("cycles", c_int), ("matvecs_c128", c_int),
                 ("matvecs_c64", c_int), ("converged", c_int), ("fell_back", c_int),
                 ("precond_applies", c_int), ("rel_residual", c_double), ("bnorm", c_double)]
+
+
+class Instance(ctypes.Structure):
+    """msc_instance: one solve of msc_sweep."""
+    _fields_ = [("M", c_int), ("N_z", c_int), ("periodic", c_int), ("a", c_double),
+                ("omega", c_double), ("alpha", c_double), ("beta", c_double), ("rho", c_double),
+                ("k_mag", c_double), ("khat", c_double * 3), ("inc18", c_double * 18),
+                ("dlambda", c_void_p), ("dmu", c_void_p), ("drho", c_void_p), ("psi", c_void_p),
+                ("psi0", c_void_p), ("layer_sum", c_void_p), ("phased", c_void_p), ("stats", GmresStats),
+                ("status", c_int), ("error", ctypes.c_char * 160)]
 
 
 _lib = None

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -27,10 +29,12 @@ struct Twiddles {
   double2* t128 = nullptr;
   float2* t64 = nullptr;
 };
-static std::vector<Twiddles> g_tw;
+static std::deque<Twiddles> g_tw;  // stable element addresses
+static std::mutex g_tw_mutex;        // plans may be created from several host threads
 
 static int twiddles(int dev, Twiddles** out) {
-  if ((int)g_tw.size() <= dev) g_tw.resize(dev + 1);
+  std::lock_guard<std::mutex> lock(g_tw_mutex);
+  while ((int)g_tw.size() <= dev) g_tw.emplace_back();
   Twiddles& t = g_tw[dev];
   if (t.t128 == nullptr) {
     constexpr int N = 1024;  // kTwMax
@@ -182,15 +186,16 @@ bool autotune_enabled() {
 }
 
 int alloc_plan_buffers(Plan& p) {
-  const size_t b1 = (size_t)p.Nz * p.M * p.Pxy * 9;
-  const size_t b2 = (size_t)p.Nz * p.Pxy * p.Pxy * 9;
+  const size_t b1 = (size_t)p.capNz * p.capM * p.Pxy * 9;
+  const size_t b2 = (size_t)p.capNz * p.Pxy * p.Pxy * 9;
   const size_t tab = (size_t)p.NUV * kNE * p.NC;
   MSC_HIP_CHECK(hipMalloc(&p.buf1, b1 * sizeof(double2)));
   MSC_HIP_CHECK(hipMalloc(&p.buf2, b2 * sizeof(double2)));
   MSC_HIP_CHECK(hipMalloc(&p.table128, tab * sizeof(double2)));
   MSC_HIP_CHECK(hipMalloc(&p.table64, tab * sizeof(float2)));
-  MSC_HIP_CHECK(hipMalloc(&p.tp128, p.nvox * 4 * sizeof(double2)));
-  MSC_HIP_CHECK(hipMalloc(&p.tp64, p.nvox * 4 * sizeof(float2)));
+  const size_t capvox = (size_t)p.capNz * p.capM * p.capM;
+  MSC_HIP_CHECK(hipMalloc(&p.tp128, capvox * 4 * sizeof(double2)));
+  MSC_HIP_CHECK(hipMalloc(&p.tp64, capvox * 4 * sizeof(float2)));
   MSC_HIP_CHECK(hipMalloc(&p.tmax, sizeof(unsigned int)));
   return kOk;
 }
@@ -203,7 +208,9 @@ void free_plan_buffers(Plan& p) {
   if (p.tp128) (void)hipFree(p.tp128);
   if (p.tp64) (void)hipFree(p.tp64);
   if (p.tmax) (void)hipFree(p.tmax);
+  if (p.work) (void)hipFree(p.work);
   p.tmax = nullptr;
+  p.work = nullptr;
   p.buf1 = p.buf2 = nullptr;
   p.table128 = nullptr;
   p.table64 = nullptr;
@@ -374,10 +381,10 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
       rc = pl->k64.ensure_z();
       if (rc) return rc;
     }
-    if (pl->b64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->b64, n * sizeof(float2)));
-    if (pl->r64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r64, n * sizeof(float2)));
+    if (pl->b64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->b64, p.cap_n * sizeof(float2)));
+    if (pl->r64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r64, p.cap_n * sizeof(float2)));
   }
-  if (pl->r128 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r128, n * sizeof(double2)));
+  if (pl->r128 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r128, p.cap_n * sizeof(double2)));
   double2* r = pl->r128;
   // tile autotuning once per plan and precision (MSC_AUTOTUNE=0 keeps the defaults)
   if (autotune_enabled()) {
@@ -561,6 +568,9 @@ int msc_plan_create(msc_plan** plan, int M, int N_z, double a, int periodic, int
   p.Pxy = pxy;
   p.nvox = (size_t)N_z * M * M;
   p.n = p.nvox * 9;
+  p.capM = M;
+  p.capNz = N_z;
+  p.cap_n = p.n;
   p.NC = pz / 2 + 1;
   const int h = pxy / 2;
   p.NUV = (h + 1) * (h + 2) / 2;
@@ -583,6 +593,43 @@ int msc_plan_create(msc_plan** plan, int M, int N_z, double a, int periodic, int
   p.tw128 = tw->t128;
   p.tw64 = tw->t64;
   *plan = pl;
+  return kOk;
+}
+
+int msc_plan_reshape(msc_plan* plan, int M, int N_z, double a) {
+  MSC_REQUIRE(plan != nullptr, "plan must not be NULL");
+  Plan& p = plan->p;
+  MSC_REQUIRE(M >= 1 && N_z >= 1, "M and N_z must be >= 1");
+  MSC_REQUIRE(a > 0.0 && std::isfinite(a), "a must be > 0");
+  MSC_REQUIRE(M <= p.capM && N_z <= p.capNz,
+              "the slab exceeds the plan's capacity (create the plan at the largest size)");
+  MSC_REQUIRE(2 * N_z - 1 <= p.Pz, "N_z does not fit the plan's vertical FFT length");
+  if (p.periodic) {
+    MSC_REQUIRE(M == p.M, "a periodic plan cannot change M");
+  } else {
+    MSC_REQUIRE(2 * M - 1 <= p.Pxy, "M does not fit the plan's lateral FFT length");
+  }
+  const bool same = M == p.M && N_z == p.Nz && a == p.a;
+  if (same) return kOk;
+  p.have_table = false;  // the kernel window (and for a new a, the lattice) changes
+  p.have_tparams = false;
+  p.M = M;
+  p.Nz = N_z;
+  p.a = a;
+  p.nvox = (size_t)N_z * M * M;
+  p.n = p.nvox * 9;
+  plan->k128.drop_graphs();
+  plan->k64.drop_graphs();
+  return kOk;
+}
+
+int msc_plan_use_graphs(msc_plan* plan, int enable) {
+  MSC_REQUIRE(plan != nullptr, "plan must not be NULL");
+  plan->use_graphs = enable != 0;
+  if (!plan->use_graphs) {
+    plan->k128.drop_graphs();
+    plan->k64.drop_graphs();
+  }
   return kOk;
 }
 

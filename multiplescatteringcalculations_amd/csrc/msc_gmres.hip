@@ -454,9 +454,17 @@ void Krylov::release() {
 }
 
 int Krylov::ensure_state(size_t n_, int m_, int elem_bytes_) {
-  if (st != nullptr && n_ == n && m_ == m && elem_bytes_ == elem_bytes) return kOk;
+  if (st != nullptr && n_ <= cap_n && m_ == m && elem_bytes_ == elem_bytes) {
+    if (n_ != n) {  // a smaller problem in the same buffers: re-derive the sizes
+      drop_graphs();
+      n = n_;
+      nblk = (int)((n + kChunk - 1) / kChunk);
+    }
+    return kOk;
+  }
   release();
   n = n_;
+  cap_n = n_;
   m = m_;
   elem_bytes = elem_bytes_;
   ldv = (n + 63) & ~size_t(63);  // 64-element aligned rows
@@ -488,7 +496,9 @@ int Krylov::ensure_state(size_t n_, int m_, int elem_bytes_) {
 }
 
 int Krylov::ensure(size_t n_, int m_, int elem_bytes_) {
-  if (V != nullptr && owns_V && n_ == n && m_ == m && elem_bytes_ == elem_bytes) return kOk;
+  if (V != nullptr && owns_V && n_ <= cap_n && m_ == m && elem_bytes_ == elem_bytes)
+    return ensure_state(n_, m_, elem_bytes_);
+  release();
   int rc = ensure_state(n_, m_, elem_bytes_);
   if (rc) return rc;
   drop_graphs();

@@ -25,6 +25,7 @@ struct KState {
 
 struct Krylov {
   size_t n = 0, ldv = 0;
+  size_t cap_n = 0;           // n the buffers were allocated for (n <= cap_n)
   int m = 0, ldh = 0, nblk = 0, elem_bytes = 0;
   void* V = nullptr;          // (m+1) x ldv basis, complex64 or complex128
   KState* st = nullptr;
@@ -51,7 +52,8 @@ struct Krylov {
 
   ~Krylov();
   void release();
-  // state only (caller-owned basis) / state plus an owned basis
+  // state only (caller-owned basis) / state plus an owned basis; a smaller n
+  // than allocated reuses the buffers (graphs are dropped)
   int ensure_state(size_t n, int m, int elem_bytes);
   int ensure(size_t n, int m, int elem_bytes);
   // allocate Z (flexible GMRES) for the current n, m, precision

@@ -28,6 +28,9 @@ struct Plan {
   bool periodic = false;
   int Pz = 0, Pxy = 0;
   size_t nvox = 0, n = 0;  // voxels, complex unknowns (9 per voxel)
+  // the largest slab the buffers hold (msc_plan_reshape stays within it)
+  int capM = 0, capNz = 0;
+  size_t cap_n = 0;
   int NC = 0, NUV = 0;
 
   double omega = 0.0, alpha = 0.0, beta = 0.0, rho = 0.0;
@@ -45,6 +48,7 @@ struct Plan {
   unsigned int* tmax = nullptr;
   void* buf1 = nullptr;  // (Nz, M, Pxy, 9) complex128 bytes
   void* buf2 = nullptr;  // (Nz, Pxy, Pxy, 9) complex128 bytes
+  void* work = nullptr;  // G0 table build: (Pz, Pxy, Pxy, 45) complex128, kept for rebuilds
   double2* tw128 = nullptr;
   float2* tw64 = nullptr;
 

@@ -371,8 +371,8 @@ int grid1d(size_t n, int bs) { return (int)std::min<size_t>((n + bs - 1) / bs, 6
 
 int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st) {
   const size_t P = (size_t)p.Pz * p.Pxy * p.Pxy;
-  double2* work = nullptr;
-  MSC_HIP_CHECK(hipMalloc(&work, P * kNE * sizeof(double2)));
+  if (p.work == nullptr) MSC_HIP_CHECK(hipMalloc(&p.work, P * kNE * sizeof(double2)));
+  double2* work = reinterpret_cast<double2*>(p.work);
   const double kP = omega / alpha, kS = omega / beta;
   const double pre = 1.0 / (4.0 * M_PI * rho * omega * omega);
   hipLaunchKernelGGL(k_g0_spatial, dim3(grid1d(P, 128)), dim3(128), 0, st, work, p.Pz, p.Pxy,
@@ -394,7 +394,6 @@ int build_table(Plan& p, double omega, double alpha, double beta, double rho, hi
     }
   }
   hipError_t e1 = hipStreamSynchronize(st);
-  (void)hipFree(work);
   if (rc) return rc;
   if (e1 != hipSuccess) {
     set_error(std::string("G0 table build: ") + hipGetErrorString(e1));

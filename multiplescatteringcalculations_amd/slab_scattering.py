@@ -171,6 +171,17 @@ class SlabPlan:
     def stream(self) -> ctypes.c_void_p:
         return _lib.stream_ptr(self.device)
 
+    def reshape(self, geometry: SlabGeometry) -> None:
+        """Re-target the plan to a slab no larger than the one it was created
+        for (same FFT grid) without reallocating; G0 and T0 must be set again."""
+        torch.cuda.current_stream(self.device).synchronize()
+        _lib.check(self.lib.msc_plan_reshape(self._h, geometry.M, geometry.N_z,
+                                             float(geometry.a)))
+        self.geometry = geometry
+        self.n = geometry.n_cubes * 9
+        self.omega = None
+        self.ref = None
+
     def build_g0(self, omega: float, ref: ReferenceMedium, *, volume_averaged: bool = False,
                  n_orders: int = 2) -> None:
         """Spectral G0 table on the device (replaces _build_slab_kernels[_gpu])."""
@@ -489,6 +500,15 @@ def _voigt_to_tensor(voigt_6: NDArray) -> NDArray:
 
 
 def _source_sums(result: SlabResult, T_local, kP: float, kS: float, r_hat):
+    sums = getattr(result, "_sums", None)
+    if sums is not None and T_local is None:
+        # reduced on the device by the sweep: layer sums always apply, the
+        # phased sums only for the far-field direction and wavenumbers they used
+        layer_sum, phased, k2, rh = sums
+        if kP == 0.0 and kS == 0.0:
+            return layer_sum, None
+        if np.allclose([kP, kS], k2, rtol=1e-14, atol=0.0) and np.allclose(r_hat, rh, rtol=0, atol=1e-14):
+            return layer_sum, phased
     plan = getattr(result, "_plan", None)
     g = result.geometry
     if plan is None or T_local is not None:

@@ -282,3 +282,22 @@ def test_random_iid_schema_and_cpu_backend_refused(cfg_file):
     assert mat.Dlambda.shape == (2, 4, 4)
     with pytest.raises(ValueError, match="HIP devices only"):
         run_from_config(c)
+
+
+def test_sweep_and_reshape_validate_arguments():
+    """msc_sweep and msc_plan_reshape reject bad arguments before any device work."""
+    from multiplescatteringcalculations_amd import _lib
+
+    lib = _lib.load()
+    opts = _lib.GmresOpts(30, 100, 1e-8, 1, 1, 1e-4, 0, None)
+    inst = (_lib.Instance * 1)()
+    assert lib.msc_sweep(None, 1, ctypes.byref(opts), 1) == _lib.MSC_ERR_INVALID
+    assert lib.msc_sweep(ctypes.cast(inst, ctypes.c_void_p), 1, None, 1) == _lib.MSC_ERR_INVALID
+    assert lib.msc_sweep(ctypes.cast(inst, ctypes.c_void_p), 1, ctypes.byref(opts), 0) == _lib.MSC_ERR_INVALID
+    # an instance without arrays is invalid and says which one
+    assert lib.msc_sweep(ctypes.cast(inst, ctypes.c_void_p), 1, ctypes.byref(opts), 1) == _lib.MSC_ERR_INVALID
+    assert inst[0].status == _lib.MSC_ERR_INVALID and b"instance 0" in inst[0].error
+    # a preconditioner is not allowed in a sweep
+    opts.precond = ctypes.c_void_p(1)
+    assert lib.msc_sweep(ctypes.cast(inst, ctypes.c_void_p), 1, ctypes.byref(opts), 1) == _lib.MSC_ERR_INVALID
+    assert lib.msc_plan_reshape(None, 4, 4, 1.0) == _lib.MSC_ERR_INVALID
