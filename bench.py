@@ -134,6 +134,65 @@ def time_pass3(plan, psi0, precision, steps):
     return tim
 
 
+def sweep_cases():
+    """The batched-sweep workload: 64 frequencies ka = 0.1..1.0 x 4 slownesses
+    p = 0, 0.1, 0.2, 0.3 s/km, each on its own seeded slab of uneven size
+    (lateral 32-128, depth 4-32 voxels), phi 0.3, contrasts +-0.2."""
+    import multiplescatteringcalculations_amd as msc
+
+    ref = msc.ReferenceMedium(VP, VS, RHO)
+    rng = np.random.default_rng(0)
+    cases = []
+    for i, ka in enumerate(np.linspace(0.1, 1.0, 64)):
+        for j, p in enumerate((0.0, 1e-4, 2e-4, 3e-4)):
+            g = msc.SlabGeometry(M=int(rng.integers(32, 129)), N_z=int(rng.integers(4, 33)), a=A)
+            mat = msc.random_iid_slab_material(g, ref, PHI, CONTRAST, seed=1000 * i + j)
+            khat = np.array([np.sqrt(1.0 - (p * VP) ** 2), p * VP, 0.0])
+            cases.append(msc.SweepCase(g, mat, float(ka) * VS / A, khat, "P"))
+    return cases
+
+
+def time_sweep(workers: int = 4):
+    """One timed pass of the batched sweep (plans, G0 tables, T0, incident fields
+    and solves all inside the timed region; inputs already on the host)."""
+    import torch
+
+    import multiplescatteringcalculations_amd as msc
+
+    cases = sweep_cases()
+    msc.compute_slab_sweep(cases[:8], gmres_tol=TOL, precision="mixed", workers=workers,
+                           keep_on_device=True)  # warm-up (library, allocator)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = msc.compute_slab_sweep(cases, gmres_tol=TOL, precision="mixed", workers=workers,
+                                 keep_on_device=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    its = [r.n_gmres_iter for r in res]
+    out = {"solves": len(res), "seconds": round(dt, 3), "solves_per_s": round(len(res) / dt, 1),
+           "workers": workers, "voxels": int(sum(c.geometry.n_cubes for c in cases)),
+           "mean_iterations": round(float(np.mean(its)), 2), "max_iterations": int(max(its)),
+           "all_converged": bool(all(r.stats["converged"] for r in res)),
+           "max_rel_residual": float(max(r.gmres_residual for r in res))}
+    log(f"sweep: {out}")
+    return out
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the same counters of a copy of
+    known size); None when no summary for this kernel and workload exists."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_z_mult_c64_128x128x32_traffic.txt")
+    if kernel != "k_z_mult<float,64,8>" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch"):
+                return int(line.split("=")[1])
+    return None
+
+
 def cpu_baseline(mat, omega, n_matvec: int):
     """The oracle's restatement of the reference CPU matvec (slab_scattering.py:
     359-418: per-layer 2-D FFTs and the N_z^2 layer-pair 9x9 loop) timed on one
@@ -167,6 +226,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: independent (frequency, slowness, slab) solves shard "
@@ -209,15 +269,17 @@ def main():
     bytes_pass3 = 2 * spectra + table
     avg_ms = ms64 / max(n64, 1)
     achieved = bytes_pass3 / (avg_ms * 1e-3) / 1e9
+    kname = f"k_z_mult<float,64,{plan.tiles()['c64'][2]}>"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": None,
-                "kernel": f"k_z_mult<float,64,{plan.tiles()['c64'][2]}> (z FFT + 9x9 block multiply "
+                "frac": round(achieved / 8000.0, 4), "traffic": measured_traffic(kname),
+                "kernel": f"{kname} (z FFT + 9x9 block multiply "
                           "+ inverse z FFT), complex64 launches of the mixed solves",
                 "launches": n64, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_pass3}
 
     mean_mx = float(np.mean(tmx))
     mean_64 = float(np.mean(t64))
     mean_ln = float(np.mean(tln))
+    sweep = None if args.no_sweep else time_sweep()
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement of the reference CPU matvec, 1 matvec) ...")
@@ -264,6 +326,7 @@ def main():
             "fp64_rel_residual": st64["rel_residual"], "mixed_rel_residual": stmx["rel_residual"],
             "mixed_vs_fp64_max_rel_diff": diff, "setup_s": round(t_setup, 3),
             "pass3_c128_avg_us": round(tim64["c128"][1] / max(tim64["c128"][0], 1) * 1e3, 2),
+            "sweep_256_uneven_slabs_mixed": sweep,
         },
     }
     print(json.dumps(out), flush=True)

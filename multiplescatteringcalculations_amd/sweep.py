@@ -77,8 +77,8 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
             raise ValueError("periodic cases need M to be a power of two")
         k_hat, k_mag, inc = _incident_vector(c.omega, mat.ref, c.k_hat, c.wave_type)
         shape = (g.N_z, g.M, g.M)
-        arrs = [torch.as_tensor(np.ascontiguousarray(np.broadcast_to(a, shape), dtype=np.float64),
-                                device=device) for a in (mat.Dlambda, mat.Dmu, mat.Drho)]
+        arrs = [torch.tensor(np.broadcast_to(np.asarray(a, dtype=np.float64), shape),
+                             device=device) for a in (mat.Dlambda, mat.Dmu, mat.Drho)]
         psi = torch.empty((*shape, 9), dtype=torch.complex128, device=device)
         psi0 = torch.empty_like(psi)
         ls = torch.empty((g.N_z, 9), dtype=torch.complex128, device=device)
