@@ -185,7 +185,11 @@ int msc_cube_coefficients(double omega, double a, double alpha, double beta, dou
  * Spectral G0 table on the device (FP64 evaluation, stored complex128 and
  * complex64).  Replaces _build_slab_kernels (slab_scattering.py:267-351) and
  * _build_slab_kernels_gpu (slab_scattering_gpu.py:34-92).  volume_averaged
- * != 0 is not supported by this build (MSC_ERR_UNSUPPORTED).
+ * != 0 replaces the point propagator at the 26 nearest-neighbour offsets by
+ * the inter-voxel block of inter_voxel_propagator_9x9 (slab_scattering.py:
+ * 316-329; inter_voxel_propagator.py:1295) with n_orders in 0..3 frequency orders;
+ * it is applied as a 26-block stencil after the spectral passes, so every
+ * operator entry point (matvec, residual, scattered, GMRES) carries it.
  */
 int msc_build_g0(msc_plan* plan, double omega, double alpha, double beta, double rho,
                  int volume_averaged, int n_orders, void* stream);

@@ -209,8 +209,14 @@ void free_plan_buffers(Plan& p) {
   if (p.tp64) (void)hipFree(p.tp64);
   if (p.tmax) (void)hipFree(p.tmax);
   if (p.work) (void)hipFree(p.work);
+  if (p.nn128) (void)hipFree(p.nn128);
+  if (p.nn64) (void)hipFree(p.nn64);
+  if (p.nn_stage) (void)hipFree(p.nn_stage);
   p.tmax = nullptr;
   p.work = nullptr;
+  p.nn128 = nullptr;
+  p.nn64 = nullptr;
+  p.nn_stage = nullptr;
   p.buf1 = p.buf2 = nullptr;
   p.table128 = nullptr;
   p.table64 = nullptr;
@@ -612,6 +618,7 @@ int msc_plan_reshape(msc_plan* plan, int M, int N_z, double a) {
   const bool same = M == p.M && N_z == p.Nz && a == p.a;
   if (same) return kOk;
   p.have_table = false;  // the kernel window (and for a new a, the lattice) changes
+  p.have_nn = false;
   p.have_tparams = false;
   p.M = M;
   p.Nz = N_z;
@@ -727,13 +734,10 @@ int msc_build_g0(msc_plan* plan, double omega, double alpha, double beta, double
   MSC_REQUIRE(plan != nullptr, "plan must not be NULL");
   MSC_REQUIRE(omega > 0 && alpha > 0 && beta > 0 && rho > 0,
               "omega, alpha, beta and rho must be positive");
-  (void)n_orders;
-  if (volume_averaged) {
-    set_error("volume_averaged=True (inter_voxel_propagator_9x9 near-neighbour blocks) is not "
-              "supported by this build");
-    return kUnsupported;
-  }
-  return build_table(plan->p, omega, alpha, beta, rho, S(stream));
+  MSC_REQUIRE(!volume_averaged || (n_orders >= 0 && n_orders <= 3), "n_orders must be in 0..3");
+  int rc = build_table(plan->p, omega, alpha, beta, rho, S(stream));
+  if (rc) return rc;
+  return set_nn_correction(plan->p, volume_averaged != 0, n_orders, S(stream));
 }
 
 int msc_set_tmatrix(msc_plan* plan, const double* dlambda, const double* dmu, const double* drho,

@@ -14,6 +14,7 @@
 //   4. x-inverse  : inverse FFT along x, keep x < M
 //   5. y-inverse  : inverse FFT along y, keep y < M, y = x - acc (or the
 //                   residual b - x + acc)
+#include <algorithm>
 #include <cstdint>
 
 #include "msc_fft.hpp"
@@ -321,16 +322,87 @@ __global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
   }
 }
 
+// ── volume-averaged nearest-neighbour correction (after pass 5) ──────────────
+// acc_v = sum over the 26 offsets R of nn[R] tau(v - R), tau = T0 x, with
+// periodic lateral wrap when asked; MODE 0: out -= acc, 1 and 2: out += acc
+// (the operator, the residual and the scattered part).  The 26 x 81 blocks
+// are staged in LDS.
+template <typename R, int MODE>
+__global__ __launch_bounds__(kNT) void k_nn_apply(const cx<R>* __restrict__ x,
+                                                 const cx<R>* __restrict__ tp,
+                                                 const cx<R>* __restrict__ nn,
+                                                 cx<R>* __restrict__ out, int M, int Nz,
+                                                 int periodic, const int* __restrict__ done) {
+  __shared__ cx<R> sd[26 * 81];
+  if (done != nullptr && *done) return;
+  for (int i = threadIdx.x; i < 26 * 81; i += kNT) sd[i] = nn[i];
+  __syncthreads();
+  const size_t nvox = (size_t)Nz * M * M;
+  for (size_t v = (size_t)blockIdx.x * kNT + threadIdx.x; v < nvox;
+       v += (size_t)gridDim.x * kNT) {
+    const int y = (int)(v % M);
+    const size_t q = v / M;
+    const int xr = (int)(q % M);
+    const int z = (int)(q / M);
+    cx<R> acc[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) acc[c] = czero<R>();
+    int slot = 0;
+    for (int r = 0; r < 27; ++r) {
+      if (r == 13) continue;
+      const int dz = r / 9 - 1, dx = (r / 3) % 3 - 1, dy = r % 3 - 1;
+      const cx<R>* K = sd + (slot++) * 81;
+      const int uz = z - dz;
+      int ux = xr - dx, uy = y - dy;
+      if (uz < 0 || uz >= Nz) continue;
+      if (periodic) {
+        if (M == 1 && (dx != 0 || dy != 0)) continue;  // lateral images span |d| <= M-1
+        ux = (ux + M) % M;
+        uy = (uy + M) % M;
+      } else if (ux < 0 || ux >= M || uy < 0 || uy >= M) {
+        continue;
+      }
+      const size_t u = ((size_t)uz * M + ux) * M + uy;
+      const cx<R>* t = tp + u * 4;
+      const cx<R> tu = t[0], td = t[1], to = t[2], ts = t[3];
+      cx<R> xv[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) xv[c] = x[u * 9 + c];
+      const cx<R> sum = cadd<R>(cadd<R>(xv[3], xv[4]), xv[5]);
+      const cx<R> dmo = csub<R>(td, to);
+      const cx<R> tos = cmul<R>(to, sum);
+      cx<R> tau[9];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tau[c] = cmul<R>(tu, xv[c]);
+#pragma unroll
+      for (int c = 3; c < 6; ++c) tau[c] = cadd<R>(cmul<R>(dmo, xv[c]), tos);
+#pragma unroll
+      for (int c = 6; c < 9; ++c) tau[c] = cmul<R>(ts, xv[c]);
+#pragma unroll
+      for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) cfma<R>(acc[i], K[i * 9 + j], tau[j]);
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const cx<R> o = out[v * 9 + c];
+      out[v * 9 + c] = MODE == 0 ? csub<R>(o, acc[c]) : cadd<R>(o, acc[c]);
+    }
+  }
+}
+
 // ── host dispatch ───────────────────────────────────────────────────────────
 
 template <typename R> struct PlanBufs;
 template <> struct PlanBufs<double> {
   static const double2* table(const Plan& p) { return p.table128; }
+  static const double2* nn(const Plan& p) { return p.nn128; }
   static const double2* tp(const Plan& p) { return p.tp128; }
   static const double2* tw(const Plan& p) { return p.tw128; }
 };
 template <> struct PlanBufs<float> {
   static const float2* table(const Plan& p) { return p.table64; }
+  static const float2* nn(const Plan& p) { return p.nn64; }
   static const float2* tp(const Plan& p) { return p.tp64; }
   static const float2* tw(const Plan& p) { return p.tw64; }
 };
@@ -604,7 +676,21 @@ static int apply(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, int mod
   if (rc) return rc;
   rc = z_pass<R>(p, b2, st, done);
   if (rc) return rc;
-  return xy_inv<R>(p, x, bvec, out, b1, b2, mode, st, done);
+  rc = xy_inv<R>(p, x, bvec, out, b1, b2, mode, st, done);
+  if (rc || !p.have_nn) return rc;
+  const size_t nvox = (size_t)p.Nz * p.M * p.M;
+  const int grid = (int)std::min<size_t>((nvox + kNT - 1) / kNT, 4096);
+  const cx<R>* tp = PlanBufs<R>::tp(p);
+  const cx<R>* nn = PlanBufs<R>::nn(p);
+  const int per = p.periodic ? 1 : 0;
+  if (mode == 0)
+    hipLaunchKernelGGL((k_nn_apply<R, 0>), dim3(grid), dim3(kNT), 0, st, x, tp, nn, out, p.M,
+                       p.Nz, per, done);
+  else
+    hipLaunchKernelGGL((k_nn_apply<R, 1>), dim3(grid), dim3(kNT), 0, st, x, tp, nn, out, p.M,
+                       p.Nz, per, done);
+  MSC_LAUNCH_CHECK();
+  return kOk;
 }
 
 template <typename R>

@@ -49,6 +49,13 @@ struct Plan {
   void* buf1 = nullptr;  // (Nz, M, Pxy, 9) complex128 bytes
   void* buf2 = nullptr;  // (Nz, Pxy, Pxy, 9) complex128 bytes
   void* work = nullptr;  // G0 table build: (Pz, Pxy, Pxy, 45) complex128, kept for rebuilds
+  // volume-averaged nearest-neighbour correction: for the 26 offsets R in
+  // {-1,0,1}^3 \ 0 (lexicographic (dz, dx, dy)), the 9x9 difference between the
+  // volume-averaged block and the point propagator the table holds, row-major
+  bool have_nn = false;
+  double2* nn128 = nullptr;  // 26 x 81
+  float2* nn64 = nullptr;
+  double* nn_stage = nullptr;  // 26 x 81 real volume-averaged blocks (device copy)
   double2* tw128 = nullptr;
   float2* tw64 = nullptr;
 
@@ -90,5 +97,8 @@ int fft_axis(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, bool i
              const cx<R>* tw, hipStream_t st);
 
 int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st);
+// volume_averaged: set the nearest-neighbour correction for the plan's table
+// (after build_table, same frequency and medium); otherwise clear it.
+int set_nn_correction(Plan& p, bool volume_averaged, int n_orders, hipStream_t st);
 
 }  // namespace msc

@@ -367,6 +367,48 @@ int grid1d(size_t n, int bs) { return (int)std::min<size_t>((n + bs - 1) / bs, 6
 
 }  // namespace
 
+// nn[slot] = VA[slot] - (point propagator at R d), both as full 9x9 blocks; the
+// point block is expanded from the table's 45 independent entries
+// (G symmetric, H = W^-1 C^T, S = S' W with S' symmetric, W = diag(1,1,1,1/2,1/2,1/2)).
+__global__ __launch_bounds__(32) void k_nn_delta(const double* __restrict__ va, double d,
+                                                 double kP, double kS, double pre,
+                                                 double2* __restrict__ nn128,
+                                                 float2* __restrict__ nn64) {
+  const int slot = threadIdx.x;
+  if (slot >= 26) return;
+  const int r = slot < 13 ? slot : slot + 1;  // skip the centre of the 3x3x3 cube
+  const int dz = r / 9 - 1, dx = (r / 3) % 3 - 1, dy = r % 3 - 1;
+  double2 e45[45];
+  for (int e = 0; e < 45; ++e) e45[e] = make_double2(0.0, 0.0);
+  add_prop45(dz * d, dx * d, dy * d, kP, kS, pre, e45);
+  double2 K[81];
+  const int gi[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  for (int e = 0; e < 6; ++e) {
+    K[gi[e][0] * 9 + gi[e][1]] = e45[e];
+    K[gi[e][1] * 9 + gi[e][0]] = e45[e];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int al = 0; al < 6; ++al) {
+      const double2 c = e45[6 + 6 * i + al];
+      K[i * 9 + 3 + al] = c;
+      const double f = al < 3 ? 1.0 : 2.0;
+      K[(3 + al) * 9 + i] = make_double2(f * c.x, f * c.y);
+    }
+  int e = 24;
+  for (int al = 0; al < 6; ++al)
+    for (int be = al; be < 6; ++be, ++e) {
+      const double2 sv = e45[e];
+      const double wb = be < 3 ? 1.0 : 0.5, wa = al < 3 ? 1.0 : 0.5;
+      K[(3 + al) * 9 + 3 + be] = make_double2(sv.x * wb, sv.y * wb);
+      K[(3 + be) * 9 + 3 + al] = make_double2(sv.x * wa, sv.y * wa);
+    }
+  for (int q = 0; q < 81; ++q) {
+    const double2 v = make_double2(va[slot * 81 + q] - K[q].x, -K[q].y);
+    nn128[slot * 81 + q] = v;
+    nn64[slot * 81 + q] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
 // ── host entry points ────────────────────────────────────────────────────────
 
 int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st) {
@@ -427,6 +469,27 @@ static int update_tmax(Plan& p, hipStream_t st) {
   hipLaunchKernelGGL(k_tmax, dim3(std::min(grid1d(p.nvox * 4, 256), 1024)), dim3(256), 0, st,
                      p.tp128, p.nvox * 4, p.tmax);
   MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+int set_nn_correction(Plan& p, bool volume_averaged, int n_orders, hipStream_t st) {
+  p.have_nn = false;
+  if (!volume_averaged) return kOk;
+  MSC_REQUIRE(p.have_table, "the G0 table must be built first");
+  MSC_REQUIRE(n_orders >= 0 && n_orders <= 3, "n_orders must be in 0..3");
+  double host[26 * 81];
+  va_blocks(p.alpha, p.beta, p.rho, p.omega, n_orders, host);
+  if (p.nn128 == nullptr) MSC_HIP_CHECK(hipMalloc(&p.nn128, 26 * 81 * sizeof(double2)));
+  if (p.nn64 == nullptr) MSC_HIP_CHECK(hipMalloc(&p.nn64, 26 * 81 * sizeof(float2)));
+  if (p.nn_stage == nullptr) MSC_HIP_CHECK(hipMalloc(&p.nn_stage, 26 * 81 * sizeof(double)));
+  MSC_HIP_CHECK(hipMemcpyAsync(p.nn_stage, host, sizeof(host), hipMemcpyHostToDevice, st));
+  const double kP = p.omega / p.alpha, kS = p.omega / p.beta;
+  const double pre = 1.0 / (4.0 * M_PI * p.rho * p.omega * p.omega);
+  hipLaunchKernelGGL(k_nn_delta, dim3(1), dim3(32), 0, st, p.nn_stage, 2.0 * p.a, kP, kS, pre,
+                     p.nn128, p.nn64);
+  MSC_LAUNCH_CHECK();
+  MSC_HIP_CHECK(hipStreamSynchronize(st));  // `host` is a stack buffer
+  p.have_nn = true;
   return kOk;
 }
 

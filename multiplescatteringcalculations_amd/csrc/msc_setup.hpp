@@ -10,6 +10,10 @@ int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, cons
 int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st);
 int incident_field(Plan& p, double k_mag, const double* khat3, const double* inc18, double2* psi0,
                    hipStream_t st);
+// The 26 volume-averaged nearest-neighbour 9x9 blocks (row-major, real) for
+// R = (dz, dx, dy) in {-1,0,1}^3 \ 0, lexicographic (msc_va.hip).
+void va_blocks(double alpha, double beta, double rho, double omega, int n_orders,
+               double* out26x81);
 int source_sums(Plan& p, const double2* psi, double kP, double kS, const double* rhat3,
                 double2* layer_sum, double2* phased, hipStream_t st);
 

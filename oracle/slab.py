@@ -53,13 +53,18 @@ def tparams_from_T(T: np.ndarray) -> np.ndarray:
 # ── G0 spectral table (slab_scattering.py:267-351) ──────────────────────
 
 
-def slab_kernels(M, N_z, a, omega, ref: tm.Medium, *, periodic=False) -> np.ndarray:
+def slab_kernels(M, N_z, a, omega, ref: tm.Medium, *, periodic=False, volume_averaged=False,
+                 n_orders=2) -> np.ndarray:
     """xy-FFT'd kernels (2N_z-1, H, H, 9, 9), H = M if periodic else 2M-1.
 
     For each dz, the propagator is evaluated on the fundamental domain
     0 <= dy <= dx < M, mapped to its D4h images, the self term is left zero;
     periodic mode folds the (2M-1)^2 spatial kernel mod M without a Bloch phase.
+    ``volume_averaged`` replaces the 26 nearest-neighbour offsets by the
+    volume-averaged block of each signed offset (slab_scattering.py:316-329).
     """
+    from . import intervoxel
+
     d = 2.0 * a
     S = 2 * M - 1
     H = M if periodic else S
@@ -70,6 +75,11 @@ def slab_kernels(M, N_z, a, omega, ref: tm.Medium, *, periodic=False) -> np.ndar
         for dx in range(M):
             for dy in range(dx + 1):
                 if dx == 0 and dy == 0 and dzv == 0:
+                    continue
+                if volume_averaged and max(abs(dzv), dx, dy) <= 1:
+                    for sdx, sdy, _ in prop.d4h_orbit(dx, dy):
+                        sp[sdx + M - 1, sdy + M - 1] = intervoxel.propagator_9x9(
+                            (dzv, sdx, sdy), ref.alpha, ref.beta, ref.rho, omega, n_orders)
                     continue
                 G0 = prop.propagator_9x9([dzv * d, dx * d, dy * d], omega, ref.alpha, ref.beta, ref.rho)
                 for sdx, sdy, f in prop.d4h_orbit(dx, dy):
@@ -235,13 +245,15 @@ def incident_field(M, N_z, a, omega, ref: tm.Medium, k_hat, wave_type) -> np.nda
 
 
 def solve(M, N_z, a, Dlambda, Dmu, Drho, omega, ref, k_hat, wave_type="P",
-          tol=1e-6, max_iter=500, *, periodic=False, restart=20):
+          tol=1e-6, max_iter=500, *, periodic=False, restart=20, volume_averaged=False,
+          n_orders=2):
     """SciPy GMRES(restart), x0 = b, followed by the true residual.
 
     Returns (psi (N_z,M,M,9), psi0, T, n_matvec, rel_residual).
     """
     T = slab_tmatrices(M, N_z, a, Dlambda, Dmu, Drho, omega, ref)
-    kh = slab_kernels(M, N_z, a, omega, ref, periodic=periodic)
+    kh = slab_kernels(M, N_z, a, omega, ref, periodic=periodic, volume_averaged=volume_averaged,
+                      n_orders=n_orders)
     psi0 = incident_field(M, N_z, a, omega, ref, k_hat, wave_type)
     n = N_z * M * M * 9
     count = [0]
