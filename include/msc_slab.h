@@ -84,6 +84,8 @@ typedef struct msc_gmres_stats {
 typedef struct msc_instance {
   int M, N_z;               /* slab of M x M x N_z voxels                      */
   int periodic;             /* 1: laterally periodic (M a power of two)        */
+  int volume_averaged;      /* 1: inter-voxel blocks at the 26 neighbours      */
+  int n_orders;             /* their frequency orders, 0..3 (volume_averaged)  */
   double a;                 /* cube half-width                                 */
   double omega;             /* angular frequency                               */
   double alpha, beta, rho;  /* background medium                               */

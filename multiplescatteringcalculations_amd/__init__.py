@@ -26,7 +26,16 @@ from .slab_scattering import (
     slab_rpp_periodic,
     uniform_slab_material,
 )
+from .kennett_layers import FluidLayer, IsotropicLayer, KennettResult, LayerStack
 from .learned_precond import LearnedPreconditioner, harvest_pairs, merge_pairs
+from .ocean_bottom import (
+    OceanBottomConfig,
+    OceanBottomResult,
+    compute_ocean_bottom_reflection,
+    load_ocean_bottom_config,
+    ricker_source_spectrum,
+    write_log,
+)
 from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
 from .solver_config import load_config, run_from_config
 from .sweep import SweepCase, compute_slab_sweep
@@ -40,4 +49,7 @@ __all__ = [
     "slab_rpp_periodic", "uniform_slab_material", "SlabOperator", "load_config",
     "run_from_config", "get_device", "select_dtype", "to_numpy", "to_torch", "torch_gmres",
     "LearnedPreconditioner", "harvest_pairs", "merge_pairs", "SweepCase", "compute_slab_sweep",
+    "FluidLayer", "IsotropicLayer", "KennettResult", "LayerStack",
+    "OceanBottomConfig", "OceanBottomResult", "compute_ocean_bottom_reflection",
+    "load_ocean_bottom_config", "ricker_source_spectrum", "write_log",
 ]

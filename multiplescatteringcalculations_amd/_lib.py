@@ -93,7 +93,8 @@ class GmresStats(ctypes.Structure):
 
 class Instance(ctypes.Structure):
     """msc_instance: one solve of msc_sweep."""
-    _fields_ = [("M", c_int), ("N_z", c_int), ("periodic", c_int), ("a", c_double),
+    _fields_ = [("M", c_int), ("N_z", c_int), ("periodic", c_int), ("volume_averaged", c_int),
+                ("n_orders", c_int), ("a", c_double),
                 ("omega", c_double), ("alpha", c_double), ("beta", c_double), ("rho", c_double),
                 ("k_mag", c_double), ("khat", c_double * 3), ("inc18", c_double * 18),
                 ("dlambda", c_void_p), ("dmu", c_void_p), ("drho", c_void_p), ("psi", c_void_p),

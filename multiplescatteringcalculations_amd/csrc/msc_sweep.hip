@@ -41,7 +41,7 @@ struct CachedPlan {
   void* b = nullptr;  // psi0 workspace (capacity of the plan, complex128)
   // table key (valid while the plan's slab is unchanged)
   bool have_table = false;
-  int M = 0, Nz = 0;
+  int M = 0, Nz = 0, va = 0, n_orders = 0;
   double a = 0, omega = 0, alpha = 0, beta = 0, rho = 0;
 };
 
@@ -101,15 +101,20 @@ int run_one(msc_instance& I, std::vector<CachedPlan>& cache, const msc_gmres_opt
   }
   const bool same_table = cp->have_table && cp->M == I.M && cp->Nz == I.N_z && cp->a == I.a &&
                           cp->omega == I.omega && cp->alpha == I.alpha &&
-                          cp->beta == I.beta && cp->rho == I.rho;
+                          cp->beta == I.beta && cp->rho == I.rho &&
+                          cp->va == I.volume_averaged &&
+                          (!I.volume_averaged || cp->n_orders == I.n_orders);
   if (!same_table) {
-    rc = msc_build_g0(cp->plan, I.omega, I.alpha, I.beta, I.rho, 0, 2, st);
+    rc = msc_build_g0(cp->plan, I.omega, I.alpha, I.beta, I.rho, I.volume_averaged, I.n_orders,
+                      st);
     if (rc) {
       cp->have_table = false;
       record(I, rc, "G0 table");
       return rc;
     }
     cp->have_table = true;
+    cp->va = I.volume_averaged;
+    cp->n_orders = I.n_orders;
     cp->omega = I.omega;
     cp->alpha = I.alpha;
     cp->beta = I.beta;
@@ -198,7 +203,9 @@ extern "C" int msc_sweep(msc_instance* instances, int count, const msc_gmres_opt
     std::memset(&I.stats, 0, sizeof(I.stats));
     const bool ok = I.M >= 1 && I.N_z >= 1 && I.a > 0.0 && I.omega > 0.0 && I.alpha > 0.0 &&
                     I.beta > 0.0 && I.rho > 0.0 && I.dlambda && I.dmu && I.drho && I.psi &&
-                    (I.periodic == 0 || I.periodic == 1);
+                    (I.periodic == 0 || I.periodic == 1) &&
+                    (I.volume_averaged == 0 ||
+                     (I.volume_averaged == 1 && I.n_orders >= 0 && I.n_orders <= 3));
     if (!ok) {
       I.status = MSC_ERR_INVALID;
       std::snprintf(I.error, sizeof(I.error), "invalid instance %d", i);

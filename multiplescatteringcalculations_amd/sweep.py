@@ -35,6 +35,8 @@ class SweepCase:
     k_hat: NDArray
     wave_type: str = "P"
     periodic: bool = False
+    volume_averaged: bool = False
+    n_orders: int = 2
 
 
 def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_iter: int = 500,
@@ -87,6 +89,7 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
         outs.append((psi, psi0, ls, ph, k_hat))
         I = inst[i]
         I.M, I.N_z, I.periodic, I.a = g.M, g.N_z, int(bool(c.periodic)), float(g.a)
+        I.volume_averaged, I.n_orders = int(bool(c.volume_averaged)), int(c.n_orders)
         I.omega = float(c.omega)
         I.alpha, I.beta, I.rho = mat.ref.alpha, mat.ref.beta, mat.ref.rho
         I.k_mag = float(k_mag)

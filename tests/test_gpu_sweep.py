@@ -93,3 +93,27 @@ def test_plan_reshape_reuses_buffers(hip_device):
     # growing beyond the allocation is refused
     with pytest.raises(ValueError, match="capacity"):
         plan.reshape(msc.SlabGeometry(M=9, N_z=4, a=1.0))
+
+
+def test_volume_averaged_sweep_equals_individual_solves(hip_device):
+    """Volume-averaged cases in a sweep, mixed with point-propagator cases of
+    the same grid (so the table key must switch): each equals its lone solve."""
+    import multiplescatteringcalculations_amd as msc
+
+    ref = msc.ReferenceMedium(3000.0, 1732.0, 2500.0)
+    g = msc.SlabGeometry(M=8, N_z=3, a=1.0)
+    mat = msc.random_iid_slab_material(g, ref, 0.3, 0.2, seed=7)
+    kh = np.array([1.0, 0.0, 0.0])
+    om = 0.3 * ref.beta
+    cases = [msc.SweepCase(g, mat, om, kh, "P", periodic=True, volume_averaged=True, n_orders=2),
+             msc.SweepCase(g, mat, om, kh, "P", periodic=True),
+             msc.SweepCase(g, mat, om, kh, "P", periodic=True, volume_averaged=True, n_orders=1),
+             msc.SweepCase(g, mat, om, kh, "P", periodic=False, volume_averaged=True)]
+    res = msc.compute_slab_sweep(cases, gmres_tol=1e-11, precision="fp64", workers=1)
+    for c, r in zip(cases, res):
+        one = msc.compute_slab_scattering_gpu(g, mat, om, kh, "P", gmres_tol=1e-11, restart=30,
+                                              periodic=c.periodic,
+                                              volume_averaged=c.volume_averaged,
+                                              n_orders=c.n_orders)
+        assert _rel(r.psi, one.psi) < 1e-9, (c.periodic, c.volume_averaged, c.n_orders)
+    assert _rel(res[0].psi, res[1].psi) > 1e-6  # the averaging changed the field

@@ -301,3 +301,37 @@ def test_sweep_and_reshape_validate_arguments():
     opts.precond = ctypes.c_void_p(1)
     assert lib.msc_sweep(ctypes.cast(inst, ctypes.c_void_p), 1, ctypes.byref(opts), 1) == _lib.MSC_ERR_INVALID
     assert lib.msc_plan_reshape(None, 4, 4, 1.0) == _lib.MSC_ERR_INVALID
+
+
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """Every ctypes mirror of a struct in include/msc_slab.h has the C size and
+    field offsets (a C program built with gcc prints them)."""
+    import shutil
+    import subprocess
+
+    from multiplescatteringcalculations_amd import _lib
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    pairs = [("msc_gmres_opts", _lib.GmresOpts), ("msc_gmres_stats", _lib.GmresStats),
+             ("msc_instance", _lib.Instance)]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "msc_slab.h"', "int main(void){"]
+    for cname, cls in pairs:
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = Path(__file__).resolve().parents[1] / "include"
+    subprocess.run(["gcc", "-std=c99", f"-I{inc}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for ln in out.splitlines():
+        parts = ln.split()
+        got[(parts[0], parts[1])] = int(parts[2])
+    for cname, cls in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
