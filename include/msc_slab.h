@@ -106,6 +106,31 @@ typedef struct msc_instance {
   char error[160];          /* out: message when status != MSC_OK              */
 } msc_instance;
 
+/*
+ * Resonance-regime T-matrices of `count` cubes of half-width a, each split
+ * into n_sub^3 sub-cells (n_sub 1..8): the internal 9N x 9N Foldy-Lax system
+ * (N = n_sub^3) with 9 incident patterns, solved by Gaussian elimination with
+ * partial pivoting on the device, and the composite 9x9 T-matrix
+ * T_comp = sum_m T_loc psi_m.  Replaces compute_resonance_tmatrix /
+ * _solve_coupled (cubic_scattering/resonance_tmatrix.py:650-850) and
+ * _build_incident_field_coupled (526-593) for a batch of contrasts.
+ *   khat3, wave_s: incidence direction (normalised here) and 0 = P, 1 = S
+ *   dlambda, dmu, drho: device float64 arrays of `count` contrasts
+ *   neumann_order > 0: truncated Neumann series instead of the direct solve,
+ *     stopping per cube when |term| / |sum| < neumann_tol
+ *   tcomp: device out, count x 9 x 9 complex128 (row-major)
+ *   psi_exc: device out or NULL, count x 9N x 9 complex128 (row-major)
+ *   pivot_ratio: host out or NULL, max |u_ii| / min |u_ii| (NaN for Neumann)
+ *   neumann_iters: host out or NULL, converged order, -1 if none / direct
+ * Synchronises `stream` before returning.  A singular system is
+ * MSC_ERR_INVALID.
+ */
+int msc_resonance_tmatrices(double omega, double a, double alpha, double beta, double rho,
+                            int n_sub, const double* khat3, int wave_s, int count,
+                            const double* dlambda, const double* dmu, const double* drho,
+                            int neumann_order, double neumann_tol, void* tcomp, void* psi_exc,
+                            double* pivot_ratio, int* neumann_iters, void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* msc_last_error(void);
 

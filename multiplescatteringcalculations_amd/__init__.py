@@ -36,6 +36,12 @@ from .ocean_bottom import (
     ricker_source_spectrum,
     write_log,
 )
+from .resonance_tmatrix import (
+    ResonanceTmatrixResult,
+    compute_resonance_tmatrices,
+    compute_resonance_tmatrix,
+    suggest_n_subcells,
+)
 from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
 from .solver_config import load_config, run_from_config
 from .sweep import SweepCase, compute_slab_sweep
@@ -52,4 +58,6 @@ __all__ = [
     "FluidLayer", "IsotropicLayer", "KennettResult", "LayerStack",
     "OceanBottomConfig", "OceanBottomResult", "compute_ocean_bottom_reflection",
     "load_ocean_bottom_config", "ricker_source_spectrum", "write_log",
+    "ResonanceTmatrixResult", "compute_resonance_tmatrices", "compute_resonance_tmatrix",
+    "suggest_n_subcells",
 ]

@@ -33,6 +33,10 @@ SIGNATURES = {
     "msc_plan_reshape": (c_int, [c_void_p, c_int, c_int, c_double]),
     "msc_plan_use_graphs": (c_int, [c_void_p, c_int]),
     "msc_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "msc_resonance_tmatrices": (c_int, [c_double, c_double, c_double, c_double, c_double, c_int,
+                                        c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                        c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
     "msc_plan_info": (c_int, [c_void_p, POINTER(c_longlong)]),
     "msc_plan_autotune": (c_int, [c_void_p, c_int, c_void_p]),
     "msc_plan_tiles": (c_int, [c_void_p, POINTER(c_int)]),

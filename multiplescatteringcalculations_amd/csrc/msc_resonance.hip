@@ -1,0 +1,611 @@
+// Resonance-regime cube T-matrices: the internal Foldy-Lax system of a cube
+// split into n^3 sub-cells, solved for a batch of material contrasts
+// (msc_resonance_tmatrices, include/msc_slab.h).
+//
+// Replaces compute_resonance_tmatrix / _solve_coupled
+// (cubic_scattering/resonance_tmatrix.py:650-850): for each contrast the
+// 9N x 9N system (I - P (I x T_loc)) psi = psi_inc with 9 incident patterns
+// (N = n^3 sub-cells, P the 9x9 Kupradze propagator between sub-cell centres,
+// zero self block, T_loc the Rayleigh sub-cell T-matrix), then
+// T_comp = sum_m T_loc psi_m.  On the device:
+//   * k_res_offsets: P at the (2n-1)^3 distinct centre offsets (FP64), once;
+//   * k_res_assemble: every system's augmented column-major matrix
+//     [I - P T_loc | psi_inc] (T_loc in its 4-number form);
+//   * k_res_lu: one workgroup per system, right-looking blocked Gaussian
+//     elimination with partial pivoting (panel of NB columns and the U12 block
+//     staged in LDS, trailing update streamed from HBM with lanes along the
+//     contiguous rows), then back substitution of the 9 right-hand sides;
+//   * k_res_tcomp: the 9x9 composite T-matrix of every system.
+// The optional Neumann series of the reference (neumann_order > 0) runs as
+// repeated products with B = I - A on the same assembled matrices.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/msc_slab.h"
+#include "msc_common.hpp"
+#include "msc_prop.hpp"
+#include "msc_setup.hpp"
+
+namespace msc {
+namespace {
+
+constexpr int kRT = 256;
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) {
+  return make_double2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+// a -= b c
+__device__ __forceinline__ void zfms(double2& a, double2 b, double2 c) {
+  a.x = fma(-b.x, c.x, a.x);
+  a.x = fma(b.y, c.y, a.x);
+  a.y = fma(-b.x, c.y, a.y);
+  a.y = fma(-b.y, c.x, a.y);
+}
+
+// 9x9 propagator blocks at offsets d * (i0, i1, i2), |i| < n, index
+// ((i0 + n-1) (2n-1) + i1 + n-1) (2n-1) + i2 + n-1; zero block at the origin.
+__global__ __launch_bounds__(64) void k_res_offsets(double2* __restrict__ P, int n, double d,
+                                                   double kP, double kS, double pre) {
+  const int no = 2 * n - 1;
+  const int tot = no * no * no;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < tot; o += gridDim.x * blockDim.x) {
+    const int i0 = o / (no * no) - (n - 1);
+    const int i1 = (o / no) % no - (n - 1);
+    const int i2 = o % no - (n - 1);
+    double2 e45[45];
+    for (int e = 0; e < 45; ++e) e45[e] = make_double2(0.0, 0.0);
+    double2 K[81];
+    if (i0 != 0 || i1 != 0 || i2 != 0) {
+      add_prop45(i0 * d, i1 * d, i2 * d, kP, kS, pre, e45);
+      expand45(e45, K);
+    } else {
+      for (int q = 0; q < 81; ++q) K[q] = make_double2(0.0, 0.0);
+    }
+    for (int q = 0; q < 81; ++q) P[(size_t)o * 81 + q] = K[q];
+  }
+}
+
+// System b (blockIdx.y): A[c * n9 + r] for c < n9 is I - P T_loc, for
+// c = n9 + q the incident pattern q.  T_loc = t_u I3 (+) [[t_d, t_o, t_o], ...]
+// (+) t_s I3 (the sub-cell's Rayleigh T-matrix).
+__global__ __launch_bounds__(kRT) void k_res_assemble(double2* __restrict__ Aall, size_t stride,
+                                                     const double2* __restrict__ P,
+                                                     const double2* __restrict__ tp,
+                                                     const double2* __restrict__ psi, int n) {
+  const int N = n * n * n, n9 = 9 * N, nc = n9 + 9, no = 2 * n - 1;
+  double2* A = Aall + (size_t)blockIdx.y * stride;
+  const double2* t = tp + (size_t)blockIdx.y * 4;
+  const double2 tu = t[0], tdo = zsub(t[1], t[2]), to = t[2], ts = t[3];
+  const size_t tot = (size_t)n9 * nc;
+  for (size_t e = (size_t)blockIdx.x * kRT + threadIdx.x; e < tot; e += (size_t)gridDim.x * kRT) {
+    const int c = (int)(e / n9), r = (int)(e % n9);
+    double2 v;
+    if (c >= n9) {
+      v = psi[(size_t)(c - n9) * n9 + r];
+    } else {
+      const int m = r / 9, i = r % 9, s = c / 9, j = c % 9;
+      if (m == s) {
+        v = make_double2(i == j ? 1.0 : 0.0, 0.0);
+      } else {
+        const int d0 = m / (n * n) - s / (n * n), d1 = (m / n) % n - (s / n) % n,
+                  d2 = m % n - s % n;
+        const double2* K = P + ((size_t)((d0 + n - 1) * no + d1 + n - 1) * no + d2 + n - 1) * 81 +
+                           i * 9;
+        double2 pt;
+        if (j < 3) {
+          pt = zmul(K[j], tu);
+        } else if (j < 6) {
+          pt = zadd(zmul(K[j], tdo), zmul(zadd(zadd(K[3], K[4]), K[5]), to));
+        } else {
+          pt = zmul(K[j], ts);
+        }
+        v = make_double2(-pt.x, -pt.y);
+      }
+    }
+    A[e] = v;
+  }
+}
+
+// One workgroup per system: [A | Y] (column-major, lda n, nc columns) is
+// reduced in place to [U | U^-1 Y...] and the 9 solution columns overwrite Y.
+// Dynamic LDS: NB*n (panel) + NB*nc (U12) complex.  rhs_lds: the back
+// substitution keeps the right-hand sides in LDS (needs 9 n complex).
+template <int NB>
+__global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size_t stride, int n,
+                                               int nc, int rhs_lds, double* __restrict__ ratio,
+                                               int* __restrict__ singular) {
+  extern __shared__ double2 lds[];
+  __shared__ double red_v[kRT / 64];
+  __shared__ int red_i[kRT / 64];
+  __shared__ int piv[NB];
+  __shared__ double umin, umax;
+  __shared__ int sing;
+  __shared__ double2 xs[9];
+  double2* A = Aall + (size_t)blockIdx.x * stride;
+  double2* pan = lds;
+  double2* U = lds + (size_t)NB * n;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) {
+    umin = INFINITY;
+    umax = 0.0;
+    sing = 0;
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = min(NB, n - k0);
+    const int m = n - k0;
+    for (int e = t; e < nb * m; e += kRT) {
+      const int j = e / m, i = e - j * m;
+      pan[j * n + i] = A[(size_t)(k0 + j) * n + k0 + i];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+      double best = -1.0;
+      int bi = j;
+      for (int i = j + t; i < m; i += kRT) {
+        const double2 v = pan[j * n + i];
+        const double a = v.x * v.x + v.y * v.y;
+        if (a > best) {
+          best = a;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (lane == 0) {
+        red_v[w] = best;
+        red_i[w] = bi;
+      }
+      __syncthreads();
+      if (t == 0) {
+        double b = red_v[0];
+        int ii = red_i[0];
+        for (int q = 1; q < kRT / 64; ++q)
+          if (red_v[q] > b || (red_v[q] == b && red_i[q] < ii)) {
+            b = red_v[q];
+            ii = red_i[q];
+          }
+        piv[j] = ii;
+      }
+      __syncthreads();
+      const int p = piv[j];
+      if (p != j)
+        for (int jj = t; jj < nb; jj += kRT) {
+          const double2 a = pan[jj * n + j];
+          pan[jj * n + j] = pan[jj * n + p];
+          pan[jj * n + p] = a;
+        }
+      __syncthreads();
+      const double2 dgn = pan[j * n + j];
+      const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
+      if (t == 0) {
+        const double ad = sqrt(dd);
+        umin = fmin(umin, ad);
+        umax = fmax(umax, ad);
+        if (!(dd > 0.0)) sing = 1;
+      }
+      const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
+      for (int i = j + 1 + t; i < m; i += kRT) {
+        const double2 l = zmul(pan[j * n + i], inv);
+        pan[j * n + i] = l;
+        for (int jj = j + 1; jj < nb; ++jj) zfms(pan[jj * n + i], l, pan[jj * n + j]);
+      }
+      __syncthreads();
+    }
+    for (int e = t; e < nb * m; e += kRT) {
+      const int j = e / m, i = e - j * m;
+      A[(size_t)(k0 + j) * n + k0 + i] = pan[j * n + i];
+    }
+    // swaps and U12 = L11^-1 A12 for the columns right of the panel
+    const int c0 = k0 + nb;
+    for (int c = c0 + t; c < nc; c += kRT) {
+      double2* col = A + (size_t)c * n;
+      for (int j = 0; j < nb; ++j) {
+        const int p = k0 + piv[j];
+        if (p != k0 + j) {
+          const double2 a = col[k0 + j];
+          col[k0 + j] = col[p];
+          col[p] = a;
+        }
+      }
+      double2 x[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) x[j] = j < nb ? col[k0 + j] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int i = 1; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          if (j < i && i < nb) zfms(x[i], pan[j * n + i], x[j]);
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j < nb) col[k0 + j] = x[j];
+        U[(size_t)(c - c0) * NB + j] = x[j];
+      }
+    }
+    __syncthreads();
+    // A22 -= L21 U12: lanes along rows (contiguous), waves along columns
+    for (int ib = c0 + lane; ib < n; ib += 64) {
+      double2 l[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) l[j] = j < nb ? pan[j * n + ib - k0] : make_double2(0.0, 0.0);
+      int c = c0 + w;
+      for (; c + 3 * (kRT / 64) < nc; c += 4 * (kRT / 64)) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = A[(size_t)(c + u * (kRT / 64)) * n + ib];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double2* uu = U + (size_t)(c + u * (kRT / 64) - c0) * NB;
+#pragma unroll
+          for (int j = 0; j < NB; ++j) zfms(v[u], l[j], uu[j]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) A[(size_t)(c + u * (kRT / 64)) * n + ib] = v[u];
+      }
+      for (; c < nc; c += kRT / 64) {
+        double2 v = A[(size_t)c * n + ib];
+        const double2* uu = U + (size_t)(c - c0) * NB;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) zfms(v, l[j], uu[j]);
+        A[(size_t)c * n + ib] = v;
+      }
+    }
+    __syncthreads();
+  }
+  // back substitution U X = Y for the 9 right-hand sides (columns n..n+8)
+  double2* Y = A + (size_t)n * n;
+  double2* ys = rhs_lds ? lds : Y;
+  if (rhs_lds) {
+    for (int e = t; e < 9 * n; e += kRT) ys[e] = Y[e];
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    if (t < 9) {
+      const double2 dgn = A[(size_t)j * n + j];
+      const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
+      const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
+      const double2 x = zmul(ys[(size_t)t * n + j], inv);
+      ys[(size_t)t * n + j] = x;
+      xs[t] = x;
+    }
+    __syncthreads();
+    const double2* uc = A + (size_t)j * n;
+    for (int e = t; e < 9 * j; e += kRT) {
+      const int q = e / j, i = e - q * j;
+      zfms(ys[(size_t)q * n + i], uc[i], xs[q]);
+    }
+    __syncthreads();
+  }
+  if (rhs_lds)
+    for (int e = t; e < 9 * n; e += kRT) Y[e] = ys[e];
+  if (t == 0) {
+    if (ratio) ratio[blockIdx.x] = umin > 0.0 ? umax / umin : INFINITY;
+    if (singular) singular[blockIdx.x] = sing;
+  }
+}
+
+// T_comp[b] (row-major 9x9) = T_loc sum_m X_m, X the 9 solution columns;
+// psi_exc (optional) = X as (9N, 9) row-major.
+__global__ __launch_bounds__(kRT) void k_res_tcomp(const double2* __restrict__ Aall, size_t stride,
+                                                  const double2* __restrict__ tp, int n9,
+                                                  double2* __restrict__ tcomp,
+                                                  double2* __restrict__ psi_exc) {
+  const int b = blockIdx.x;
+  const double2* X = Aall + (size_t)b * stride + (size_t)n9 * n9;
+  __shared__ double2 part[kRT / 64][81];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // column sums s[k][q] = sum_m X[9m + k][q]: wave w, lane -> (k, q) pairs
+  for (int kq = lane; kq < 81; kq += 64) {
+    const int k = kq / 9, q = kq % 9;
+    double2 s = make_double2(0.0, 0.0);
+    for (int m = w; m < n9 / 9; m += kRT / 64) s = zadd(s, X[(size_t)q * n9 + 9 * m + k]);
+    part[w][kq] = s;
+  }
+  if (psi_exc != nullptr)
+    for (int e = t; e < n9 * 9; e += kRT) {
+      const int r = e / 9, q = e % 9;
+      psi_exc[(size_t)b * n9 * 9 + e] = X[(size_t)q * n9 + r];
+    }
+  __syncthreads();
+  if (t < 81) {
+    const int i = t / 9, q = t % 9;
+    double2 s[9];
+    for (int k = 0; k < 9; ++k) {
+      s[k] = make_double2(0.0, 0.0);
+      for (int ww = 0; ww < kRT / 64; ++ww) s[k] = zadd(s[k], part[ww][k * 9 + q]);
+    }
+    const double2* tl = tp + (size_t)b * 4;
+    double2 v;
+    if (i < 3) {
+      v = zmul(tl[0], s[i]);
+    } else if (i < 6) {
+      v = zadd(zmul(zsub(tl[1], tl[2]), s[i]), zmul(tl[2], zadd(zadd(s[3], s[4]), s[5])));
+    } else {
+      v = zmul(tl[3], s[i]);
+    }
+    tcomp[(size_t)b * 81 + t] = v;
+  }
+}
+
+// Neumann step: out = (I - A) in = in - A in over the n9 x n9 part, 9 columns;
+// sum += out; norms[b] = (|out|^2, |sum|^2).
+__global__ __launch_bounds__(kRT) void k_res_neumann(const double2* __restrict__ Aall,
+                                                    size_t stride, int n9,
+                                                    const double2* __restrict__ tin,
+                                                    double2* __restrict__ tout,
+                                                    double2* __restrict__ acc,
+                                                    double* __restrict__ norms,
+                                                    const int* __restrict__ active) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;  // converged: its sum stays as it is
+  const double2* A = Aall + (size_t)b * stride;
+  const double2* x = tin + (size_t)b * n9 * 9;
+  double2* y = tout + (size_t)b * n9 * 9;
+  double2* s = acc + (size_t)b * n9 * 9;
+  double nt = 0.0, ns = 0.0;
+  for (int e = blockIdx.x * kRT + threadIdx.x; e < n9 * 9; e += gridDim.x * kRT) {
+    const int q = e / n9, r = e - q * n9;
+    double2 v = x[(size_t)q * n9 + r];
+    for (int c = 0; c < n9; ++c) zfms(v, A[(size_t)c * n9 + r], x[(size_t)q * n9 + c]);
+    y[e] = v;
+    const double2 sv = zadd(s[e], v);
+    s[e] = sv;
+    nt += v.x * v.x + v.y * v.y;
+    ns += sv.x * sv.x + sv.y * sv.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nt += __shfl_xor(nt, off);
+    ns += __shfl_xor(ns, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(norms + 2 * b, nt);
+    atomicAdd(norms + 2 * b + 1, ns);
+  }
+}
+
+__global__ void k_zero_d(double* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// The 9 incident patterns at the sub-cell centres (resonance_tmatrix.py:526-593),
+// column-major (9N, 9): uniform displacement e_p (columns 0-2); linear
+// displacement eps(e_al) (x - x_c) with uniform Voigt strain e_al (3-8);
+// all times exp(i k khat.x).
+std::vector<double2> incident_patterns(int n, double a, double k, const double* kh) {
+  const int N = n * n * n, n9 = 9 * N;
+  std::vector<double> coord(n);
+  const double as = a / n;
+  for (int i = 0; i < n; ++i)
+    coord[i] = n == 1 ? 0.0 : (-a + as) + i * ((a - as) - (-a + as)) / (n - 1);
+  if (n > 1) coord[n - 1] = a - as;
+  std::vector<double> cx((size_t)N * 3);
+  double mean[3] = {0.0, 0.0, 0.0};
+  for (int m = 0; m < N; ++m) {
+    const int i0 = m / (n * n), i1 = (m / n) % n, i2 = m % n;
+    cx[3 * m] = coord[i0];
+    cx[3 * m + 1] = coord[i1];
+    cx[3 * m + 2] = coord[i2];
+    for (int d = 0; d < 3; ++d) mean[d] += cx[3 * m + d];
+  }
+  for (int d = 0; d < 3; ++d) mean[d] /= N;
+  const int VP[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+  std::vector<double2> psi((size_t)n9 * 9, make_double2(0.0, 0.0));
+  for (int m = 0; m < N; ++m) {
+    const double ph = k * (cx[3 * m] * kh[0] + cx[3 * m + 1] * kh[1] + cx[3 * m + 2] * kh[2]);
+    const double2 e = make_double2(std::cos(ph), std::sin(ph));
+    const double dx[3] = {cx[3 * m] - mean[0], cx[3 * m + 1] - mean[1], cx[3 * m + 2] - mean[2]};
+    for (int p = 0; p < 3; ++p) psi[(size_t)p * n9 + 9 * m + p] = e;
+    for (int al = 0; al < 6; ++al) {
+      const int p = VP[al][0], q = VP[al][1];
+      double u[3] = {0.0, 0.0, 0.0};
+      if (p == q) {
+        u[p] = dx[p];
+      } else {
+        u[p] = 0.5 * dx[q];
+        u[q] = 0.5 * dx[p];
+      }
+      double2* col = psi.data() + (size_t)(3 + al) * n9 + 9 * m;
+      for (int i = 0; i < 3; ++i) col[i] = make_double2(u[i] * e.x, u[i] * e.y);
+      col[3 + al] = e;
+    }
+  }
+  return psi;
+}
+
+template <int NB>
+int launch_lu(double2* A, size_t stride, int n, int nc, int count, double* ratio, int* sing,
+              hipStream_t st) {
+  const size_t lds = (size_t)NB * (n + nc) * sizeof(double2);
+  const int rhs_lds = (size_t)9 * n * sizeof(double2) <= lds ? 1 : 0;
+  MSC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_lu<NB>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_res_lu<NB>, dim3(count), dim3(kRT), lds, st, A, stride, n, nc, rhs_lds,
+                     ratio, sing);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace
+}  // namespace msc
+
+using namespace msc;
+
+extern "C" int msc_resonance_tmatrices(double omega, double a, double alpha, double beta,
+                                       double rho, int n_sub, const double* khat3, int wave_s,
+                                       int count, const double* dlambda, const double* dmu,
+                                       const double* drho, int neumann_order, double neumann_tol,
+                                       void* tcomp, void* psi_exc, double* pivot_ratio,
+                                       int* neumann_iters, void* stream) {
+  set_error("");
+  MSC_REQUIRE(omega > 0.0 && a > 0.0 && alpha > 0.0 && beta > 0.0 && rho > 0.0,
+              "omega, a, alpha, beta, rho must be positive");
+  MSC_REQUIRE(n_sub >= 1, "n_sub must be >= 1");
+  MSC_REQUIRE(count >= 0, "count must be >= 0");
+  MSC_REQUIRE(count == 0 || (dlambda && dmu && drho && tcomp), "null contrast or output pointer");
+  MSC_REQUIRE(khat3 != nullptr, "khat must not be NULL");
+  MSC_REQUIRE(wave_s == 0 || wave_s == 1, "wave_type must be 0 (P) or 1 (S)");
+  MSC_REQUIRE(neumann_order >= 0, "neumann_order must be >= 0");
+  const double kn = std::sqrt(khat3[0] * khat3[0] + khat3[1] * khat3[1] + khat3[2] * khat3[2]);
+  MSC_REQUIRE(kn > 0.0, "khat must be non-zero");
+  const int n = n_sub, N = n * n * n, n9 = 9 * N, nc = n9 + 9;
+  int nb = 8;
+  const size_t lds_cap = 160 * 1024 - 1024;
+  while (nb > 1 && (size_t)nb * (n9 + nc) * sizeof(double2) > lds_cap) nb >>= 1;
+  if ((size_t)nb * (n9 + nc) * sizeof(double2) > lds_cap) {
+    set_error("n_sub too large for the resonance solver (at most 8)");
+    return kUnsupported;
+  }
+  if (count == 0) return kOk;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const double as = a / n, kP = omega / alpha, kS = omega / beta;
+  const double kh[3] = {khat3[0] / kn, khat3[1] / kn, khat3[2] / kn};
+  const std::vector<double2> psi = incident_patterns(n, a, wave_s ? kS : kP, kh);
+  double coef[8];
+  int rc = msc_cube_coefficients(omega, as, alpha, beta, rho, coef);
+  if (rc) return rc;
+  const size_t stride = (size_t)n9 * nc;
+  const size_t budget = (size_t)4 << 30;
+  const int chunk = (int)std::max<size_t>(
+      1, std::min<size_t>(std::min<size_t>(count, 65535), budget / (stride * 16)));
+  const int no = 2 * n - 1;
+  double2 *dP = nullptr, *dpsi = nullptr, *dtp = nullptr, *dA = nullptr, *dt0 = nullptr,
+          *dt1 = nullptr, *dacc = nullptr;
+  double *dratio = nullptr, *dnorm = nullptr;
+  int *dsing = nullptr, *dact = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)dP, (void*)dpsi, (void*)dtp, (void*)dA, (void*)dt0, (void*)dt1,
+                    (void*)dacc, (void*)dratio, (void*)dnorm, (void*)dsing, (void*)dact})
+      if (p) (void)hipFree(p);
+  };
+#define RES_CHECK(expr)                                                   \
+  do {                                                                    \
+    hipError_t _e = (expr);                                               \
+    if (_e != hipSuccess) {                                               \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));       \
+      cleanup();                                                          \
+      return kHip;                                                        \
+    }                                                                     \
+  } while (0)
+  RES_CHECK(hipMalloc(&dP, (size_t)no * no * no * 81 * sizeof(double2)));
+  RES_CHECK(hipMalloc(&dpsi, psi.size() * sizeof(double2)));
+  RES_CHECK(hipMalloc(&dtp, (size_t)count * 4 * sizeof(double2)));
+  RES_CHECK(hipMalloc(&dA, (size_t)chunk * stride * sizeof(double2)));
+  RES_CHECK(hipMalloc(&dratio, (size_t)chunk * sizeof(double)));
+  RES_CHECK(hipMalloc(&dsing, (size_t)chunk * sizeof(int)));
+  if (neumann_order > 0) {
+    RES_CHECK(hipMalloc(&dt0, (size_t)chunk * n9 * 9 * sizeof(double2)));
+    RES_CHECK(hipMalloc(&dt1, (size_t)chunk * n9 * 9 * sizeof(double2)));
+    RES_CHECK(hipMalloc(&dacc, (size_t)chunk * n9 * 9 * sizeof(double2)));
+    RES_CHECK(hipMalloc(&dnorm, (size_t)chunk * 2 * sizeof(double)));
+    RES_CHECK(hipMalloc(&dact, (size_t)chunk * sizeof(int)));
+  }
+  RES_CHECK(hipMemcpyAsync(dpsi, psi.data(), psi.size() * sizeof(double2), hipMemcpyHostToDevice,
+                           st));
+  hipLaunchKernelGGL(k_res_offsets, dim3((no * no * no + 63) / 64), dim3(64), 0, st, dP, n,
+                     2.0 * as, kP, kS, 1.0 / (4.0 * M_PI * rho * omega * omega));
+  RES_CHECK(hipGetLastError());
+  rc = tparams_batch(dlambda, dmu, drho, (size_t)count, omega, coef, 8.0 * as * as * as, dtp, st);
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  std::vector<double> hratio(chunk);
+  std::vector<int> hsing(chunk);
+  std::vector<double> hnorm(2 * (size_t)chunk);
+  double2* out = static_cast<double2*>(tcomp);
+  double2* pex = static_cast<double2*>(psi_exc);
+  for (int b0 = 0; b0 < count; b0 += chunk) {
+    const int cb = std::min(chunk, count - b0);
+    const int gx = (int)std::min<size_t>((stride + kRT - 1) / kRT, 2048);
+    hipLaunchKernelGGL(k_res_assemble, dim3(gx, cb), dim3(kRT), 0, st, dA, stride, dP,
+                       dtp + (size_t)b0 * 4, dpsi, n);
+    RES_CHECK(hipGetLastError());
+    if (neumann_order == 0) {
+      switch (nb) {
+        case 8: rc = launch_lu<8>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+        case 4: rc = launch_lu<4>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+        case 2: rc = launch_lu<2>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+        default: rc = launch_lu<1>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+      }
+      if (rc) {
+        cleanup();
+        return rc;
+      }
+      RES_CHECK(hipMemcpyAsync(hratio.data(), dratio, cb * sizeof(double), hipMemcpyDeviceToHost,
+                               st));
+      RES_CHECK(hipMemcpyAsync(hsing.data(), dsing, cb * sizeof(int), hipMemcpyDeviceToHost, st));
+      RES_CHECK(hipStreamSynchronize(st));
+      for (int i = 0; i < cb; ++i) {
+        if (hsing[i]) {
+          set_error("the internal Foldy-Lax matrix is singular");
+          cleanup();
+          return kInvalid;
+        }
+        if (pivot_ratio) pivot_ratio[b0 + i] = hratio[i];
+        if (neumann_iters) neumann_iters[b0 + i] = -1;
+      }
+    } else {
+      // psi = sum_k B^k psi_inc: the incident columns are the first term
+      for (int b = 0; b < cb; ++b) {
+        RES_CHECK(hipMemcpyAsync(dt0 + (size_t)b * n9 * 9, dpsi, (size_t)n9 * 9 * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, st));
+        RES_CHECK(hipMemcpyAsync(dacc + (size_t)b * n9 * 9, dpsi,
+                                 (size_t)n9 * 9 * sizeof(double2), hipMemcpyDeviceToDevice, st));
+      }
+      std::vector<int> conv(cb, -1), act(cb, 1);
+      double2 *tin = dt0, *tout = dt1;
+      for (int k = 1; k <= neumann_order; ++k) {
+        RES_CHECK(hipMemcpyAsync(dact, act.data(), cb * sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, st, dnorm, 2 * cb);
+        const int gxn = (n9 * 9 + kRT - 1) / kRT;
+        hipLaunchKernelGGL(k_res_neumann, dim3(gxn, cb), dim3(kRT), 0, st, dA, stride, n9, tin,
+                           tout, dacc, dnorm, dact);
+        RES_CHECK(hipGetLastError());
+        RES_CHECK(hipMemcpyAsync(hnorm.data(), dnorm, 2 * cb * sizeof(double),
+                                 hipMemcpyDeviceToHost, st));
+        RES_CHECK(hipStreamSynchronize(st));
+        bool all = true;
+        for (int i = 0; i < cb; ++i) {
+          const double rel = std::sqrt(hnorm[2 * i]) / (std::sqrt(hnorm[2 * i + 1]) + 1e-300);
+          if (conv[i] < 0 && act[i] && rel < neumann_tol) conv[i] = k;
+          if (conv[i] >= 0) act[i] = 0;
+          all = all && conv[i] >= 0;
+        }
+        std::swap(tin, tout);
+        if (all) break;
+      }
+      // the Neumann sums replace the solution columns
+      for (int b = 0; b < cb; ++b)
+        RES_CHECK(hipMemcpyAsync(dA + (size_t)b * stride + (size_t)n9 * n9,
+                                 dacc + (size_t)b * n9 * 9, (size_t)n9 * 9 * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, st));
+      for (int i = 0; i < cb; ++i) {
+        if (pivot_ratio) pivot_ratio[b0 + i] = NAN;
+        if (neumann_iters) neumann_iters[b0 + i] = conv[i];
+      }
+    }
+    hipLaunchKernelGGL(k_res_tcomp, dim3(cb), dim3(kRT), 0, st, dA, stride, dtp + (size_t)b0 * 4,
+                       n9, out + (size_t)b0 * 81, pex ? pex + (size_t)b0 * n9 * 9 : nullptr);
+    RES_CHECK(hipGetLastError());
+  }
+  RES_CHECK(hipStreamSynchronize(st));
+#undef RES_CHECK
+  cleanup();
+  return kOk;
+}

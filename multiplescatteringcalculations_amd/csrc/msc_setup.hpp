@@ -8,6 +8,10 @@ namespace msc {
 int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, const double* Dr,
                                double omega, const double* coef8, hipStream_t st);
 int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st);
+// (t_u, t_d, t_o, t_s) of `count` cubes of volume V from contrast arrays
+// (device), with the frequency constants of msc_cube_coefficients.
+int tparams_batch(const double* Dl, const double* Dm, const double* Dr, size_t count,
+                  double omega, const double* coef8, double V, double2* t128, hipStream_t st);
 int incident_field(Plan& p, double k_mag, const double* khat3, const double* inc18, double2* psi0,
                    hipStream_t st);
 // The 26 volume-averaged nearest-neighbour 9x9 blocks (row-major, real) for
