@@ -26,6 +26,13 @@ from .slab_scattering import (
     slab_rpp_periodic,
     uniform_slab_material,
 )
+from .cpa_iteration import (
+    CPAResult,
+    CubicEffectiveMedium,
+    Phase,
+    compute_cpa,
+    compute_cpa_two_phase,
+)
 from .kennett_layers import FluidLayer, IsotropicLayer, KennettResult, LayerStack
 from .learned_precond import LearnedPreconditioner, harvest_pairs, merge_pairs
 from .ocean_bottom import (
@@ -59,5 +66,6 @@ __all__ = [
     "OceanBottomConfig", "OceanBottomResult", "compute_ocean_bottom_reflection",
     "load_ocean_bottom_config", "ricker_source_spectrum", "write_log",
     "ResonanceTmatrixResult", "compute_resonance_tmatrices", "compute_resonance_tmatrix",
-    "suggest_n_subcells",
+    "suggest_n_subcells", "CPAResult", "CubicEffectiveMedium", "Phase", "compute_cpa",
+    "compute_cpa_two_phase",
 ]

@@ -116,50 +116,59 @@ __global__ __launch_bounds__(kRT) void k_res_assemble(double2* __restrict__ Aall
   }
 }
 
-// One workgroup per system: [A | Y] (column-major, lda n, nc columns) is
-// reduced in place to [U | U^-1 Y...] and the 9 solution columns overwrite Y.
-// Dynamic LDS: NB*n (panel) + NB*nc (U12) complex.  rhs_lds: the back
-// substitution keeps the right-hand sides in LDS (needs 9 n complex).
+// One workgroup (kLT threads) per system: [A | Y] (column-major, lda n, nc
+// columns) is reduced in place to [U | Y'] by right-looking blocked
+// elimination with partial pivoting, then the 9 solution columns overwrite Y.
+// One dynamic LDS region of NB * max(n, nc) complex holds, per panel of NB
+// columns, first the panel (factorised in LDS), then the U12 block; L11 is
+// copied to a small array and each thread reads its rows of L21 from global
+// memory (just written, L2-resident) into registers for the trailing update.
+// Per pivot: the maxima of the next column are found while the current one is
+// applied, so two barriers per column.  rhs_lds: the back substitution keeps
+// the right-hand sides in LDS.
+constexpr int kLT = 512;
+constexpr int kLW = kLT / 64;
 template <int NB>
-__global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size_t stride, int n,
+__global__ __launch_bounds__(kLT) void k_res_lu(double2* __restrict__ Aall, size_t stride, int n,
                                                int nc, int rhs_lds, double* __restrict__ ratio,
                                                int* __restrict__ singular) {
   extern __shared__ double2 lds[];
-  __shared__ double red_v[kRT / 64];
-  __shared__ int red_i[kRT / 64];
+  __shared__ double red_v[kLW];
+  __shared__ int red_i[kLW];
   __shared__ int piv[NB];
+  __shared__ double2 L11[NB * NB];
+  __shared__ double2 prow[NB];
   __shared__ double umin, umax;
   __shared__ int sing;
   __shared__ double2 xs[9];
   double2* A = Aall + (size_t)blockIdx.x * stride;
-  double2* pan = lds;
-  double2* U = lds + (size_t)NB * n;
+  double2* pan = lds;  // panel: column j at pan + j * n (local rows)
+  double2* U = lds;    // then U12: column c - c0 at U + (c - c0) * NB
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) {
     umin = INFINITY;
     umax = 0.0;
     sing = 0;
   }
-  __syncthreads();
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = min(NB, n - k0);
     const int m = n - k0;
-    for (int e = t; e < nb * m; e += kRT) {
+    for (int e = t; e < nb * m; e += kLT) {
       const int j = e / m, i = e - j * m;
       pan[j * n + i] = A[(size_t)(k0 + j) * n + k0 + i];
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-      double best = -1.0;
-      int bi = j;
-      for (int i = j + t; i < m; i += kRT) {
-        const double2 v = pan[j * n + i];
-        const double a = v.x * v.x + v.y * v.y;
-        if (a > best) {
-          best = a;
-          bi = i;
-        }
+    double best = -1.0;
+    int bi = 0;
+    for (int i = t; i < m; i += kLT) {
+      const double2 v = pan[i];
+      const double a = v.x * v.x + v.y * v.y;
+      if (a > best) {
+        best = a;
+        bi = i;
       }
+    }
+    for (int j = 0; j < nb; ++j) {
       for (int off = 32; off > 0; off >>= 1) {
         const double ob = __shfl_xor(best, off);
         const int oi = __shfl_xor(bi, off);
@@ -173,26 +182,22 @@ __global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size
         red_i[w] = bi;
       }
       __syncthreads();
-      if (t == 0) {
-        double b = red_v[0];
-        int ii = red_i[0];
-        for (int q = 1; q < kRT / 64; ++q)
-          if (red_v[q] > b || (red_v[q] == b && red_i[q] < ii)) {
-            b = red_v[q];
-            ii = red_i[q];
-          }
-        piv[j] = ii;
-      }
-      __syncthreads();
-      const int p = piv[j];
-      if (p != j)
-        for (int jj = t; jj < nb; jj += kRT) {
-          const double2 a = pan[jj * n + j];
-          pan[jj * n + j] = pan[jj * n + p];
-          pan[jj * n + p] = a;
+      double b = red_v[0];
+      int p = red_i[0];
+      for (int q = 1; q < kLW; ++q)
+        if (red_v[q] > b || (red_v[q] == b && red_i[q] < p)) {
+          b = red_v[q];
+          p = red_i[q];
         }
+      if (t < nb) prow[t] = pan[t * n + p];
+      if (t == 0) piv[j] = p;
       __syncthreads();
-      const double2 dgn = pan[j * n + j];
+      // rows j and p exchange (old row p = prow is the pivot row)
+      if (p != j && t < nb) {
+        pan[t * n + p] = pan[t * n + j];
+        pan[t * n + j] = prow[t];
+      }
+      const double2 dgn = prow[j];
       const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
       if (t == 0) {
         const double ad = sqrt(dd);
@@ -201,20 +206,38 @@ __global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size
         if (!(dd > 0.0)) sing = 1;
       }
       const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
-      for (int i = j + 1 + t; i < m; i += kRT) {
+      __syncthreads();
+      // scale column j, update the panel columns right of it, and find the
+      // maxima of column j + 1 on the way
+      best = -1.0;
+      bi = j + 1;
+      for (int i = j + 1 + t; i < m; i += kLT) {
         const double2 l = zmul(pan[j * n + i], inv);
         pan[j * n + i] = l;
-        for (int jj = j + 1; jj < nb; ++jj) zfms(pan[jj * n + i], l, pan[jj * n + j]);
+        for (int jj = j + 1; jj < nb; ++jj) zfms(pan[jj * n + i], l, prow[jj]);
+        if (j + 1 < nb) {
+          const double2 v = pan[(j + 1) * n + i];
+          const double a = v.x * v.x + v.y * v.y;
+          if (a > best) {
+            best = a;
+            bi = i;
+          }
+        }
       }
-      __syncthreads();
     }
-    for (int e = t; e < nb * m; e += kRT) {
+    __syncthreads();
+    for (int e = t; e < nb * m; e += kLT) {
       const int j = e / m, i = e - j * m;
       A[(size_t)(k0 + j) * n + k0 + i] = pan[j * n + i];
+      if (i < nb) L11[i * NB + j] = pan[j * n + i];
     }
+    __syncthreads();
     // swaps and U12 = L11^-1 A12 for the columns right of the panel
     const int c0 = k0 + nb;
-    for (int c = c0 + t; c < nc; c += kRT) {
+#pragma unroll 1
+    for (int c = c0 + t; c < nc; c += kLT) {
+      // keeps the L11 reads inside the loop (hoisting all NB^2 of them spills)
+      asm volatile("" ::: "memory");
       double2* col = A + (size_t)c * n;
       for (int j = 0; j < nb; ++j) {
         const int p = k0 + piv[j];
@@ -228,10 +251,14 @@ __global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size
 #pragma unroll
       for (int j = 0; j < NB; ++j) x[j] = j < nb ? col[k0 + j] : make_double2(0.0, 0.0);
 #pragma unroll
-      for (int i = 1; i < NB; ++i)
+      for (int i = 1; i < NB; ++i) {
 #pragma unroll
         for (int j = 0; j < NB; ++j)
-          if (j < i && i < nb) zfms(x[i], pan[j * n + i], x[j]);
+          if (j < i) zfms(x[i], L11[i * NB + j], x[j]);
+        // x[i] is final here and the next row's L11 reads stay below: one
+        // row of L11 live at a time (all NB^2 at once spills)
+        asm volatile("" : "+v"(x[i].x), "+v"(x[i].y) : : "memory");
+      }
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         if (j < nb) col[k0 + j] = x[j];
@@ -240,25 +267,30 @@ __global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size
     }
     __syncthreads();
     // A22 -= L21 U12: lanes along rows (contiguous), waves along columns
+#pragma unroll 1
     for (int ib = c0 + lane; ib < n; ib += 64) {
       double2 l[NB];
 #pragma unroll
-      for (int j = 0; j < NB; ++j) l[j] = j < nb ? pan[j * n + ib - k0] : make_double2(0.0, 0.0);
+      for (int j = 0; j < NB; ++j)
+        l[j] = j < nb ? A[(size_t)(k0 + j) * n + ib] : make_double2(0.0, 0.0);
+      constexpr int CU = 4;  // columns in flight per thread
       int c = c0 + w;
-      for (; c + 3 * (kRT / 64) < nc; c += 4 * (kRT / 64)) {
-        double2 v[4];
+#pragma unroll 1
+      for (; c + (CU - 1) * kLW < nc; c += CU * kLW) {
+        double2 v[CU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = A[(size_t)(c + u * (kRT / 64)) * n + ib];
+        for (int u = 0; u < CU; ++u) v[u] = A[(size_t)(c + u * kLW) * n + ib];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double2* uu = U + (size_t)(c + u * (kRT / 64) - c0) * NB;
+        for (int u = 0; u < CU; ++u) {
+          const double2* uu = U + (size_t)(c + u * kLW - c0) * NB;
 #pragma unroll
           for (int j = 0; j < NB; ++j) zfms(v[u], l[j], uu[j]);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) A[(size_t)(c + u * (kRT / 64)) * n + ib] = v[u];
+        for (int u = 0; u < CU; ++u) A[(size_t)(c + u * kLW) * n + ib] = v[u];
       }
-      for (; c < nc; c += kRT / 64) {
+#pragma unroll 1
+      for (; c < nc; c += kLW) {
         double2 v = A[(size_t)c * n + ib];
         const double2* uu = U + (size_t)(c - c0) * NB;
 #pragma unroll
@@ -272,7 +304,7 @@ __global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size
   double2* Y = A + (size_t)n * n;
   double2* ys = rhs_lds ? lds : Y;
   if (rhs_lds) {
-    for (int e = t; e < 9 * n; e += kRT) ys[e] = Y[e];
+    for (int e = t; e < 9 * n; e += kLT) ys[e] = Y[e];
     __syncthreads();
   }
   for (int j = n - 1; j >= 0; --j) {
@@ -286,14 +318,14 @@ __global__ __launch_bounds__(kRT) void k_res_lu(double2* __restrict__ Aall, size
     }
     __syncthreads();
     const double2* uc = A + (size_t)j * n;
-    for (int e = t; e < 9 * j; e += kRT) {
+    for (int e = t; e < 9 * j; e += kLT) {
       const int q = e / j, i = e - q * j;
       zfms(ys[(size_t)q * n + i], uc[i], xs[q]);
     }
     __syncthreads();
   }
   if (rhs_lds)
-    for (int e = t; e < 9 * n; e += kRT) Y[e] = ys[e];
+    for (int e = t; e < 9 * n; e += kLT) Y[e] = ys[e];
   if (t == 0) {
     if (ratio) ratio[blockIdx.x] = umin > 0.0 ? umax / umin : INFINITY;
     if (singular) singular[blockIdx.x] = sing;
@@ -428,14 +460,18 @@ std::vector<double2> incident_patterns(int n, double a, double k, const double* 
   return psi;
 }
 
+// Dynamic LDS of the elimination kernel for panel width nb.
+size_t lu_lds(int nb, int n, int nc) { return (size_t)nb * std::max(n, nc) * sizeof(double2); }
+constexpr size_t kLuLdsCap = 150 * 1024;  // leaves room for the static arrays (NB = 16)
+
 template <int NB>
 int launch_lu(double2* A, size_t stride, int n, int nc, int count, double* ratio, int* sing,
               hipStream_t st) {
-  const size_t lds = (size_t)NB * (n + nc) * sizeof(double2);
+  const size_t lds = lu_lds(NB, n, nc);
   const int rhs_lds = (size_t)9 * n * sizeof(double2) <= lds ? 1 : 0;
   MSC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_lu<NB>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_res_lu<NB>, dim3(count), dim3(kRT), lds, st, A, stride, n, nc, rhs_lds,
+  hipLaunchKernelGGL(k_res_lu<NB>, dim3(count), dim3(kLT), lds, st, A, stride, n, nc, rhs_lds,
                      ratio, sing);
   MSC_LAUNCH_CHECK();
   return kOk;
@@ -464,10 +500,9 @@ extern "C" int msc_resonance_tmatrices(double omega, double a, double alpha, dou
   const double kn = std::sqrt(khat3[0] * khat3[0] + khat3[1] * khat3[1] + khat3[2] * khat3[2]);
   MSC_REQUIRE(kn > 0.0, "khat must be non-zero");
   const int n = n_sub, N = n * n * n, n9 = 9 * N, nc = n9 + 9;
-  int nb = 8;
-  const size_t lds_cap = 160 * 1024 - 1024;
-  while (nb > 1 && (size_t)nb * (n9 + nc) * sizeof(double2) > lds_cap) nb >>= 1;
-  if ((size_t)nb * (n9 + nc) * sizeof(double2) > lds_cap) {
+  int nb = 16;
+  while (nb > 1 && lu_lds(nb, n9, nc) > kLuLdsCap) nb >>= 1;
+  if (n_sub > 8 || lu_lds(nb, n9, nc) > kLuLdsCap) {
     set_error("n_sub too large for the resonance solver (at most 8)");
     return kUnsupported;
   }
@@ -538,6 +573,7 @@ extern "C" int msc_resonance_tmatrices(double omega, double a, double alpha, dou
     RES_CHECK(hipGetLastError());
     if (neumann_order == 0) {
       switch (nb) {
+        case 16: rc = launch_lu<16>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
         case 8: rc = launch_lu<8>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
         case 4: rc = launch_lu<4>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
         case 2: rc = launch_lu<2>(dA, stride, n9, nc, cb, dratio, dsing, st); break;

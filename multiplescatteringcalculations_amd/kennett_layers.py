@@ -335,3 +335,64 @@ def kennett_layers(stack: LayerStack, p: float, omega) -> KennettResult:
             Msh = eb * eb * Rsh
             Rsh = sh.Rd + sh.Tu * Msh * sh.Td / (1.0 - sh.Ru * Msh)
     return KennettResult(RD_psv=R, RD_sh=Rsh, omega=omega, p=p, stack=stack)
+
+
+# ── CPA bridge (kennett_layers.py:916-1097) ─────────────────────────────
+
+
+def cubic_to_isotropic_layer(eff, thickness: float, Q_alpha: float = np.inf,
+                             Q_beta: float = np.inf) -> IsotropicLayer:
+    """Isotropic layer from a CPA medium with mu = mu_off."""
+    return IsotropicLayer(alpha=float(np.sqrt((eff.lam + 2.0 * eff.mu_off) / eff.rho)),
+                          beta=float(np.sqrt(eff.mu_off / eff.rho)), rho=float(eff.rho),
+                          thickness=thickness, Q_alpha=Q_alpha, Q_beta=Q_beta)
+
+
+def cpa_stack_from_phases(layer_phases, omega: float, a: float, thickness: float | None = None,
+                          Q_alpha: float = np.inf, Q_beta: float = np.inf,
+                          half_space_index: int = -1, **cpa_kwargs) -> LayerStack:
+    """One CPA-homogenised layer per phase list; thickness defaults to one cube (2a)."""
+    from .cpa_iteration import compute_cpa
+
+    h0 = 2.0 * a if thickness is None else thickness
+    n = len(layer_phases)
+    lays = []
+    for i, ph in enumerate(layer_phases):
+        eff = compute_cpa(ph, omega, a, **cpa_kwargs).effective_medium
+        half = (i == n - 1) if half_space_index == -1 else (i == half_space_index)
+        lays.append(cubic_to_isotropic_layer(eff, np.inf if half else h0, Q_alpha, Q_beta))
+    return LayerStack(layers=lays)
+
+
+def random_heterogeneous_stack(ref_alpha, ref_beta, ref_rho, n_layers, a, omega, Dlambda, Dmu,
+                               Drho, phi_mean=0.5, phi_std=0.1, seed=None,
+                               **cpa_kwargs) -> LayerStack:
+    """Two-phase layers with phi ~ N(phi_mean, phi_std^2) clipped to [0.01, 0.99],
+    CPA-homogenised (same draw order as the reference)."""
+    from .cpa_iteration import Phase
+
+    rng = np.random.default_rng(seed)
+    lam = ref_rho * (ref_alpha ** 2 - 2 * ref_beta ** 2)
+    mu = ref_rho * ref_beta ** 2
+    phases = []
+    for _ in range(n_layers):
+        phi = float(np.clip(rng.normal(phi_mean, phi_std), 0.01, 0.99))
+        phases.append([Phase(lam, mu, ref_rho, 1.0 - phi),
+                       Phase(lam + Dlambda, mu + Dmu, ref_rho + Drho, phi)])
+    return cpa_stack_from_phases(phases, omega, a, thickness=2.0 * a, **cpa_kwargs)
+
+
+def random_velocity_stack(alpha_mean, beta_mean, rho_mean, n_layers, thickness, dalpha_std=0.0,
+                          dbeta_std=0.0, drho_std=0.0, seed=None, Q_alpha=np.inf,
+                          Q_beta=np.inf) -> LayerStack:
+    """Layers with normally perturbed velocities and density (floored at half the
+    mean), the last one a half-space (same draw order as the reference)."""
+    rng = np.random.default_rng(seed)
+    lays = []
+    for i in range(n_layers):
+        al = max(alpha_mean + rng.normal(0, dalpha_std), alpha_mean * 0.5)
+        be = max(beta_mean + rng.normal(0, dbeta_std), beta_mean * 0.5)
+        rh = max(rho_mean + rng.normal(0, drho_std), rho_mean * 0.5)
+        lays.append(IsotropicLayer(float(al), float(be), float(rh),
+                                   np.inf if i == n_layers - 1 else thickness, Q_alpha, Q_beta))
+    return LayerStack(layers=lays)

@@ -1,0 +1,94 @@
+"""CPU tests of the CPA iteration (multiplescatteringcalculations_amd.
+cpa_iteration) and its Kennett bridge against the oracle (oracle/cpa.py) and
+the CPA condition, with the reference's checks
+(cubic_scattering/tests/test_cpa_iteration.py:40-300)."""
+
+import numpy as np
+import pytest
+
+import multiplescatteringcalculations_amd as msc
+from multiplescatteringcalculations_amd import cpa_iteration as cp
+from multiplescatteringcalculations_amd import kennett_layers as kl
+from oracle import cpa as ocpa
+from oracle import tmatrix as otm
+
+REF = msc.ReferenceMedium(alpha=5000.0, beta=3000.0, rho=2500.0)
+A = 1.0
+OMEGA_LOW = 0.01 * REF.beta / A
+CONTRAST = msc.MaterialContrast(Dlambda=2e9, Dmu=1e9, Drho=100.0)
+WEAK = msc.MaterialContrast(Dlambda=REF.mu * 1e-4, Dmu=REF.mu * 1e-4, Drho=REF.rho * 1e-4)
+
+
+@pytest.mark.parametrize("contrast,phi,om,damping", [
+    (CONTRAST, 0.3, OMEGA_LOW, 1.0), (CONTRAST, 0.5, 0.3 * REF.beta, 1.0),
+    (msc.MaterialContrast(-3e9, -2e9, -300.0), 0.4, 0.2 * REF.beta, 0.7)])
+def test_cpa_matches_oracle_and_satisfies_the_cpa_condition(contrast, phi, om, damping):
+    res = cp.compute_cpa_two_phase(REF, contrast, phi, om, A, damping=damping, max_iter=300)
+    assert res.converged
+    e = res.effective_medium
+    ph = [(REF.lam, REF.mu, REF.rho, 1 - phi),
+          (REF.lam + contrast.Dlambda, REF.mu + contrast.Dmu, REF.rho + contrast.Drho, phi)]
+    lam, mo, md, rho, it, ok = ocpa.cpa(ph, om, A, damping=damping, max_iter=300)
+    assert ok and it == res.n_iterations
+    np.testing.assert_allclose([e.lam, e.mu_off, e.mu_diag, e.rho], [lam, mo, md, rho],
+                               rtol=1e-11)
+    # <T> = 0: the averaged effective contrasts in the converged medium vanish
+    ref = ocpa.medium(e.lam, e.mu_off, e.rho)
+    s = np.zeros(3)
+    for la, mu, r, f in ph:
+        t = otm.cube_tmatrix(om, A, ref, la - e.lam, mu - e.mu_off, r - e.rho)
+        s += f * np.array([t.Dlambda_star.real, t.Dmu_star_off.real, t.Drho_star.real])
+    assert abs(s[0]) < 1e-8 * REF.lam and abs(s[1]) < 1e-8 * REF.mu and abs(s[2]) < 1e-8 * REF.rho
+
+
+def test_single_phase_and_voigt_limit():
+    one = cp.compute_cpa([cp.Phase(REF.lam, REF.mu, REF.rho, 1.0)], OMEGA_LOW, A)
+    assert one.converged and one.n_iterations <= 2
+    np.testing.assert_allclose([one.effective_medium.lam, one.effective_medium.mu_off,
+                                one.effective_medium.rho], [REF.lam, REF.mu, REF.rho], rtol=1e-12)
+    phases = cp.phases_from_two_phase(REF, WEAK, 0.5)
+    v = cp.voigt_average(phases)
+    r = cp.compute_cpa(phases, OMEGA_LOW, A)
+    assert r.converged and r.n_iterations <= 5
+    np.testing.assert_allclose([r.effective_medium.lam, r.effective_medium.mu_off,
+                                r.effective_medium.rho], [v.lam, v.mu_off, v.rho], rtol=1e-3)
+    p1, p2 = cp.Phase(1e10, 5e9, 2000.0, 0.5), cp.Phase(2e10, 1e10, 3000.0, 0.5)
+    avg = cp.voigt_average([p1, p2])
+    assert (avg.lam, avg.mu_off, avg.rho) == (1.5e10, 7.5e9, 2500.0)
+    with pytest.raises(ValueError, match="Volume fractions must sum to 1.0"):
+        cp.compute_cpa([cp.Phase(1e10, 5e9, 2000.0, 0.3), cp.Phase(2e10, 1e10, 3000.0, 0.3)],
+                       OMEGA_LOW, A)
+
+
+def test_effective_medium_between_phases_and_residual_decreasing():
+    r = cp.compute_cpa_two_phase(REF, CONTRAST, 0.3, OMEGA_LOW, A)
+    e = r.effective_medium
+    assert REF.lam < e.lam < REF.lam + CONTRAST.Dlambda
+    assert REF.mu < e.mu_off < REF.mu + CONTRAST.Dmu
+    assert REF.rho < e.rho < REF.rho + CONTRAST.Drho
+    assert e.mu_diag > 0 and e.bulk_modulus > 0 and e.cubic_anisotropy != 0.0
+    h = r.residual_history
+    assert h[-1] < h[0] and len(r.tmatrix_results) == 2 and r.omega == OMEGA_LOW
+
+
+def test_kennett_bridge():
+    eff = cp.compute_cpa_two_phase(REF, CONTRAST, 0.3, OMEGA_LOW, A).effective_medium
+    lay = kl.cubic_to_isotropic_layer(eff, 2.0)
+    assert lay.alpha == pytest.approx(np.sqrt((eff.lam + 2 * eff.mu_off) / eff.rho), rel=1e-15)
+    assert lay.beta == pytest.approx(np.sqrt(eff.mu_off / eff.rho), rel=1e-15)
+    st = kl.cpa_stack_from_phases([[cp.Phase(REF.lam, REF.mu, REF.rho, 1.0)]] * 3, OMEGA_LOW, A)
+    assert st.n_layers == 3 and np.isinf(st.layers[-1].thickness)
+    assert st.layers[0].thickness == 2 * A
+    for l in st.layers:
+        assert l.alpha == pytest.approx(REF.alpha, rel=1e-10)
+    res = kl.kennett_layers(st, 1e-4, 2 * np.pi * np.array([5.0, 20.0]))
+    np.testing.assert_allclose(res.RD_psv, 0.0, atol=1e-9)
+    s1 = kl.random_heterogeneous_stack(REF.alpha, REF.beta, REF.rho, 4, A, OMEGA_LOW,
+                                       CONTRAST.Dlambda, CONTRAST.Dmu, CONTRAST.Drho, seed=3)
+    s2 = kl.random_heterogeneous_stack(REF.alpha, REF.beta, REF.rho, 4, A, OMEGA_LOW,
+                                       CONTRAST.Dlambda, CONTRAST.Dmu, CONTRAST.Drho, seed=3)
+    assert [l.alpha for l in s1.layers] == [l.alpha for l in s2.layers]
+    v = kl.random_velocity_stack(3000.0, 1700.0, 2200.0, 5, 10.0, 100.0, 50.0, 20.0, seed=1)
+    rng = np.random.default_rng(1)
+    assert v.layers[0].alpha == max(3000.0 + rng.normal(0, 100.0), 1500.0)
+    assert all(l.alpha > 0 and l.beta > 0 for l in v.layers)
