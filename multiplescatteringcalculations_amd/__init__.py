@@ -50,6 +50,12 @@ from .resonance_tmatrix import (
     suggest_n_subcells,
 )
 from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
+from .sphere_scattering_fft import (
+    SphereDecompositionResult,
+    compute_sphere_foldy_lax,
+    compute_sphere_foldy_lax_fft,
+    compute_sphere_foldy_lax_fft_gpu,
+)
 from .solver_config import load_config, run_from_config
 from .sweep import SweepCase, compute_slab_sweep
 from .torch_gmres import get_device, select_dtype, to_numpy, to_torch, torch_gmres
@@ -67,5 +73,6 @@ __all__ = [
     "load_ocean_bottom_config", "ricker_source_spectrum", "write_log",
     "ResonanceTmatrixResult", "compute_resonance_tmatrices", "compute_resonance_tmatrix",
     "suggest_n_subcells", "CPAResult", "CubicEffectiveMedium", "Phase", "compute_cpa",
-    "compute_cpa_two_phase",
+    "compute_cpa_two_phase", "SphereDecompositionResult", "compute_sphere_foldy_lax",
+    "compute_sphere_foldy_lax_fft", "compute_sphere_foldy_lax_fft_gpu",
 ]
