@@ -51,9 +51,24 @@ __device__ __forceinline__ void swap_xy(cx<R>* t) {
 // with H = W^-1 C^T, S = S' W and W = diag(1,1,1,1/2,1/2,1/2).  The entries are
 // read one at a time and applied at once to every product.  t_p are already
 // sign-flipped / swapped into the table's octant.
+// Table loads kept in flight by the block multiply: every KG entries the
+// accumulators are pinned, so later entries' loads cannot be hoisted above
+// (the compiler otherwise issues all 45 at once: ~180 VGPRs in complex128).
+#ifndef MSC_KMUL_GROUP
+#define MSC_KMUL_GROUP 9
+#endif
+template <typename R, int NP>
+__device__ __forceinline__ void kmul_pin(cx<R> (&o)[NP][9]) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) asm volatile("" : "+v"(o[p][i].x), "+v"(o[p][i].y) : : "memory");
+}
+
 template <typename R, int NP>
 __device__ __forceinline__ void kmul45_stream(const cx<R>* __restrict__ kp, int stride,
                                               const cx<R> (&t)[NP][9], cx<R> (&o)[NP][9]) {
+  constexpr int KG = MSC_KMUL_GROUP;
 #pragma unroll
   for (int p = 0; p < NP; ++p)
 #pragma unroll
@@ -69,6 +84,7 @@ __device__ __forceinline__ void kmul45_stream(const cx<R>* __restrict__ kp, int 
       cfma<R>(o[p][i], kv, t[p][j]);
       if (i != j) cfma<R>(o[p][j], kv, t[p][i]);
     }
+    if (KG > 0 && e % KG == KG - 1) kmul_pin<R, NP>(o);
   }
   // C (3x6) and H = W^-1 C^T: o_u += C t_e, o_e += (1 or 2) C^T t_u
 #pragma unroll
@@ -80,6 +96,7 @@ __device__ __forceinline__ void kmul45_stream(const cx<R>* __restrict__ kp, int 
       cfma<R>(o[p][i], kv, t[p][3 + al]);
       cfma<R>(o[p][3 + al], al < 3 ? kv : cscale<R>(kv, R(2)), t[p][i]);
     }
+    if (KG > 0 && (6 + e) % KG == KG - 1) kmul_pin<R, NP>(o);
   }
   // S' (symmetric 6x6 upper triangle), S = S' W: o_e += S' (W t_e)
 #pragma unroll
@@ -102,6 +119,7 @@ __device__ __forceinline__ void kmul45_stream(const cx<R>* __restrict__ kp, int 
         cfma<R>(o[p][3 + be], kv, twa);
       }
     }
+    if (KG > 0 && (24 + e) % KG == KG - 1) kmul_pin<R, NP>(o);
   }
 }
 
