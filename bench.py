@@ -184,13 +184,68 @@ def measured_traffic(kernel: str):
     known size); None when no summary for this kernel and workload exists."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "k_z_mult_c64_128x128x32_traffic.txt")
-    if kernel != "k_z_mult<float,64,8>" or not os.path.exists(path):
+    if not os.path.exists(path):
         return None
+    found, val = False, None
     with open(path) as f:
         for line in f:
+            if line.startswith("kernel ="):
+                found = line.split("=", 1)[1].strip() == kernel
             if line.startswith("traffic_bytes_per_launch"):
-                return int(line.split("=")[1])
-    return None
+                val = int(line.split("=")[1])
+    return val if found else None
+
+
+def time_resonance(n_sub: int = 4, count: int = 1024):
+    """Resonance-regime cube T-matrices (internal Foldy-Lax over n_sub^3 sub-cells)
+    of `count` random contrasts (+-0.3) at ka = 1.5 in one device call."""
+    import torch
+
+    import multiplescatteringcalculations_amd as msc
+
+    ref = msc.ReferenceMedium(VP, VS, RHO)
+    rng = np.random.default_rng(5)
+    lam, mu = ref.rho * (VP ** 2 - 2 * VS ** 2), ref.rho * VS ** 2
+    Dl, Dm, Dr = (rng.uniform(-0.3, 0.3, count) * s for s in (lam, mu, ref.rho))
+    om = 1.5 * VS / A
+    msc.compute_resonance_tmatrices(om, A, ref, Dl[:4], Dm[:4], Dr[:4], n_sub=n_sub)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    T, ratio = msc.compute_resonance_tmatrices(om, A, ref, Dl, Dm, Dr, n_sub=n_sub,
+                                               keep_on_device=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n = 9 * n_sub ** 3
+    flops = 8.0 / 3.0 * n ** 3
+    out = {"n_sub": n_sub, "cubes": count, "seconds": round(dt, 4),
+           "us_per_cube": round(dt / count * 1e6, 2),
+           "fp64_tflops": round(flops * count / dt / 1e12, 2), "fp64_peak_tflops": 78.6,
+           "max_pivot_ratio": float(np.max(ratio))}
+    log(f"resonance: {out}")
+    return out
+
+
+def time_sphere(n_sub: int = 32):
+    """Voxelised-sphere Foldy-Lax T-matrix (9 right-hand sides, FP64 GMRES to 1e-8)
+    with n_sub cells per edge, ka_sphere = 2."""
+    import torch
+
+    import multiplescatteringcalculations_amd as msc
+
+    ref = msc.ReferenceMedium(VP, VS, RHO)
+    R = n_sub * A
+    om = 2.0 * VS / R
+    c = msc.MaterialContrast(0.2 * ref.rho * (VP ** 2 - 2 * VS ** 2), 0.2 * ref.rho * VS ** 2,
+                             0.2 * ref.rho)
+    msc.compute_sphere_foldy_lax_fft_gpu(om, R, ref, c, 4, gmres_tol=TOL)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = msc.compute_sphere_foldy_lax_fft_gpu(om, R, ref, c, n_sub, gmres_tol=TOL, restart=30)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"n_sub": n_sub, "cells": int(res.n_cells), "seconds_9_rhs": round(dt, 3)}
+    log(f"sphere: {out}")
+    return out
 
 
 def cpu_baseline(mat, omega, n_matvec: int):
@@ -227,6 +282,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the resonance-batch and sphere legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: independent (frequency, slowness, slab) solves shard "
@@ -280,6 +337,8 @@ def main():
     mean_64 = float(np.mean(t64))
     mean_ln = float(np.mean(tln))
     sweep = None if args.no_sweep else time_sweep()
+    extras = None if args.no_extras else {"resonance_batch": time_resonance(),
+                                          "sphere_fft": time_sphere()}
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle restatement of the reference CPU matvec, 1 matvec) ...")
@@ -327,6 +386,7 @@ def main():
             "mixed_vs_fp64_max_rel_diff": diff, "setup_s": round(t_setup, 3),
             "pass3_c128_avg_us": round(tim64["c128"][1] / max(tim64["c128"][0], 1) * 1e3, 2),
             "sweep_256_uneven_slabs_mixed": sweep,
+            "other_paths": extras,
         },
     }
     print(json.dumps(out), flush=True)

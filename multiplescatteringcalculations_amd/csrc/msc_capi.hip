@@ -197,7 +197,7 @@ int alloc_plan_buffers(Plan& p) {
   MSC_HIP_CHECK(hipMalloc(&p.tp128, capvox * 4 * sizeof(double2)));
   MSC_HIP_CHECK(hipMalloc(&p.tp64, capvox * 4 * sizeof(float2)));
   MSC_HIP_CHECK(hipMalloc(&p.tmax, sizeof(unsigned int)));
-  return kOk;
+  return build_zmap(p);
 }
 
 void free_plan_buffers(Plan& p) {
@@ -212,6 +212,8 @@ void free_plan_buffers(Plan& p) {
   if (p.nn128) (void)hipFree(p.nn128);
   if (p.nn64) (void)hipFree(p.nn64);
   if (p.nn_stage) (void)hipFree(p.nn_stage);
+  if (p.zmap) (void)hipFree(p.zmap);
+  p.zmap = nullptr;
   p.tmax = nullptr;
   p.work = nullptr;
   p.nn128 = nullptr;
@@ -223,6 +225,56 @@ void free_plan_buffers(Plan& p) {
   p.tp128 = nullptr;
   p.tp64 = nullptr;
 }
+
+}  // namespace
+
+// Tile order of the z pass: workgroups go to the XCDs in round-robin order
+// (slot b on XCD b % 8), and each XCD has its own L2.  The rows kx and
+// Pxy - kx fold onto the same table entries, so they form one group; group g
+// (0 <= g <= Pxy/2) is placed on XCD g % 8, its tiles in chunk order with the
+// two rows adjacent, so every table entry is fetched by at most the two XCDs
+// of its folded (kx, ky) pair.  For Pxy >= 16 the XCDs get equal shares.
+int msc::build_zmap(Plan& p) {
+  constexpr int kXcd = 8;
+  const int P = p.Pxy;
+  std::vector<int> all;
+  for (int Q = 1; Q <= 16; Q <<= 1) {
+    const int nck = P / Q;
+    std::vector<std::vector<int>> per(kXcd);
+    if (nck >= 1 && P % Q == 0) {
+      for (int g = 0; g <= P / 2; ++g) {
+        std::vector<int>& L = per[g % kXcd];
+        for (int c = 0; c < nck; ++c) {
+          L.push_back(g * nck + c);
+          if (g != 0 && 2 * g != P) L.push_back((P - g) * nck + c);
+        }
+      }
+    }
+    const size_t before = all.size();
+    size_t longest = 0;
+    for (auto& L : per) longest = std::max(longest, L.size());
+    for (size_t s = 0; s < longest; ++s)
+      for (int x = 0; x < kXcd; ++x)
+        if (s < per[x].size()) all.push_back(per[x][s]);
+    // every tile exactly once
+    const size_t ntile = (size_t)P * std::max(nck, 0);
+    std::vector<char> seen(ntile, 0);
+    bool ok = all.size() - before == ntile;
+    for (size_t i = before; ok && i < all.size(); ++i) {
+      ok = all[i] >= 0 && (size_t)all[i] < ntile && !seen[all[i]];
+      if (ok) seen[all[i]] = 1;
+    }
+    if (!ok) {
+      set_error("internal: z-pass tile order is not a permutation");
+      return kHip;
+    }
+  }
+  MSC_HIP_CHECK(hipMalloc(&p.zmap, all.size() * sizeof(int)));
+  MSC_HIP_CHECK(hipMemcpy(p.zmap, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+namespace {
 
 int check_opts(const msc_gmres_opts* o) {
   MSC_REQUIRE(o != nullptr, "opts must not be NULL");

@@ -210,6 +210,7 @@ template <typename R, int PZ, int Q>
 __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
                                                const cx<R>* __restrict__ table, int Nz,
                                                int Pxy, const cx<R>* __restrict__ tw,
+                                               const int* __restrict__ zmap,
                                                const int* __restrict__ done) {
   constexpr int NTR = 9 * Q;
   constexpr int LD = lds_ld<NTR>::value;
@@ -217,8 +218,9 @@ __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
   __shared__ cx<R> s[PZ * LD];
   if (done != nullptr && *done) return;
   const int nck = Pxy / Q;
-  const int kx = blockIdx.x / nck;
-  const int ky0 = (blockIdx.x - kx * nck) * Q;
+  const int tile = zmap[blockIdx.x];
+  const int kx = tile / nck;
+  const int ky0 = (tile - kx * nck) * Q;
   const size_t plane = (size_t)Pxy * Pxy * 9;
   const size_t col0 = ((size_t)kx * Pxy + ky0) * 9;
   for (int w = threadIdx.x; w < PZ * NTR; w += kNT) {
@@ -569,7 +571,8 @@ int timing_end(Plan& p, hipStream_t st, size_t slot) {
 template <typename R, int PZ, int Q>
 static void z_q(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
   hipLaunchKernelGGL((k_z_mult<R, PZ, Q>), dim3(p.Pxy * (p.Pxy / Q)), dim3(kNT), 0, st, b2,
-                     PlanBufs<R>::table(p), p.Nz, p.Pxy, PlanBufs<R>::tw(p), done);
+                     PlanBufs<R>::table(p), p.Nz, p.Pxy, PlanBufs<R>::tw(p),
+                     p.zmap + zmap_offset(p.Pxy, Q), done);
 }
 
 template <typename R, int PZ>

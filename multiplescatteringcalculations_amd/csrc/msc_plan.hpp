@@ -56,6 +56,10 @@ struct Plan {
   double2* nn128 = nullptr;  // 26 x 81
   float2* nn64 = nullptr;
   double* nn_stage = nullptr;  // 26 x 81 real volume-averaged blocks (device copy)
+  // z-pass tile order per column tile width Q = 1, 2, 4, 8, 16 (concatenated,
+  // zmap_offset): dispatch slot -> tile kx * (Pxy / Q) + chunk, grouping the
+  // rows kx and Pxy - kx (which read the same table entries) on one XCD.
+  int* zmap = nullptr;
   double2* tw128 = nullptr;
   float2* tw64 = nullptr;
 
@@ -100,5 +104,12 @@ int build_table(Plan& p, double omega, double alpha, double beta, double rho, hi
 // volume_averaged: set the nearest-neighbour correction for the plan's table
 // (after build_table, same frequency and medium); otherwise clear it.
 int set_nn_correction(Plan& p, bool volume_averaged, int n_orders, hipStream_t st);
+// Offset of the tile order for width Q (a power of two <= 16) in Plan::zmap.
+inline size_t zmap_offset(int Pxy, int Q) {
+  size_t off = 0;
+  for (int q = 1; q < Q; q <<= 1) off += (size_t)Pxy * (Pxy / q);
+  return off;
+}
+int build_zmap(Plan& p);
 
 }  // namespace msc
