@@ -24,6 +24,29 @@ namespace msc {
 
 constexpr int kNT = 256;  // threads per workgroup for every pass
 
+// Element-wise staging loop over n items, thread-strided: the global loads of
+// SB consecutive items of a thread are issued before any of them is consumed,
+// so a thread waits for memory once per SB items (a plain load -> store loop
+// waits once per item: the compiler does not pipeline a loop whose trip count
+// depends on the thread index).
+constexpr int kSB = 8;
+template <typename T, typename Load, typename Use>
+__device__ __forceinline__ void staged(int n, Load load, Use use) {
+  for (int w0 = 0; w0 < n; w0 += kSB * kNT) {
+    T v[kSB];
+#pragma unroll
+    for (int k = 0; k < kSB; ++k) {
+      const int w = w0 + k * kNT + (int)threadIdx.x;
+      if (w < n) v[k] = load(w);
+    }
+#pragma unroll
+    for (int k = 0; k < kSB; ++k) {
+      const int w = w0 + k * kNT + (int)threadIdx.x;
+      if (w < n) use(w, v[k]);
+    }
+  }
+}
+
 // Component sign flips of the lattice reflections (z, x, y) acting on the
 // 9-vector (u_z, u_x, u_y, e_zz, e_xx, e_yy, e_xy, e_zy, e_zx):
 // a component flips when it carries an odd number of the reflected index.
@@ -135,24 +158,42 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
   const int row0 = blockIdx.x * B;
-  for (int w = threadIdx.x; w < B * P * 9; w += kNT) {
-    const int b = w / (P * 9);
-    const int rem = w - b * P * 9;
-    const int y = rem / 9;
-    const int c = rem - y * 9;
-    const int row = row0 + b;
-    const bool ok = y < M && row < nrows;
-    const cx<R> v = x[ok ? (size_t)row * M * 9 + rem : 0];
-    s[y * LD + b * 9 + c] = ok ? v : czero<R>();
-  }
+  staged<cx<R>>(
+      B * P * 9,
+      [&](int w) {
+        const int b = w / (P * 9);
+        const int rem = w - b * P * 9;
+        const int y = rem / 9;
+        const int row = row0 + b;
+        const bool ok = y < M && row < nrows;
+        const cx<R> v = x[ok ? (size_t)row * M * 9 + rem : 0];
+        return ok ? v : czero<R>();
+      },
+      [&](int w, cx<R> v) {
+        const int b = w / (P * 9);
+        const int rem = w - b * P * 9;
+        const int y = rem / 9;
+        const int c = rem - y * 9;
+        s[y * LD + b * 9 + c] = v;
+      });
   __syncthreads();
-  for (int w = threadIdx.x; w < B * M; w += kNT) {
+  struct T4 {
+    cx<R> u, d, o, s;
+  };
+  staged<T4>(
+      B * M,
+      [&](int w) {
+        const int b = w / M;
+        const int y = w - b * M;
+        const int row = min(row0 + b, nrows - 1);
+        const cx<R>* t = tp + ((size_t)row * M + y) * 4;
+        return T4{t[0], t[1], t[2], t[3]};
+      },
+      [&](int w, T4 tt) {
     const int b = w / M;
     const int y = w - b * M;
-    const int row = row0 + b;
-    if (row >= nrows) continue;
-    const cx<R>* t = tp + ((size_t)row * M + y) * 4;
-    const cx<R> tu = t[0], td = t[1], to = t[2], ts = t[3];
+    if (row0 + b >= nrows) return;
+    const cx<R> tu = tt.u, td = tt.d, to = tt.o, ts = tt.s;
     cx<R>* v = s + y * LD + b * 9;
     const cx<R> sum = cadd<R>(cadd<R>(v[3], v[4]), v[5]);
     const cx<R> dmo = csub<R>(td, to);
@@ -163,7 +204,7 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
     for (int c = 3; c < 6; ++c) v[c] = cadd<R>(cmul<R>(dmo, v[c]), tos);
 #pragma unroll
     for (int c = 6; c < 9; ++c) v[c] = cmul<R>(ts, v[c]);
-  }
+      });
   __syncthreads();
   lds_fft<R, P, false, NTR, LD, kNT>(s, tw);
   for (int w = threadIdx.x; w < B * P * 9; w += kNT) {
@@ -189,13 +230,19 @@ __global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
   constexpr int nck = P / Q;
   const int z = blockIdx.x / nck;
   const int ky0 = (blockIdx.x - z * nck) * Q;
-  for (int w = threadIdx.x; w < P * NTR; w += kNT) {
-    const int xr = w / NTR;
-    const int e = w - xr * NTR;
-    const bool ok = xr < M;
-    const cx<R> v = buf1[ok ? (((size_t)z * M + xr) * P + ky0) * 9 + e : 0];
-    s[xr * LD + e] = ok ? v : czero<R>();
-  }
+  staged<cx<R>>(
+      P * NTR,
+      [&](int w) {
+        const int xr = w / NTR;
+        const int e = w - xr * NTR;
+        const bool ok = xr < M;
+        const cx<R> v = buf1[ok ? (((size_t)z * M + xr) * P + ky0) * 9 + e : 0];
+        return ok ? v : czero<R>();
+      },
+      [&](int w, cx<R> v) {
+        const int xr = w / NTR;
+        s[xr * LD + (w - xr * NTR)] = v;
+      });
   __syncthreads();
   lds_fft<R, P, false, NTR, LD, kNT>(s, tw);
   for (int w = threadIdx.x; w < P * NTR; w += kNT) {
@@ -281,11 +328,16 @@ __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
   constexpr int nck = P / Q;
   const int z = blockIdx.x / nck;
   const int ky0 = (blockIdx.x - z * nck) * Q;
-  for (int w = threadIdx.x; w < P * NTR; w += kNT) {
-    const int kx = w / NTR;
-    const int e = w - kx * NTR;
-    s[kx * LD + e] = buf2[(((size_t)z * P + kx) * P + ky0) * 9 + e];
-  }
+  staged<cx<R>>(
+      P * NTR,
+      [&](int w) {
+        const int kx = w / NTR;
+        return buf2[(((size_t)z * P + kx) * P + ky0) * 9 + (w - kx * NTR)];
+      },
+      [&](int w, cx<R> v) {
+        const int kx = w / NTR;
+        s[kx * LD + (w - kx * NTR)] = v;
+      });
   __syncthreads();
   lds_fft<R, P, true, NTR, LD, kNT>(s, tw);
   for (int w = threadIdx.x; w < M * NTR; w += kNT) {
@@ -311,35 +363,48 @@ __global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
   const int row0 = blockIdx.x * B;
-  for (int w = threadIdx.x; w < B * P * 9; w += kNT) {
-    const int b = w / (P * 9);
-    const int rem = w - b * P * 9;
-    const int y = rem / 9;
-    const int c = rem - y * 9;
-    const int row = row0 + b;
-    const bool ok = row < nrows;
-    const cx<R> v = buf1[ok ? (size_t)row * P * 9 + rem : 0];
-    s[y * LD + b * 9 + c] = ok ? v : czero<R>();
-  }
+  staged<cx<R>>(
+      B * P * 9,
+      [&](int w) {
+        const int b = w / (P * 9);
+        const int rem = w - b * P * 9;
+        const int row = row0 + b;
+        const bool ok = row < nrows;
+        const cx<R> v = buf1[ok ? (size_t)row * P * 9 + rem : 0];
+        return ok ? v : czero<R>();
+      },
+      [&](int w, cx<R> v) {
+        const int b = w / (P * 9);
+        const int rem = w - b * P * 9;
+        const int y = rem / 9;
+        s[y * LD + b * 9 + (rem - y * 9)] = v;
+      });
   __syncthreads();
   lds_fft<R, P, true, NTR, LD, kNT>(s, tw);
-  for (int w = threadIdx.x; w < B * M * 9; w += kNT) {
-    const int b = w / (M * 9);
-    const int rem = w - b * M * 9;
-    const int y = rem / 9;
-    const int c = rem - y * 9;
-    const int row = row0 + b;
-    if (row >= nrows) continue;
-    const size_t g = (size_t)row * M * 9 + rem;
-    const cx<R> acc = s[y * LD + b * 9 + c];
-    if constexpr (MODE == 1) {
-      out[g] = cadd<R>(csub<R>(bvec[g], x[g]), acc);
-    } else if constexpr (MODE == 2) {
-      out[g] = acc;
-    } else {
-      out[g] = csub<R>(x[g], acc);
-    }
-  }
+  // the operator's own x (and b) terms: loaded in batches as above
+  staged<cx<R>>(
+      B * M * 9,
+      [&](int w) {
+        const int b = w / (M * 9);
+        const int rem = w - b * M * 9;
+        const int row = min(row0 + b, nrows - 1);
+        const size_t g = (size_t)row * M * 9 + rem;
+        if constexpr (MODE == 1) return csub<R>(bvec[g], x[g]);
+        else if constexpr (MODE == 2) return czero<R>();
+        else return x[g];
+      },
+      [&](int w, cx<R> xv) {
+        const int b = w / (M * 9);
+        const int rem = w - b * M * 9;
+        const int y = rem / 9;
+        const int row = row0 + b;
+        if (row >= nrows) return;
+        const size_t g = (size_t)row * M * 9 + rem;
+        const cx<R> acc = s[y * LD + b * 9 + (rem - y * 9)];
+        if constexpr (MODE == 2) out[g] = acc;
+        else if constexpr (MODE == 1) out[g] = cadd<R>(xv, acc);
+        else out[g] = csub<R>(xv, acc);
+      });
 }
 
 // ── volume-averaged nearest-neighbour correction (after pass 5) ──────────────

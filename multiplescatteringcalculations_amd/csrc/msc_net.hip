@@ -105,21 +105,37 @@ __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
   const int bh = b % nbh;
   const int d = b / nbh;
   const int h0 = bh * kTH, w0 = bw * kTW;
-  // stage the halo: 4-channel groups (8 bytes), zero outside the grid
+  // stage the halo: 4-channel groups (8 bytes), zero outside the grid.  The
+  // loads go out in batches of SB before their LDS stores, so a thread waits
+  // for memory once per batch rather than once per element.
   constexpr int G4 = NS / 4;
   constexpr int NSTAGE = 3 * ROWS * COLS * G4;
-  for (int e = threadIdx.x; e < NSTAGE; e += kNetNT) {
-    const int g = e % G4;
-    int q = e / G4;
-    const int c = q % COLS;
-    q /= COLS;
-    const int rr = q % ROWS;
-    const int pl = q / ROWS;
-    const int gd = d + pl - 1, gh = h0 + rr - 1, gw = w0 + c - 1;
-    v4s val = v4s{0, 0, 0, 0};
-    if (gd >= 0 && gd < D && gh >= 0 && gh < H && gw >= 0 && gw < W)
-      val = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI + g * 4);
-    *reinterpret_cast<v4s*>(&sh[((pl * ROWS + rr) * COLS + c) * LDV + g * 4]) = val;
+  constexpr int SB = 8;
+  for (int e0 = 0; e0 < NSTAGE; e0 += SB * kNetNT) {
+    v4s val[SB];
+    int off[SB];
+#pragma unroll
+    for (int k = 0; k < SB; ++k) {
+      const int e = e0 + k * kNetNT + threadIdx.x;
+      val[k] = v4s{0, 0, 0, 0};
+      off[k] = -1;
+      if (e < NSTAGE) {
+        const int g = e % G4;
+        int q = e / G4;
+        const int c = q % COLS;
+        q /= COLS;
+        const int rr = q % ROWS;
+        const int pl = q / ROWS;
+        const int gd = d + pl - 1, gh = h0 + rr - 1, gw = w0 + c - 1;
+        off[k] = ((pl * ROWS + rr) * COLS + c) * LDV + g * 4;
+        if (gd >= 0 && gd < D && gh >= 0 && gh < H && gw >= 0 && gw < W)
+          val[k] = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI +
+                                                 g * 4);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SB; ++k)
+      if (off[k] >= 0) *reinterpret_cast<v4s*>(&sh[off[k]]) = val[k];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
