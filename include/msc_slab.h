@@ -159,6 +159,17 @@ int msc_plan_destroy(msc_plan* plan);
 int msc_plan_reshape(msc_plan* plan, int M, int N_z, double a);
 
 /*
+ * Copy the G0 table (with its volume-averaged near-neighbour correction) and
+ * the autotuned tiles of `src` into `dst`, a plan of the same slab size,
+ * lattice, periodicity and FFT grid on the same device, in order on `stream`:
+ * the same state as msc_build_g0 on `dst` at src's frequency and medium, at
+ * the cost of a device copy.  `dst`'s T0 is unchanged.  No reference
+ * counterpart: it gives the concurrent solves of one sphere (one plan per
+ * host thread) a single table build.
+ */
+int msc_plan_copy_g0(msc_plan* dst, const msc_plan* src, void* stream);
+
+/*
  * Run each Arnoldi step of msc_gmres as one instantiated HIP graph (enable = 1,
  * the default) or as individual launches (0).  Graph capture must not overlap
  * another host thread's allocation or synchronous copy calls, so callers that

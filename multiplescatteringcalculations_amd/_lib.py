@@ -32,6 +32,7 @@ SIGNATURES = {
     "msc_plan_destroy": (c_int, [c_void_p]),
     "msc_plan_reshape": (c_int, [c_void_p, c_int, c_int, c_double]),
     "msc_plan_use_graphs": (c_int, [c_void_p, c_int]),
+    "msc_plan_copy_g0": (c_int, [c_void_p, c_void_p, c_void_p]),
     "msc_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int]),
     "msc_resonance_tmatrices": (c_int, [c_double, c_double, c_double, c_double, c_double, c_int,
                                         c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,

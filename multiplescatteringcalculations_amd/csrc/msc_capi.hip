@@ -692,6 +692,48 @@ int msc_plan_use_graphs(msc_plan* plan, int enable) {
   return kOk;
 }
 
+int msc_plan_copy_g0(msc_plan* dst, const msc_plan* src, void* stream) {
+  MSC_REQUIRE(dst != nullptr && src != nullptr, "NULL argument");
+  MSC_REQUIRE(dst != src, "dst and src must be different plans");
+  const Plan& s = src->p;
+  Plan& d = dst->p;
+  if (!s.have_table) {
+    set_error("the source plan has no G0 table");
+    return kNotReady;
+  }
+  MSC_REQUIRE(s.M == d.M && s.Nz == d.Nz && s.a == d.a && s.periodic == d.periodic &&
+                  s.Pz == d.Pz && s.Pxy == d.Pxy && s.NUV == d.NUV && s.NC == d.NC,
+              "plans differ in slab size, lattice, periodicity or FFT grid");
+  MSC_REQUIRE(d.device == s.device, "plans are on different devices");
+  hipStream_t st = S(stream);
+  const size_t tab = (size_t)s.NUV * kNE * s.NC;
+  MSC_HIP_CHECK(hipMemcpyAsync(d.table128, s.table128, tab * sizeof(double2),
+                               hipMemcpyDeviceToDevice, st));
+  MSC_HIP_CHECK(hipMemcpyAsync(d.table64, s.table64, tab * sizeof(float2),
+                               hipMemcpyDeviceToDevice, st));
+  if (s.have_nn) {
+    if (d.nn128 == nullptr) MSC_HIP_CHECK(hipMalloc(&d.nn128, 26 * 81 * sizeof(double2)));
+    if (d.nn64 == nullptr) MSC_HIP_CHECK(hipMalloc(&d.nn64, 26 * 81 * sizeof(float2)));
+    MSC_HIP_CHECK(hipMemcpyAsync(d.nn128, s.nn128, 26 * 81 * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, st));
+    MSC_HIP_CHECK(hipMemcpyAsync(d.nn64, s.nn64, 26 * 81 * sizeof(float2),
+                                 hipMemcpyDeviceToDevice, st));
+  }
+  d.have_nn = s.have_nn;
+  d.omega = s.omega;
+  d.alpha = s.alpha;
+  d.beta = s.beta;
+  d.rho = s.rho;
+  d.have_table = true;
+  for (int pr = 0; pr < 2; ++pr) {
+    for (int k = 0; k < 5; ++k) d.tile[pr][k] = s.tile[pr][k];
+    d.tuned[pr] = s.tuned[pr];
+  }
+  dst->k128.drop_graphs();
+  dst->k64.drop_graphs();
+  return kOk;
+}
+
 int msc_plan_destroy(msc_plan* plan) {
   if (plan == nullptr) return kOk;
   (void)hipDeviceSynchronize();

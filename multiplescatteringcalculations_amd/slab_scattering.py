@@ -320,6 +320,15 @@ class SlabPlan:
                                                 _lib.ptr(ls), _lib.ptr(ph), self.stream))
         return ls.cpu().numpy(), ph.cpu().numpy()
 
+    def copy_g0_from(self, other: "SlabPlan") -> None:
+        """Take G0 (and its near-neighbour correction) and the tuned tiles from
+        ``other``, a plan of the same geometry, grid and device, by a device
+        copy instead of a table build."""
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.msc_plan_copy_g0(self._h, other._h, self.stream))
+        self.omega = other.omega
+        self.ref = other.ref
+
     def autotune(self, precision: str = "c64") -> None:
         """Time every tile candidate of the FFT passes in place and keep the fastest."""
         prec = {"c128": _lib.MSC_C128, "c64": _lib.MSC_C64}[precision]

@@ -13,9 +13,10 @@ Accepts the reference's ``device / solver / problem`` schema
   ``seed`` (Bernoulli inclusions with i.i.d. uniform relative contrasts);
 * ``problem.slab.periodic`` and ``problem.slab.pad: [Pz, Pxy]``.
 
-The slab is solved on the HIP device only; ``backend: cpu`` is accepted by the
-validator (the reference schema) but ``run_from_config`` raises for it, and
-``type: sphere`` is validated but not solved by this build.
+Problems are solved on the HIP device only; ``backend: cpu`` is accepted by
+the validator (the reference schema) but ``run_from_config`` raises for it.
+``type: sphere`` runs the FFT Foldy-Lax sphere solve
+(``compute_sphere_foldy_lax_fft_gpu``) as the reference's GPU branch does.
 """
 
 from __future__ import annotations
@@ -235,16 +236,27 @@ def build_material(config: ScatteringConfig):
     return geometry, uniform_slab_material(geometry, ref, prob.to_material_contrast())
 
 
-def run_from_config(config: ScatteringConfig) -> SlabResult:
-    """Dispatch a slab config to the HIP solver (solver_config.py:375-438)."""
+def run_from_config(config: ScatteringConfig):
+    """Dispatch a config to the HIP solver (solver_config.py:375-475): a
+    ``SlabResult`` for ``type: slab``, a ``SphereDecompositionResult`` for
+    ``type: sphere``."""
     device = config.device.resolve_device()
-    if config.problem.type != "slab":
-        raise ValueError("this build solves problem.type 'slab' only (sphere: not supported)")
     if device.type != "cuda":
         raise ValueError(f"backend '{config.device.backend}' resolved to '{device}': this build runs "
-                         f"the slab solve on HIP devices only (no CPU solver)")
+                         f"the solve on HIP devices only (no CPU solver)")
     dtype = config.device.resolve_dtype(device)
     prob = config.problem
+    if prob.type == "sphere":
+        from .sphere_scattering_fft import compute_sphere_foldy_lax_fft_gpu
+
+        sp = prob.sphere or {}
+        return compute_sphere_foldy_lax_fft_gpu(
+            omega=float(prob.frequency["omega"]), radius=float(sp["radius"]),
+            ref=prob.to_reference_medium(), contrast=prob.to_material_contrast(),
+            n_sub=int(sp["n_sub"]), k_hat=np.array(prob.incident["k_hat"], dtype=float),
+            wave_type=prob.incident["wave_type"], gmres_tol=config.solver.gmres_tol,
+            gmres_maxiter=config.solver.max_iter, initial_guess=config.solver.initial_guess,
+            device=device, dtype=dtype, restart=config.solver.restart)
     geometry, material = build_material(config)
     slab_cfg = prob.slab or {}
     pad = slab_cfg.get("pad")

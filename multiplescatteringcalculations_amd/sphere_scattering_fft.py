@@ -12,12 +12,15 @@ plan of this package (spectral G0 table of the sub-cube lattice, five-pass
 FFT matvec, device GMRES), once per incident pattern (9 right-hand sides,
 resonance_tmatrix.py:526-593, built at the inside centres), and the composite
 T-matrix column T_loc sum_m psi_m is the device sum of tau = T0 psi over the
-slab.
+slab.  The 9 solves are independent: ``workers`` host threads each run a plan
+on a stream of their own, so their launches overlap on the device.
 """
 
 from __future__ import annotations
 
+import os
 import warnings
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
@@ -99,11 +102,13 @@ def _incident_patterns(centres, omega, ref, k_hat, wave_type):
 
 
 def _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type, inclusive, tol, max_iter,
-           restart, precision, x0, device):
+           restart, precision, x0, device, workers=1):
     if wave_type not in ("P", "S"):
         raise ValueError(f"wave_type must be 'P' or 'S', got '{wave_type}'")
     if n_sub < 1:
         raise ValueError("n_sub must be >= 1")
+    if int(workers) < 1:
+        raise ValueError("workers must be >= 1")
     if device is None:
         device = _lib.require_device()
     _, cen, a_sub, keep = _grid(radius, n_sub, inclusive)
@@ -113,25 +118,66 @@ def _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type, inclusive, tol
     mat = SlabMaterial(Dlambda=np.where(mask, float(contrast.Dlambda), 0.0),
                        Dmu=np.where(mask, float(contrast.Dmu), 0.0),
                        Drho=np.where(mask, float(contrast.Drho), 0.0), ref=ref)
-    plan = SlabPlan(g, device=device)
-    plan.build_g0(omega, ref)
-    plan.set_material(mat, omega)
     U = _incident_patterns(cen, omega, ref, k_hat, wave_type)
     flat = np.flatnonzero(keep)
     psi = np.zeros((N, 9, 9), dtype=complex)
     T = np.zeros((9, 9), dtype=complex)
     n_bad = []
-    for col in range(9):
-        b = np.zeros((n_sub ** 3, 9), dtype=complex)
-        b[flat] = U[:, :, col]
-        bt = torch.as_tensor(b.reshape(-1)).to(plan.device)
-        x, st = plan.gmres(bt, x0=x0, tol=tol, max_iter=max_iter, restart=restart,
-                           precision=precision)
-        if st["rel_residual"] > tol:
-            n_bad.append((col, st["rel_residual"], st["iterations"]))
-        ls, _ = plan.source_sums(x, 0.0, 0.0, np.zeros(3))
-        T[:, col] = ls.sum(axis=0)
-        psi[:, :, col] = x.reshape(-1, 9)[torch.as_tensor(flat, device=plan.device)].cpu().numpy()
+    nw = min(int(workers), 9)
+
+    def make_plan(src):
+        plan = SlabPlan(g, device=device)
+        if nw > 1:  # no graph capture beside the other workers' allocations
+            _lib.check(plan.lib.msc_plan_use_graphs(plan._h, 0))
+        if src is None:
+            plan.build_g0(omega, ref)
+        else:
+            plan.copy_g0_from(src)
+        plan.set_material(mat, omega)
+        return plan
+
+    # The first plan builds G0 (and, with several workers, tunes its tiles
+    # once); the others copy both.
+    with torch.cuda.device(device):
+        plan0 = make_plan(None)
+        if nw > 1 and os.environ.get("MSC_AUTOTUNE", "1") != "0":
+            plan0.autotune("c128")
+            if precision == "mixed":
+                plan0.autotune("c64")
+        torch.cuda.current_stream(device).synchronize()
+
+    def run(cols, stream, plan):
+        # One plan per worker on its own stream; the 9 right-hand sides are
+        # independent solves, so the workers' launches overlap on the device.
+        with torch.cuda.device(device), torch.cuda.stream(stream):
+            if plan is None:
+                plan = make_plan(plan0)
+            idx = torch.as_tensor(flat, device=plan.device)
+            for col in cols:
+                b = np.zeros((n_sub ** 3, 9), dtype=complex)
+                b[flat] = U[:, :, col]
+                bt = torch.as_tensor(b.reshape(-1)).to(plan.device)
+                x, st = plan.gmres(bt, x0=x0, tol=tol, max_iter=max_iter, restart=restart,
+                                   precision=precision)
+                if st["rel_residual"] > tol:
+                    n_bad.append((col, st["rel_residual"], st["iterations"]))
+                ls, _ = plan.source_sums(x, 0.0, 0.0, np.zeros(3))
+                T[:, col] = ls.sum(axis=0)
+                psi[:, :, col] = x.reshape(-1, 9)[idx].cpu().numpy()
+            stream.synchronize()
+
+    streams = [torch.cuda.current_stream(device)] if nw == 1 else \
+        [torch.cuda.Stream(device) for _ in range(nw)]
+    groups = [list(range(w, 9, nw)) for w in range(nw)]
+    if nw == 1:
+        run(groups[0], streams[0], plan0)
+    else:
+        with ThreadPoolExecutor(max_workers=nw) as ex:
+            futs = [ex.submit(run, grp, st, plan0 if w == 0 else None)
+                    for w, (grp, st) in enumerate(zip(groups, streams))]
+            for f in futs:
+                f.result()
+    n_bad.sort()
     return cen, a_sub, N, psi.reshape(9 * N, 9), T, n_bad
 
 
@@ -151,7 +197,7 @@ def compute_sphere_foldy_lax_fft(omega: float, radius: float, ref: ReferenceMedi
     pattern to ``gmres_tol`` (sphere_scattering_fft.py:228-326)."""
     cen, a_sub, N, psi, T, bad = _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type,
                                         False, gmres_tol, gmres_maxiter * 20, 20, "fp64", "born",
-                                        None)
+                                        None, 3)
     for col, rr, it in bad:
         warnings.warn(f"GMRES did not converge for column {col} (rel_res={rr:.2e} after {it} "
                       "iterations)", UserWarning, stacklevel=2)
@@ -164,18 +210,21 @@ def compute_sphere_foldy_lax_fft_gpu(omega: float, radius: float, ref: Reference
                                      gmres_tol: float = 1e-8, gmres_maxiter: int = 200,
                                      initial_guess: str = "born", device=None,
                                      dtype: torch.dtype | None = None,
-                                     restart: int | None = None) -> SphereDecompositionResult:
+                                     restart: int | None = None,
+                                     workers: int = 3) -> SphereDecompositionResult:
     """Sphere T-matrix on the device (sphere_scattering_fft_gpu.py:132-244).
 
     ``dtype`` complex128 (default) solves in FP64; complex64 selects the mixed
     form (complex64 operator inside FP64 refinement, same FP64 residual
-    target).  ``restart=None`` keeps the reference's unrestarted GMRES."""
+    target).  ``restart=None`` keeps the reference's unrestarted GMRES.
+    ``workers`` concurrent solves (plans on their own streams) share the 9
+    right-hand sides."""
     if initial_guess not in ("born", "zero"):
         raise ValueError(f"Invalid initial_guess '{initial_guess}'. Must be 'born' or 'zero'.")
     precision = "mixed" if dtype == torch.complex64 else "fp64"
     cen, a_sub, N, psi, T, bad = _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type,
                                         False, gmres_tol, gmres_maxiter, restart, precision,
-                                        initial_guess, device)
+                                        initial_guess, device, workers)
     for col, rr, it in bad:
         warnings.warn(f"GPU GMRES did not converge for column {col}: rel_res={rr:.2e} after "
                       f"{it} iterations", UserWarning, stacklevel=2)
@@ -190,7 +239,7 @@ def compute_sphere_foldy_lax(omega: float, radius: float, ref: ReferenceMedium,
     here by the same device GMRES driven to 1e-13 FP64 relative residual;
     ``condition_number`` is not computed (NaN)."""
     cen, a_sub, N, psi, T, bad = _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type,
-                                        True, 1e-13, 2000, 60, "fp64", "born", None)
+                                        True, 1e-13, 2000, 60, "fp64", "born", None, 3)
     for col, rr, it in bad:
         warnings.warn(f"solve did not reach 1e-13 for column {col} (rel_res={rr:.2e})",
                       UserWarning, stacklevel=2)

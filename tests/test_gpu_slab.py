@@ -343,3 +343,33 @@ def test_autotune_and_graph_solves_are_reproducible(hip_device):
         outs.append(x.cpu().numpy())
     assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
     assert _rel(outs[1], outs[0]) < 1e-6
+
+
+@pytest.mark.parametrize("va", [False, True])
+def test_copied_g0_gives_the_same_operator(hip_device, va):
+    """msc_plan_copy_g0: a plan whose G0 (and near-neighbour correction and
+    tiles) is copied from another plan applies exactly the same operator."""
+    msc, ss = _mods()
+    M, Nz = 9, 4
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.4, 0.3, seed=11)
+    om = 692.8
+    g = ss.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    r = msc.ReferenceMedium(REF_C.alpha, REF_C.beta, REF_C.rho)
+    mat = ss.SlabMaterial(Dlambda=Dl, Dmu=Dm, Drho=Dr, ref=r)
+    src = ss.SlabPlan(g, device=hip_device)
+    src.build_g0(om, r, volume_averaged=va, n_orders=2)
+    src.set_material(mat, om)
+    src.set_tiles((1, 2, 4, 2, 1), (2, 1, 2, 1, 2))
+    dst = ss.SlabPlan(g, device=hip_device)
+    dst.set_material(mat, om)
+    dst.copy_g0_from(src)
+    assert dst.tiles() == src.tiles()
+    rng = np.random.default_rng(7)
+    x = torch.as_tensor(rng.standard_normal(dst.n) + 1j * rng.standard_normal(dst.n)).to(hip_device)
+    assert torch.equal(dst.matvec(x), src.matvec(x))
+    assert torch.equal(dst.matvec(x.to(torch.complex64)), src.matvec(x.to(torch.complex64)))
+    other = ss.SlabPlan(ss.SlabGeometry(M=M, N_z=Nz + 1, a=1.0), device=hip_device)
+    with pytest.raises(ValueError, match="differ"):
+        other.copy_g0_from(src)
+    with pytest.raises(RuntimeError, match="no G0"):
+        other.copy_g0_from(ss.SlabPlan(ss.SlabGeometry(M=M, N_z=Nz + 1, a=1.0), device=hip_device))

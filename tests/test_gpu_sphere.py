@@ -68,6 +68,26 @@ def test_dense_api_and_mixed_precision(hip_device):
         sph.compute_sphere_foldy_lax_fft(om, R, r, c, n, wave_type="Q")
 
 
+@pytest.mark.parametrize("dtype", [None, torch.complex64])
+def test_workers_give_the_serial_result(hip_device, dtype):
+    """Concurrent solves (plans on their own streams, G0 copied from the first
+    plan) return exactly the one-worker result."""
+    msc, sph = _mods()
+    r = msc.ReferenceMedium(REF.alpha, REF.beta, REF.rho)
+    c = msc.MaterialContrast(-3e9, 2e9, -200.0)
+    om, R, n = 2 * np.pi * 50.0, 12.0, 6
+    one = sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, wave_type="P", gmres_tol=1e-10,
+                                               restart=20, dtype=dtype, workers=1)
+    for w in (2, 4, 9):
+        res = sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, wave_type="P",
+                                                   gmres_tol=1e-10, restart=20, dtype=dtype,
+                                                   workers=w)
+        np.testing.assert_array_equal(res.T_comp_9x9, one.T_comp_9x9)
+        np.testing.assert_array_equal(res.psi_exc, one.psi_exc)
+    with pytest.raises(ValueError, match="workers"):
+        sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, workers=0)
+
+
 def test_two_cells_per_edge_equals_the_resonance_solver(hip_device):
     msc, sph = _mods()
     from multiplescatteringcalculations_amd import resonance_tmatrix as rt
@@ -78,3 +98,41 @@ def test_two_cells_per_edge_equals_the_resonance_solver(hip_device):
     s = sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, 2, wave_type="P", gmres_tol=1e-13)
     q = rt.compute_resonance_tmatrix(om, R, r, c, n_sub=2, wave_type="P")
     assert _rel(s.T_comp_9x9, q.T_comp_9x9) < 1e-10
+
+
+SPHERE_CFG = """\
+device:
+  backend: hip
+  dtype: {dtype}
+solver:
+  gmres_tol: 1.0e-11
+  max_iter: 300
+  initial_guess: born
+problem:
+  type: sphere
+  geometry: {{M: 4, N_z: 4, a: 1.0}}
+  reference: {{alpha: 5000.0, beta: 3000.0, rho: 2500.0}}
+  contrast: {{Dlambda: 2.0e9, Dmu: 1.0e9, Drho: 100.0}}
+  frequency: {{omega: 251.32741228718345}}
+  incident:
+    k_hat: [0.0, 0.0, 1.0]
+    wave_type: S
+  sphere: {{radius: 12.0, n_sub: 4}}
+"""
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-9), ("complex64", 1e-6)])
+def test_run_from_config_sphere(hip_device, tmp_path, dtype, tol):
+    """type: sphere in a YAML config runs the FFT sphere solve
+    (solver_config.py:440-475) and matches the oracle's dense solve.  The
+    reference's validator requires problem.geometry for spheres too
+    (solver_config.py:307-317), so the config carries one."""
+    from multiplescatteringcalculations_amd.solver_config import load_config, run_from_config
+
+    p = tmp_path / "sphere.yml"
+    p.write_text(SPHERE_CFG.format(dtype=dtype))
+    res = run_from_config(load_config(p))
+    Tc, _, cen = osph.sphere_tmatrix(251.32741228718345, 12.0, REF, 2e9, 1e9, 100.0, 4,
+                                     k_hat=[0.0, 0.0, 1.0], wave_type="S", inclusive=False)
+    assert res.n_cells == len(cen) and res.n_sub == 4
+    assert _rel(res.T_comp_9x9, Tc) < tol

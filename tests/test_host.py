@@ -284,6 +284,19 @@ def test_random_iid_schema_and_cpu_backend_refused(cfg_file):
         run_from_config(c)
 
 
+def test_sphere_config_validated_and_cpu_backend_refused(cfg_file):
+    from multiplescatteringcalculations_amd.solver_config import load_config, run_from_config
+
+    text = VALID_SLAB.replace("type: slab", "type: sphere").replace(
+        "  slab:\n    material_type: uniform\n", "  sphere: {radius: 10.0, n_sub: 4}\n")
+    c = load_config(cfg_file(text))
+    assert c.problem.type == "sphere" and c.problem.sphere == {"radius": 10.0, "n_sub": 4}
+    with pytest.raises(ValueError, match="HIP devices only"):
+        run_from_config(c)
+    with pytest.raises(ValueError, match="radius"):
+        load_config(cfg_file(text.replace("{radius: 10.0, n_sub: 4}", "{n_sub: 4}")))
+
+
 def test_sweep_and_reshape_validate_arguments():
     """msc_sweep and msc_plan_reshape reject bad arguments before any device work."""
     from multiplescatteringcalculations_amd import _lib
