@@ -89,55 +89,17 @@ struct Halo {
   static constexpr int ELEMS = 3 * ROWS * COLS * LDV;
 };
 
-// NS: channels of the input that can be non-zero (staged); NI: channels stored.
-template <int NI, int NO, int EPI, bool MIRROR, int NS = NI>
-__global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
-  static_assert(NS % 4 == 0 && NS <= NI, "staged channels must be a multiple of 4");
+// One output plane d of a workgroup's tile (rows h0..h0+3, voxels w0..w0+63)
+// from the three staged input planes d-1, d, d+1, which sit in LDS plane
+// slots (sbase + 0, 1, 2) mod 3 of sh.  MFMA over the 27 taps, then the
+// epilogue EPI.
+template <int NI, int NO, int EPI, bool MIRROR, int NS>
+__device__ __forceinline__ void conv_plane(const ConvArgs& a, const uint16_t* sh, int sbase,
+                                           int d, int h0, int w0) {
   constexpr int NOT = NO / 16, NIB = NI / 16, LDV = Halo<NS>::LDV;
   constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
-  __shared__ __align__(16) uint16_t sh[Halo<NS>::ELEMS];
-  if (a.done != nullptr && *a.done) return;
-  const int D = a.D, H = a.H, W = a.W;
-  const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
-  int b = blockIdx.x;
-  const int bw = b % nbw;
-  b /= nbw;
-  const int bh = b % nbh;
-  const int d = b / nbh;
-  const int h0 = bh * kTH, w0 = bw * kTW;
-  // stage the halo: 4-channel groups (8 bytes), zero outside the grid.  The
-  // loads go out in batches of SB before their LDS stores, so a thread waits
-  // for memory once per batch rather than once per element.
-  constexpr int G4 = NS / 4;
-  constexpr int NSTAGE = 3 * ROWS * COLS * G4;
-  constexpr int SB = 8;
-  for (int e0 = 0; e0 < NSTAGE; e0 += SB * kNetNT) {
-    v4s val[SB];
-    int off[SB];
-#pragma unroll
-    for (int k = 0; k < SB; ++k) {
-      const int e = e0 + k * kNetNT + threadIdx.x;
-      val[k] = v4s{0, 0, 0, 0};
-      off[k] = -1;
-      if (e < NSTAGE) {
-        const int g = e % G4;
-        int q = e / G4;
-        const int c = q % COLS;
-        q /= COLS;
-        const int rr = q % ROWS;
-        const int pl = q / ROWS;
-        const int gd = d + pl - 1, gh = h0 + rr - 1, gw = w0 + c - 1;
-        off[k] = ((pl * ROWS + rr) * COLS + c) * LDV + g * 4;
-        if (gd >= 0 && gd < D && gh >= 0 && gh < H && gw >= 0 && gw < W)
-          val[k] = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI +
-                                                 g * 4);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < SB; ++k)
-      if (off[k] >= 0) *reinterpret_cast<v4s*>(&sh[off[k]]) = val[k];
-  }
-  __syncthreads();
+  constexpr int PLANE = ROWS * COLS * LDV;
+  const int H = a.H, W = a.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
   v4f acc[NOT][kVT];
@@ -145,6 +107,9 @@ __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
   for (int o = 0; o < NOT; ++o)
 #pragma unroll
     for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  // one z offset per unrolled group: a full unroll holds every tap's weight
+  // fragments in registers across the plane loop of k_conv_zm
+#pragma unroll 9
   for (int tap = 0; tap < 27; ++tap) {
     int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
     if (MIRROR) {
@@ -159,8 +124,8 @@ __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
       for (int ib = 0; ib < NIB; ++ib)
         wf[o][ib] = *reinterpret_cast<const v4s*>(a.w + ((size_t)tap * NO + o * 16 + col) * NI +
                                                   ib * 16 + grp * 4);
-    const uint16_t* base =
-        sh + (((1 + dz) * ROWS + (1 + wave + dx)) * COLS + (1 + dy + col)) * LDV + grp * 4;
+    const uint16_t* base = sh + ((sbase + 1 + dz) % 3) * PLANE +
+                           ((1 + wave + dx) * COLS + (1 + dy + col)) * LDV + grp * 4;
 #pragma unroll
     for (int t = 0; t < kVT; ++t) {
 #pragma unroll
@@ -229,6 +194,126 @@ __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
         *reinterpret_cast<v4s*>(reinterpret_cast<uint16_t*>(a.out) + v * NO + c0) = pk;
       }
     }
+  }
+}
+
+// NS: channels of the input that can be non-zero (staged); NI: channels stored.
+template <int NI, int NO, int EPI, bool MIRROR, int NS = NI>
+__global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
+  static_assert(NS % 4 == 0 && NS <= NI, "staged channels must be a multiple of 4");
+  constexpr int LDV = Halo<NS>::LDV;
+  constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
+  __shared__ __align__(16) uint16_t sh[Halo<NS>::ELEMS];
+  if (a.done != nullptr && *a.done) return;
+  const int D = a.D, H = a.H, W = a.W;
+  const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
+  int b = blockIdx.x;
+  const int bw = b % nbw;
+  b /= nbw;
+  const int bh = b % nbh;
+  const int d = b / nbh;
+  const int h0 = bh * kTH, w0 = bw * kTW;
+  // stage the halo: 4-channel groups (8 bytes), zero outside the grid.  The
+  // loads go out in batches of SB before their LDS stores, so a thread waits
+  // for memory once per batch rather than once per element.
+  constexpr int G4 = NS / 4;
+  constexpr int NSTAGE = 3 * ROWS * COLS * G4;
+  constexpr int SB = 8;
+  for (int e0 = 0; e0 < NSTAGE; e0 += SB * kNetNT) {
+    v4s val[SB];
+    int off[SB];
+#pragma unroll
+    for (int k = 0; k < SB; ++k) {
+      const int e = e0 + k * kNetNT + threadIdx.x;
+      val[k] = v4s{0, 0, 0, 0};
+      off[k] = -1;
+      if (e < NSTAGE) {
+        const int g = e % G4;
+        int q = e / G4;
+        const int c = q % COLS;
+        q /= COLS;
+        const int rr = q % ROWS;
+        const int pl = q / ROWS;
+        const int gd = d + pl - 1, gh = h0 + rr - 1, gw = w0 + c - 1;
+        off[k] = ((pl * ROWS + rr) * COLS + c) * LDV + g * 4;
+        if (gd >= 0 && gd < D && gh >= 0 && gh < H && gw >= 0 && gw < W)
+          val[k] = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI +
+                                                 g * 4);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SB; ++k)
+      if (off[k] >= 0) *reinterpret_cast<v4s*>(&sh[off[k]]) = val[k];
+  }
+  __syncthreads();
+  conv_plane<NI, NO, EPI, MIRROR, NS>(a, sh, 0, d, h0, w0);
+}
+
+// Forward convolution marching along z: a workgroup computes planes
+// d0 .. d0+zc-1 of its tile with a ring of three staged input planes in LDS,
+// and the loads of plane d+2 are in flight while plane d is computed.  Each
+// output plane stages one new input plane (three in k_conv).
+template <int NI, int NO, int EPI, int NS = NI>
+__global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
+  static_assert(NS % 4 == 0 && NS <= NI, "staged channels must be a multiple of 4");
+  constexpr int LDV = Halo<NS>::LDV;
+  constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
+  constexpr int PLANE = ROWS * COLS * LDV;
+  constexpr int G4 = NS / 4;
+  constexpr int NPL = ROWS * COLS * G4;
+  constexpr int PT = (NPL + kNetNT - 1) / kNetNT;
+  __shared__ __align__(16) uint16_t sh[3 * PLANE];
+  if (a.done != nullptr && *a.done) return;
+  const int D = a.D, H = a.H, W = a.W;
+  const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
+  int b = blockIdx.x;
+  const int bw = b % nbw;
+  b /= nbw;
+  const int bh = b % nbh;
+  const int d0 = (b / nbh) * zc;
+  const int d1 = min(D, d0 + zc);
+  const int h0 = bh * kTH, w0 = bw * kTW;
+  v4s val[PT];
+  auto load = [&](int gd) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = k * kNetNT + (int)threadIdx.x;
+      val[k] = v4s{0, 0, 0, 0};
+      if (e < NPL && gd >= 0 && gd < D) {
+        const int g = e % G4;
+        const int q = e / G4;
+        const int c = q % COLS, rr = q / COLS;
+        const int gh = h0 + rr - 1, gw = w0 + c - 1;
+        if (gh >= 0 && gh < H && gw >= 0 && gw < W)
+          val[k] = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI +
+                                                 g * 4);
+      }
+    }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int e = k * kNetNT + (int)threadIdx.x;
+      if (e < NPL) {
+        const int g = e % G4;
+        const int q = e / G4;
+        const int c = q % COLS, rr = q / COLS;
+        *reinterpret_cast<v4s*>(&sh[slot * PLANE + (rr * COLS + c) * LDV + g * 4]) = val[k];
+      }
+    }
+  };
+  // plane gd >= -1 lives in slot (gd + 3) % 3
+  load(d0 - 1);
+  store((d0 + 2) % 3);
+  load(d0);
+  store(d0 % 3);
+  load(d0 + 1);
+  for (int d = d0; d < d1; ++d) {
+    store((d + 1) % 3);  // its slot held plane d-2, last read before the barrier below
+    __syncthreads();
+    if (d + 1 < d1) load(d + 2);
+    conv_plane<NI, NO, EPI, false, NS>(a, sh, (d + 2) % 3, d, h0, w0);
+    __syncthreads();
   }
 }
 
@@ -621,28 +706,46 @@ dim3 conv_grid(const Net& net) {
   return dim3((unsigned)(net.D * nbh * nbw));
 }
 
+// Planes per workgroup of the z-marching forward convolution: 4 stages 1.5
+// input planes per output plane and leaves 512 workgroups on a 128x128x32
+// grid (2 and 8 planes measured slower there).
+constexpr int kConvZc = 4;
+int conv_zc() { return kConvZc; }
+
+dim3 conv_grid_zm(const Net& net, int zc) {
+  const int nbw = (net.W + kTW - 1) / kTW, nbh = (net.H + kTH - 1) / kTH;
+  return dim3((unsigned)(((net.D + zc - 1) / zc) * nbh * nbw));
+}
+
 // The four-layer pass per hidden width.  forward() ends in outf (fp32);
 // apply() ends in z = r + N/s (complex, R) with the scale of net.scal.
 template <int HID>
 struct NetOps {
   static void hidden_layers(Net& net, const uint16_t* in, hipStream_t s, const int* done) {
-    const dim3 G = conv_grid(net);
-    hipLaunchKernelGGL((k_conv<kNetIOPad, HID, kEpiHidden, false, kIOStaged>), G, dim3(kNetNT), 0, s,
+    const int zc = conv_zc();
+    const dim3 G = conv_grid_zm(net, zc);
+    hipLaunchKernelGGL((k_conv_zm<kNetIOPad, HID, kEpiHidden, kIOStaged>), G, dim3(kNetNT), 0, s,
                        conv_args(net, in, net.wpk + net.wpad_off[0], net.bpk + net.bpad_off[0],
-                                 nullptr, net.act[1], done));
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiHidden, false>), G, dim3(kNetNT), 0, s,
+                                 nullptr, net.act[1], done),
+                       zc);
+    hipLaunchKernelGGL((k_conv_zm<HID, HID, kEpiHidden>), G, dim3(kNetNT), 0, s,
                        conv_args(net, net.act[1], net.wpk + net.wpad_off[1],
-                                 net.bpk + net.bpad_off[1], nullptr, net.act[2], done));
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiHidden, false>), G, dim3(kNetNT), 0, s,
+                                 net.bpk + net.bpad_off[1], nullptr, net.act[2], done),
+                       zc);
+    hipLaunchKernelGGL((k_conv_zm<HID, HID, kEpiHidden>), G, dim3(kNetNT), 0, s,
                        conv_args(net, net.act[2], net.wpk + net.wpad_off[2],
-                                 net.bpk + net.bpad_off[2], nullptr, net.act[3], done));
+                                 net.bpk + net.bpad_off[2], nullptr, net.act[3], done),
+                       zc);
   }
 
   static void forward(Net& net, const uint16_t* in, hipStream_t s, const int* done) {
     hidden_layers(net, in, s, done);
-    hipLaunchKernelGGL((k_conv<HID, kNetIOPad, kEpiLast, false>), conv_grid(net), dim3(kNetNT), 0,
-                       s, conv_args(net, net.act[3], net.wpk + net.wpad_off[3],
-                                    net.bpk + net.bpad_off[3], nullptr, net.outf, done));
+    const int zc = conv_zc();
+    hipLaunchKernelGGL((k_conv_zm<HID, kNetIOPad, kEpiLast>), conv_grid_zm(net, zc),
+                       dim3(kNetNT), 0, s,
+                       conv_args(net, net.act[3], net.wpk + net.wpad_off[3],
+                                 net.bpk + net.bpad_off[3], nullptr, net.outf, done),
+                       zc);
   }
 
   template <typename R>
@@ -653,8 +756,9 @@ struct NetOps {
     a.r = r;
     a.scal = net.scal;
     constexpr int EPI = sizeof(R) == 4 ? kEpiApply64 : kEpiApply128;
-    hipLaunchKernelGGL((k_conv<HID, kNetIOPad, EPI, false>), conv_grid(net), dim3(kNetNT), 0, s,
-                       a);
+    const int zc = conv_zc();
+    hipLaunchKernelGGL((k_conv_zm<HID, kNetIOPad, EPI>), conv_grid_zm(net, zc), dim3(kNetNT), 0,
+                       s, a, zc);
   }
 
   // Backward of the loss gradient held in gbuf[0] ([v][kNetIOPad]); `in` is
