@@ -347,7 +347,6 @@ __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
   }
 }
 
-// ── pass 5: inverse FFT along y, keep y < M, y = x - acc (or b - x + acc) ──
 // ── pass 5: inverse FFT along y, keep y < M and form the output ────────────
 // MODE 0: out = x - acc (the operator), 1: out = b - x + acc (residual),
 // 2: out = acc (the scattered part G0 T0 x).
