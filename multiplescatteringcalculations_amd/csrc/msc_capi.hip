@@ -209,6 +209,7 @@ void free_plan_buffers(Plan& p) {
   if (p.tp64) (void)hipFree(p.tp64);
   if (p.tmax) (void)hipFree(p.tmax);
   if (p.work) (void)hipFree(p.work);
+  if (p.work2) (void)hipFree(p.work2);
   if (p.nn128) (void)hipFree(p.nn128);
   if (p.nn64) (void)hipFree(p.nn64);
   if (p.nn_stage) (void)hipFree(p.nn_stage);
@@ -216,6 +217,7 @@ void free_plan_buffers(Plan& p) {
   p.zmap = nullptr;
   p.tmax = nullptr;
   p.work = nullptr;
+  p.work2 = nullptr;
   p.nn128 = nullptr;
   p.nn64 = nullptr;
   p.nn_stage = nullptr;

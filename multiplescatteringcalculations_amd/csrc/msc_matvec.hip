@@ -47,12 +47,6 @@ __device__ __forceinline__ void staged(int n, Load load, Use use) {
   }
 }
 
-// Component sign flips of the lattice reflections (z, x, y) acting on the
-// 9-vector (u_z, u_x, u_y, e_zz, e_xx, e_yy, e_xy, e_zy, e_zx):
-// a component flips when it carries an odd number of the reflected index.
-constexpr unsigned kFlipZ = (1u << 0) | (1u << 7) | (1u << 8);
-constexpr unsigned kFlipX = (1u << 1) | (1u << 6) | (1u << 8);
-constexpr unsigned kFlipY = (1u << 2) | (1u << 6) | (1u << 7);
 
 template <typename R>
 __device__ __forceinline__ cx<R> flip_if(cx<R> v, unsigned mask, int c) {
@@ -858,11 +852,12 @@ template int scattered<double>(Plan&, const double2*, double2*, hipStream_t, con
 template int scattered<float>(Plan&, const float2*, float2*, hipStream_t, const int*);
 
 // ── generic strided batched FFT (table build, tests) ───────────────────────
-// data [outer][N][inner]; a workgroup transforms NCOL consecutive inner
-// indices of one outer slice.
+// data [outer][N][inner] -> [outer][nout][inner] (the first nout frequencies
+// kept, nout <= N; in place only when nout == N); a workgroup transforms NCOL
+// consecutive inner indices of one outer slice.
 template <typename R, int N, int NCOL, bool INV>
 __global__ __launch_bounds__(kNT) void k_fft_axis(const cx<R>* __restrict__ in,
-                                                 cx<R>* __restrict__ out, size_t inner,
+                                                 cx<R>* __restrict__ out, size_t inner, int nout,
                                                  const cx<R>* __restrict__ tw) {
   constexpr int LD = lds_ld<NCOL>::value;
   __shared__ cx<R> s[N * LD];
@@ -870,6 +865,7 @@ __global__ __launch_bounds__(kNT) void k_fft_axis(const cx<R>* __restrict__ in,
   const size_t o = blockIdx.x / nchunk;
   const size_t i0 = (blockIdx.x - o * nchunk) * NCOL;
   const size_t base = o * (size_t)N * inner + i0;
+  const size_t obase = o * (size_t)nout * inner + i0;
   for (int w = threadIdx.x; w < N * NCOL; w += kNT) {
     const int r = w / NCOL;
     const int e = w - r * NCOL;
@@ -879,34 +875,36 @@ __global__ __launch_bounds__(kNT) void k_fft_axis(const cx<R>* __restrict__ in,
   }
   __syncthreads();
   lds_fft<R, N, INV, NCOL, LD, kNT>(s, tw);
-  for (int w = threadIdx.x; w < N * NCOL; w += kNT) {
+  for (int w = threadIdx.x; w < nout * NCOL; w += kNT) {
     const int r = w / NCOL;
     const int e = w - r * NCOL;
-    if (i0 + e < inner) out[base + (size_t)r * inner + e] = s[r * LD + e];
+    if (i0 + e < inner) out[obase + (size_t)r * inner + e] = s[r * LD + e];
   }
 }
 
 template <typename R, int N>
-static int launch_fft_axis(const cx<R>* in, cx<R>* out, int outer, size_t inner, bool inv,
-                           const cx<R>* tw, hipStream_t st) {
+static int launch_fft_axis(const cx<R>* in, cx<R>* out, int outer, size_t inner, int nout,
+                           bool inv, const cx<R>* tw, hipStream_t st) {
   constexpr int NCOL = (N <= 64) ? 16 : ((N * 16 * sizeof(cx<R>) <= 80 * 1024) ? 16 : 8);
   const size_t nchunk = (inner + NCOL - 1) / NCOL;
   const size_t nblk = (size_t)outer * nchunk;
   MSC_REQUIRE(nblk < (1ull << 31), "FFT batch too large");
   if (inv) {
     hipLaunchKernelGGL((k_fft_axis<R, N, NCOL, true>), dim3((unsigned)nblk), dim3(kNT), 0, st, in,
-                       out, inner, tw);
+                       out, inner, nout, tw);
   } else {
     hipLaunchKernelGGL((k_fft_axis<R, N, NCOL, false>), dim3((unsigned)nblk), dim3(kNT), 0, st, in,
-                       out, inner, tw);
+                       out, inner, nout, tw);
   }
   MSC_LAUNCH_CHECK();
   return kOk;
 }
 
 template <typename R>
-int fft_axis(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, bool inverse,
-             const cx<R>* tw, hipStream_t st) {
+int fft_axis_pruned(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, int nout,
+                    bool inverse, const cx<R>* tw, hipStream_t st) {
+  MSC_REQUIRE(nout >= 1 && nout <= N, "fft_axis: nout must be in [1, N]");
+  MSC_REQUIRE(nout == N || in != out, "fft_axis: a pruned transform cannot run in place");
   switch (N) {
     case 1: {
       if (in != out) {
@@ -915,25 +913,33 @@ int fft_axis(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, bool i
       }
       return kOk;
     }
-    case 2: return launch_fft_axis<R, 2>(in, out, outer, inner, inverse, tw, st);
-    case 4: return launch_fft_axis<R, 4>(in, out, outer, inner, inverse, tw, st);
-    case 8: return launch_fft_axis<R, 8>(in, out, outer, inner, inverse, tw, st);
-    case 16: return launch_fft_axis<R, 16>(in, out, outer, inner, inverse, tw, st);
-    case 32: return launch_fft_axis<R, 32>(in, out, outer, inner, inverse, tw, st);
-    case 64: return launch_fft_axis<R, 64>(in, out, outer, inner, inverse, tw, st);
-    case 128: return launch_fft_axis<R, 128>(in, out, outer, inner, inverse, tw, st);
-    case 256: return launch_fft_axis<R, 256>(in, out, outer, inner, inverse, tw, st);
-    case 512: return launch_fft_axis<R, 512>(in, out, outer, inner, inverse, tw, st);
-    case 1024: return launch_fft_axis<R, 1024>(in, out, outer, inner, inverse, tw, st);
+    case 2: return launch_fft_axis<R, 2>(in, out, outer, inner, nout, inverse, tw, st);
+    case 4: return launch_fft_axis<R, 4>(in, out, outer, inner, nout, inverse, tw, st);
+    case 8: return launch_fft_axis<R, 8>(in, out, outer, inner, nout, inverse, tw, st);
+    case 16: return launch_fft_axis<R, 16>(in, out, outer, inner, nout, inverse, tw, st);
+    case 32: return launch_fft_axis<R, 32>(in, out, outer, inner, nout, inverse, tw, st);
+    case 64: return launch_fft_axis<R, 64>(in, out, outer, inner, nout, inverse, tw, st);
+    case 128: return launch_fft_axis<R, 128>(in, out, outer, inner, nout, inverse, tw, st);
+    case 256: return launch_fft_axis<R, 256>(in, out, outer, inner, nout, inverse, tw, st);
+    case 512: return launch_fft_axis<R, 512>(in, out, outer, inner, nout, inverse, tw, st);
+    case 1024: return launch_fft_axis<R, 1024>(in, out, outer, inner, nout, inverse, tw, st);
     default: break;
   }
   set_error("fft_axis: length must be a power of two <= 1024");
   return kUnsupported;
 }
 
+template <typename R>
+int fft_axis(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, bool inverse,
+             const cx<R>* tw, hipStream_t st) {
+  return fft_axis_pruned<R>(in, out, outer, N, inner, N, inverse, tw, st);
+}
+
 template int fft_axis<double>(const double2*, double2*, int, int, size_t, bool, const double2*,
                               hipStream_t);
 template int fft_axis<float>(const float2*, float2*, int, int, size_t, bool, const float2*,
                              hipStream_t);
+template int fft_axis_pruned<double>(const double2*, double2*, int, int, size_t, int, bool,
+                                     const double2*, hipStream_t);
 
 }  // namespace msc

@@ -20,6 +20,15 @@ namespace msc {
 
 constexpr int kNE = 45;  // independent entries of the 9x9 spectral propagator
 
+// Component sign flips of the lattice reflections (z, x, y) acting on the
+// 9-vector (u_z, u_x, u_y, e_zz, e_xx, e_yy, e_xy, e_zy, e_zx):
+// a component flips when it carries an odd number of the reflected index.
+// The propagator block obeys P(R d) = S P(d) S (S the flip signs), in space
+// and in k.
+constexpr unsigned kFlipZ = (1u << 0) | (1u << 7) | (1u << 8);
+constexpr unsigned kFlipX = (1u << 1) | (1u << 6) | (1u << 8);
+constexpr unsigned kFlipY = (1u << 2) | (1u << 6) | (1u << 7);
+
 struct Krylov;  // GMRES workspace (msc_gmres.hip)
 
 struct Plan {
@@ -48,7 +57,8 @@ struct Plan {
   unsigned int* tmax = nullptr;
   void* buf1 = nullptr;  // (Nz, M, Pxy, 9) complex128 bytes
   void* buf2 = nullptr;  // (Nz, Pxy, Pxy, 9) complex128 bytes
-  void* work = nullptr;  // G0 table build: (Pz, Pxy, Pxy, 45) complex128, kept for rebuilds
+  void* work = nullptr;   // G0 table build: (Pz, Pxy, Pxy, 45) complex128, kept for rebuilds
+  void* work2 = nullptr;  // its pruned half: (Pz, Pxy, Pxy/2 + 1, 45) complex128
   // volume-averaged nearest-neighbour correction: for the 26 offsets R in
   // {-1,0,1}^3 \ 0 (lexicographic (dz, dx, dy)), the 9x9 difference between the
   // volume-averaged block and the point propagator the table holds, row-major
@@ -99,6 +109,11 @@ int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st);
 template <typename R>
 int fft_axis(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, bool inverse,
              const cx<R>* tw, hipStream_t st);
+// The same keeping only the first nout frequencies: out is [outer][nout][inner]
+// (not in place unless nout == N).
+template <typename R>
+int fft_axis_pruned(const cx<R>* in, cx<R>* out, int outer, int N, size_t inner, int nout,
+                    bool inverse, const cx<R>* tw, hipStream_t st);
 
 int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st);
 // volume_averaged: set the nearest-neighbour correction for the plan's table

@@ -25,21 +25,41 @@ namespace {
 // Spatial kernel on the circular grid (Pz, Pxy, Pxy): offset d stored at
 // d mod P.  Aperiodic: |dz| < Nz, |dx|,|dy| < M.  Periodic: Pxy = M and the
 // lateral offsets dx in {j, j - M} (|dx| < M) are summed (fold without Bloch
-// phase, slab_scattering.py:337-346).  The self term is zero.
+// phase, slab_scattering.py:337-346).  The self term is zero.  Each thread
+// evaluates one point of the octant 0 <= iz <= Pz/2, 0 <= ix, iy <= Pxy/2 and
+// writes it and its (up to 7) reflections: entry (i, j) of the block changes
+// sign under a reflection when exactly one of components i, j flips.
+__device__ __forceinline__ unsigned entry_pair(int e) {
+  // component pair (i, j) of entry e in add_prop45's layout, as a bit mask
+  if (e < 6) {
+    const int i = e < 3 ? 0 : (e < 5 ? 1 : 2);
+    const int j = e < 3 ? e : (e < 5 ? e - 2 : 2);
+    return (1u << i) ^ (1u << j);
+  }
+  if (e < 24) return (1u << ((e - 6) / 6)) ^ (1u << (3 + (e - 6) % 6));
+  int rem = e - 24, al = 0;
+  while (rem >= 6 - al) {
+    rem -= 6 - al;
+    ++al;
+  }
+  return (1u << (3 + al)) ^ (1u << (3 + al + rem));
+}
+
 __global__ __launch_bounds__(128) void k_g0_spatial(double2* __restrict__ work, int Pz, int Pxy,
                                                     int Nz, int M, int periodic, double d,
                                                     double kP, double kS, double pre) {
-  const size_t total = (size_t)Pz * Pxy * Pxy;
+  const int hz = Pz / 2, hxy = Pxy / 2;
+  const size_t total = (size_t)(hz + 1) * (hxy + 1) * (hxy + 1);
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (size_t)gridDim.x * blockDim.x) {
-    const int iy = (int)(idx % Pxy);
-    const int ix = (int)((idx / Pxy) % Pxy);
-    const int iz = (int)(idx / ((size_t)Pxy * Pxy));
+    const int iy = (int)(idx % (hxy + 1));
+    const int ix = (int)((idx / (hxy + 1)) % (hxy + 1));
+    const int iz = (int)(idx / ((size_t)(hxy + 1) * (hxy + 1)));
     double2 acc[kNE];
 #pragma unroll
     for (int e = 0; e < kNE; ++e) acc[e] = zc();
-    const int dz = iz <= Pz / 2 ? iz : iz - Pz;
-    if (dz > -Nz && dz < Nz) {
+    const int dz = iz;
+    if (dz < Nz) {
       int xs[2], ys[2], nx = 0, ny = 0;
       if (periodic) {
         xs[nx++] = ix;
@@ -47,10 +67,8 @@ __global__ __launch_bounds__(128) void k_g0_spatial(double2* __restrict__ work, 
         ys[ny++] = iy;
         if (iy >= 1) ys[ny++] = iy - M;
       } else {
-        const int dx = ix <= Pxy / 2 ? ix : ix - Pxy;
-        const int dy = iy <= Pxy / 2 ? iy : iy - Pxy;
-        if (dx > -M && dx < M) xs[nx++] = dx;
-        if (dy > -M && dy < M) ys[ny++] = dy;
+        if (ix < M) xs[nx++] = ix;
+        if (iy < M) ys[ny++] = iy;
       }
       for (int a = 0; a < nx; ++a) {
         for (int b = 0; b < ny; ++b) {
@@ -59,9 +77,21 @@ __global__ __launch_bounds__(128) void k_g0_spatial(double2* __restrict__ work, 
         }
       }
     }
-    double2* o = work + idx * kNE;
+    for (int r = 0; r < 8; ++r) {
+      const bool fz = r & 1, fx = r & 2, fy = r & 4;
+      // a reflection that maps the index onto itself (0 or P/2) adds no point
+      if ((fz && (iz == 0 || iz == hz)) || (fx && (ix == 0 || ix == hxy)) ||
+          (fy && (iy == 0 || iy == hxy)))
+        continue;
+      const int jz = fz ? Pz - iz : iz, jx = fx ? Pxy - ix : ix, jy = fy ? Pxy - iy : iy;
+      const unsigned m = (fz ? kFlipZ : 0u) ^ (fx ? kFlipX : 0u) ^ (fy ? kFlipY : 0u);
+      double2* o = work + (((size_t)jz * Pxy + jx) * Pxy + jy) * kNE;
 #pragma unroll
-    for (int e = 0; e < kNE; ++e) o[e] = acc[e];
+      for (int e = 0; e < kNE; ++e) {
+        const bool neg = (__popc(m & entry_pair(e)) & 1) != 0;
+        o[e] = neg ? make_double2(-acc[e].x, -acc[e].y) : acc[e];
+      }
+    }
   }
 }
 
@@ -73,9 +103,10 @@ __device__ __forceinline__ void uv_split(int uv, int& u, int& v) {
   v = uv - t * (t + 1) / 2;
 }
 
-// table[(uv*45 + e)*NC + c] = K^(kz=c, kx=u, ky=v)[e] / P
-__global__ __launch_bounds__(256) void k_table_compress(const double2* __restrict__ work, int Pz,
-                                                        int Pxy, int NC, int NUV, double invP,
+// table[(uv*45 + e)*NC + c] = K^(kz=c, kx=u, ky=v)[e] / P from the pruned
+// spectrum spec[c][u][v][45] (c < NC, u, v < H = Pxy/2 + 1)
+__global__ __launch_bounds__(256) void k_table_compress(const double2* __restrict__ spec, int H,
+                                                        int NC, int NUV, double invP,
                                                         double2* __restrict__ t128,
                                                         float2* __restrict__ t64) {
   const size_t total = (size_t)NUV * kNE * NC;
@@ -86,7 +117,7 @@ __global__ __launch_bounds__(256) void k_table_compress(const double2* __restric
     const int uv = (int)(idx / ((size_t)NC * kNE));
     int u, v;
     uv_split(uv, u, v);
-    const double2 s = work[(((size_t)c * Pxy + u) * Pxy + v) * kNE + e];
+    const double2 s = spec[(((size_t)c * H + u) * H + v) * kNE + e];
     const double2 o = make_double2(s.x * invP, s.y * invP);
     t128[idx] = o;
     t64[idx] = make_float2((float)o.x, (float)o.y);
@@ -272,22 +303,31 @@ __global__ __launch_bounds__(32) void k_nn_delta(const double* __restrict__ va, 
 
 int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st) {
   const size_t P = (size_t)p.Pz * p.Pxy * p.Pxy;
+  const int P2 = p.Pxy, H = p.Pxy / 2 + 1;
   if (p.work == nullptr) MSC_HIP_CHECK(hipMalloc(&p.work, P * kNE * sizeof(double2)));
+  if (p.work2 == nullptr)
+    MSC_HIP_CHECK(hipMalloc(&p.work2, (size_t)p.Pz * P2 * H * kNE * sizeof(double2)));
   double2* work = reinterpret_cast<double2*>(p.work);
+  double2* work2 = reinterpret_cast<double2*>(p.work2);
   const double kP = omega / alpha, kS = omega / beta;
   const double pre = 1.0 / (4.0 * M_PI * rho * omega * omega);
-  hipLaunchKernelGGL(k_g0_spatial, dim3(grid1d(P, 128)), dim3(128), 0, st, work, p.Pz, p.Pxy,
-                     p.Nz, p.M, p.periodic ? 1 : 0, 2.0 * p.a, kP, kS, pre);
+  const size_t octant = (size_t)(p.Pz / 2 + 1) * H * H;
+  hipLaunchKernelGGL(k_g0_spatial, dim3(grid1d(octant, 128)), dim3(128), 0, st, work, p.Pz,
+                     p.Pxy, p.Nz, p.M, p.periodic ? 1 : 0, 2.0 * p.a, kP, kS, pre);
   int rc = kOk;
-  const int P2 = p.Pxy;
-  // along y: [Pz*Pxy][Pxy][45]; along x: [Pz][Pxy][Pxy*45]; along z: [1][Pz][Pxy*Pxy*45]
-  rc = fft_axis<double>(work, work, p.Pz * P2, P2, kNE, false, p.tw128, st);
-  if (!rc) rc = fft_axis<double>(work, work, p.Pz, P2, (size_t)P2 * kNE, false, p.tw128, st);
-  if (!rc) rc = fft_axis<double>(work, work, 1, p.Pz, (size_t)P2 * P2 * kNE, false, p.tw128, st);
+  // The table keeps k_y <= Pxy/2, k_x <= Pxy/2, k_z <= Pz/2, so each pass keeps
+  // only those outputs:  y: [Pz*Pxy][Pxy][45] -> work2 [Pz*Pxy][H][45];
+  // x: [Pz][Pxy][H*45] -> work [Pz][H][H*45];  z: [1][Pz][H*H*45] -> work2 [NC][H*H*45]
+  rc = fft_axis_pruned<double>(work, work2, p.Pz * P2, P2, kNE, H, false, p.tw128, st);
+  if (!rc)
+    rc = fft_axis_pruned<double>(work2, work, p.Pz, P2, (size_t)H * kNE, H, false, p.tw128, st);
+  if (!rc)
+    rc = fft_axis_pruned<double>(work, work2, 1, p.Pz, (size_t)H * H * kNE, p.NC, false,
+                                 p.tw128, st);
   if (!rc) {
     const size_t tot = (size_t)p.NUV * kNE * p.NC;
-    hipLaunchKernelGGL(k_table_compress, dim3(grid1d(tot, 256)), dim3(256), 0, st, work, p.Pz,
-                       p.Pxy, p.NC, p.NUV, 1.0 / (double)P, p.table128, p.table64);
+    hipLaunchKernelGGL(k_table_compress, dim3(grid1d(tot, 256)), dim3(256), 0, st, work2, H,
+                       p.NC, p.NUV, 1.0 / (double)P, p.table128, p.table64);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       set_error(std::string("table compress launch: ") + hipGetErrorString(e));
