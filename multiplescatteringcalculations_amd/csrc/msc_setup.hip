@@ -27,8 +27,10 @@ namespace {
 // lateral offsets dx in {j, j - M} (|dx| < M) are summed (fold without Bloch
 // phase, slab_scattering.py:337-346).  The self term is zero.  Each thread
 // evaluates one point of the octant 0 <= iz <= Pz/2, 0 <= ix, iy <= Pxy/2 and
-// writes it and its (up to 7) reflections: entry (i, j) of the block changes
-// sign under a reflection when exactly one of components i, j flips.
+// writes it and its y reflection into work[iz][ix][Pxy][45]: the z and x
+// reflections are not stored (the FFT passes along x and z read them from the
+// stored half, k_fft_mirror).  Entry (i, j) of the block changes sign under a
+// reflection when exactly one of components i, j flips.
 __device__ __forceinline__ unsigned entry_pair(int e) {
   // component pair (i, j) of entry e in add_prop45's layout, as a bit mask
   if (e < 6) {
@@ -77,22 +79,91 @@ __global__ __launch_bounds__(128) void k_g0_spatial(double2* __restrict__ work, 
         }
       }
     }
-    for (int r = 0; r < 8; ++r) {
-      const bool fz = r & 1, fx = r & 2, fy = r & 4;
-      // a reflection that maps the index onto itself (0 or P/2) adds no point
-      if ((fz && (iz == 0 || iz == hz)) || (fx && (ix == 0 || ix == hxy)) ||
-          (fy && (iy == 0 || iy == hxy)))
-        continue;
-      const int jz = fz ? Pz - iz : iz, jx = fx ? Pxy - ix : ix, jy = fy ? Pxy - iy : iy;
-      const unsigned m = (fz ? kFlipZ : 0u) ^ (fx ? kFlipX : 0u) ^ (fy ? kFlipY : 0u);
-      double2* o = work + (((size_t)jz * Pxy + jx) * Pxy + jy) * kNE;
+    double2* line = work + ((size_t)iz * (hxy + 1) + ix) * Pxy * kNE;
+    double2* o = line + (size_t)iy * kNE;
+#pragma unroll
+    for (int e = 0; e < kNE; ++e) o[e] = acc[e];
+    // the y reflection, unless it maps the index onto itself (0 or Pxy/2)
+    if (iy != 0 && iy != hxy) {
+      double2* om = line + (size_t)(Pxy - iy) * kNE;
 #pragma unroll
       for (int e = 0; e < kNE; ++e) {
-        const bool neg = (__popc(m & entry_pair(e)) & 1) != 0;
-        o[e] = neg ? make_double2(-acc[e].x, -acc[e].y) : acc[e];
+        const bool neg = (__popc(kFlipY & entry_pair(e)) & 1) != 0;
+        om[e] = neg ? make_double2(-acc[e].x, -acc[e].y) : acc[e];
       }
     }
   }
+}
+
+// Transform along N (forward) of data [outer][N][inner] of which only rows
+// 0..N/2 are stored (in: [outer][N/2 + 1][inner]): row N - n is row n with
+// entry e = (inner index mod 45) negated when the reflection FLIP changes its
+// sign.  The first nout frequencies are kept: out [outer][nout][inner].
+constexpr int kFT = 256;  // threads per workgroup
+
+template <int N, int NCOL, unsigned FLIP>
+__global__ __launch_bounds__(kFT) void k_fft_mirror(const double2* __restrict__ in,
+                                                    double2* __restrict__ out, size_t inner,
+                                                    int nout, const double2* __restrict__ tw) {
+  constexpr int LD = lds_ld<NCOL>::value;
+  constexpr int NH = N / 2 + 1;
+  __shared__ double2 s[N * LD];
+  const size_t nchunk = (inner + NCOL - 1) / NCOL;
+  const size_t o = blockIdx.x / nchunk;
+  const size_t i0 = (blockIdx.x - o * nchunk) * NCOL;
+  const size_t base = o * (size_t)NH * inner + i0;
+  const size_t obase = o * (size_t)nout * inner + i0;
+  for (int w = threadIdx.x; w < N * NCOL; w += kFT) {
+    const int r = w / NCOL;
+    const int e = w - r * NCOL;
+    double2 v = make_double2(0.0, 0.0);
+    if (i0 + e < inner) {
+      const int src = r < NH ? r : N - r;
+      v = in[base + (size_t)src * inner + e];
+      if (r >= NH && (__popc(FLIP & entry_pair((int)((i0 + e) % kNE))) & 1))
+        v = make_double2(-v.x, -v.y);
+    }
+    s[r * LD + e] = v;
+  }
+  __syncthreads();
+  lds_fft<double, N, false, NCOL, LD, kFT>(s, tw);
+  for (int w = threadIdx.x; w < nout * NCOL; w += kFT) {
+    const int r = w / NCOL;
+    const int e = w - r * NCOL;
+    if (i0 + e < inner) out[obase + (size_t)r * inner + e] = s[r * LD + e];
+  }
+}
+
+template <int N, unsigned FLIP>
+int launch_fft_mirror(const double2* in, double2* out, int outer, size_t inner, int nout,
+                      const double2* tw, hipStream_t st) {
+  constexpr int NCOL = (N <= 64) ? 16 : ((N * 16 * sizeof(double2) <= 80 * 1024) ? 16 : 8);
+  const size_t nblk = (size_t)outer * ((inner + NCOL - 1) / NCOL);
+  MSC_REQUIRE(nblk < (1ull << 31), "FFT batch too large");
+  hipLaunchKernelGGL((k_fft_mirror<N, NCOL, FLIP>), dim3((unsigned)nblk), dim3(kFT), 0, st, in,
+                     out, inner, nout, tw);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <unsigned FLIP>
+int fft_mirror(const double2* in, double2* out, int outer, int N, size_t inner, int nout,
+               const double2* tw, hipStream_t st) {
+  MSC_REQUIRE(nout >= 1 && nout <= N && in != out, "fft_mirror: bad arguments");
+  switch (N) {
+    case 2: return launch_fft_mirror<2, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 4: return launch_fft_mirror<4, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 8: return launch_fft_mirror<8, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 16: return launch_fft_mirror<16, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 32: return launch_fft_mirror<32, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 64: return launch_fft_mirror<64, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 128: return launch_fft_mirror<128, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 256: return launch_fft_mirror<256, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 512: return launch_fft_mirror<512, FLIP>(in, out, outer, inner, nout, tw, st);
+    default: break;
+  }
+  set_error("fft_mirror: length must be a power of two in [2, 512]");
+  return kUnsupported;
 }
 
 __device__ __forceinline__ void uv_split(int uv, int& u, int& v) {
@@ -303,27 +374,27 @@ __global__ __launch_bounds__(32) void k_nn_delta(const double* __restrict__ va, 
 
 int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st) {
   const size_t P = (size_t)p.Pz * p.Pxy * p.Pxy;
-  const int P2 = p.Pxy, H = p.Pxy / 2 + 1;
-  if (p.work == nullptr) MSC_HIP_CHECK(hipMalloc(&p.work, P * kNE * sizeof(double2)));
+  const int P2 = p.Pxy, H = p.Pxy / 2 + 1, NC = p.NC;  // NC = Pz/2 + 1
+  if (p.work == nullptr)
+    MSC_HIP_CHECK(hipMalloc(&p.work, (size_t)NC * H * P2 * kNE * sizeof(double2)));
   if (p.work2 == nullptr)
-    MSC_HIP_CHECK(hipMalloc(&p.work2, (size_t)p.Pz * P2 * H * kNE * sizeof(double2)));
+    MSC_HIP_CHECK(hipMalloc(&p.work2, (size_t)NC * H * H * kNE * sizeof(double2)));
   double2* work = reinterpret_cast<double2*>(p.work);
   double2* work2 = reinterpret_cast<double2*>(p.work2);
   const double kP = omega / alpha, kS = omega / beta;
   const double pre = 1.0 / (4.0 * M_PI * rho * omega * omega);
-  const size_t octant = (size_t)(p.Pz / 2 + 1) * H * H;
+  const size_t octant = (size_t)NC * H * H;
   hipLaunchKernelGGL(k_g0_spatial, dim3(grid1d(octant, 128)), dim3(128), 0, st, work, p.Pz,
                      p.Pxy, p.Nz, p.M, p.periodic ? 1 : 0, 2.0 * p.a, kP, kS, pre);
   int rc = kOk;
-  // The table keeps k_y <= Pxy/2, k_x <= Pxy/2, k_z <= Pz/2, so each pass keeps
-  // only those outputs:  y: [Pz*Pxy][Pxy][45] -> work2 [Pz*Pxy][H][45];
-  // x: [Pz][Pxy][H*45] -> work [Pz][H][H*45];  z: [1][Pz][H*H*45] -> work2 [NC][H*H*45]
-  rc = fft_axis_pruned<double>(work, work2, p.Pz * P2, P2, kNE, H, false, p.tw128, st);
-  if (!rc)
-    rc = fft_axis_pruned<double>(work2, work, p.Pz, P2, (size_t)H * kNE, H, false, p.tw128, st);
-  if (!rc)
-    rc = fft_axis_pruned<double>(work, work2, 1, p.Pz, (size_t)H * H * kNE, p.NC, false,
-                                 p.tw128, st);
+  // Only the stored half of each axis is transformed or kept (the table holds
+  // k_y, k_x <= Pxy/2 and k_z <= Pz/2):
+  //   y: work [NC*H][Pxy][45] -> work2 [NC*H][H][45]
+  //   x: work2 [NC][H of Pxy rows][H*45] -> work [NC][H][H*45]   (x mirror)
+  //   z: work [1][NC of Pz rows][H*H*45] -> work2 [NC][H*H*45]   (z mirror)
+  rc = fft_axis_pruned<double>(work, work2, NC * H, P2, kNE, H, false, p.tw128, st);
+  if (!rc) rc = fft_mirror<kFlipX>(work2, work, NC, P2, (size_t)H * kNE, H, p.tw128, st);
+  if (!rc) rc = fft_mirror<kFlipZ>(work, work2, 1, p.Pz, (size_t)H * H * kNE, NC, p.tw128, st);
   if (!rc) {
     const size_t tot = (size_t)p.NUV * kNE * p.NC;
     hipLaunchKernelGGL(k_table_compress, dim3(grid1d(tot, 256)), dim3(256), 0, st, work2, H,
