@@ -831,9 +831,15 @@ int msc_build_g0(msc_plan* plan, double omega, double alpha, double beta, double
   MSC_REQUIRE(omega > 0 && alpha > 0 && beta > 0 && rho > 0,
               "omega, alpha, beta and rho must be positive");
   MSC_REQUIRE(!volume_averaged || (n_orders >= 0 && n_orders <= 3), "n_orders must be in 0..3");
+  const bool had_nn = plan->p.have_nn;
   int rc = build_table(plan->p, omega, alpha, beta, rho, S(stream));
-  if (rc) return rc;
-  return set_nn_correction(plan->p, volume_averaged != 0, n_orders, S(stream));
+  if (!rc) rc = set_nn_correction(plan->p, volume_averaged != 0, n_orders, S(stream));
+  // captured Arnoldi steps hold the near-neighbour launch only when it was on
+  if (plan->p.have_nn != had_nn) {
+    plan->k128.drop_graphs();
+    plan->k64.drop_graphs();
+  }
+  return rc;
 }
 
 int msc_set_tmatrix(msc_plan* plan, const double* dlambda, const double* dmu, const double* drho,

@@ -126,7 +126,7 @@ def compute_resonance_tmatrices(omega: float, a: float, ref: ReferenceMedium, Dl
         raise ValueError("Dlambda, Dmu, Drho must be 1-D arrays of equal length")
     tc, _, ratio, _ = _run(omega, a, ref, Dl, Dm, Dr, n_sub, k_hat, wave_type, 0, 0.0, False,
                            device)
-    return (tc if keep_on_device else tc.cpu().numpy()), ratio
+    return (tc if keep_on_device else _lib.to_host(tc)), ratio
 
 
 def compute_resonance_tmatrix(omega: float, a: float, ref: ReferenceMedium,
@@ -164,7 +164,7 @@ def compute_resonance_tmatrix(omega: float, a: float, ref: ReferenceMedium,
         T3x3=T[:3, :3].copy(), ka_cube=ka_cube, ka_sub=ka_sub, n_sub=n_sub, omega=omega, a=a,
         ref=ref, contrast=contrast, condition_number=cond, rayleigh_result=ray_full,
         T_comp_9x9=T, force_monopole=T[:3, :3].copy(), stress_dipole_voigt=T[3:, 3:].copy(),
-        psi_exc=px[0].cpu().numpy(), centres=sub_cell_centres(a, n_sub),
+        psi_exc=_lib.to_host(px[0]), centres=sub_cell_centres(a, n_sub),
         T_loc_9x9=sub_cell_tmatrix_9x9(ray_sub, omega, a_sub), n_iterations_converged=n_conv)
 
 

@@ -14,6 +14,7 @@ Coordinate system: z = 0 (down), x = 1, y = 2.  Voigt order
 from __future__ import annotations
 
 import ctypes
+import threading
 import warnings
 from dataclasses import dataclass, field
 from typing import Callable
@@ -393,7 +394,7 @@ def compute_slab_tmatrices(geometry: SlabGeometry, material: SlabMaterial, omega
     """
     plan = SlabPlan(geometry, device=device)
     plan.set_material(material, omega)
-    return tparams_to_tmatrices(plan.tparams().cpu().numpy())
+    return tparams_to_tmatrices(_lib.to_host(plan.tparams()))
 
 
 def _plane_wave_strain_voigt(k_hat, pol, k_mag) -> NDArray:
@@ -430,12 +431,44 @@ def _build_slab_incident_field(geometry: SlabGeometry, omega: float, ref: Refere
     """Incident field (N_z, M, M, 9) (slab_scattering.py:426-478), built on the device."""
     _incident_vector(omega, ref, k_hat, wave_type)  # validates wave_type first
     plan = SlabPlan(geometry, device=device)
-    return plan.incident_field(omega, ref, k_hat, wave_type).cpu().numpy()
+    return _lib.to_host(plan.incident_field(omega, ref, k_hat, wave_type))
 
 
 # ═══════════════════════════════════════════════════════════════
 #  Solver (slab_scattering.py:486-565)
 # ═══════════════════════════════════════════════════════════════
+
+
+# One plan is kept between solves of the same slab shape: a new plan pays its
+# buffer allocations, the tile autotuning and the HIP-graph captures of its
+# Arnoldi steps again.  A solve takes the plan out of the cache while it runs.
+_PLAN_CACHE: dict = {}
+_PLAN_LOCK = threading.Lock()
+
+
+def _take_plan(geometry, periodic, pad, device):
+    key = (str(device), geometry.M, geometry.N_z, float(geometry.a), bool(periodic),
+           None if pad is None else tuple(int(v) for v in pad))
+    with _PLAN_LOCK:
+        plan = _PLAN_CACHE.pop(key, None)
+    if plan is None:
+        plan = SlabPlan(geometry, periodic=periodic, pad=pad, device=device)
+    else:
+        plan._material_token = None  # its T0 is about to change
+    return plan, key
+
+
+def _give_plan(key, plan) -> None:
+    with _PLAN_LOCK:
+        _PLAN_CACHE.clear()  # a plan holds the slab's workspaces: keep one
+        _PLAN_CACHE[key] = plan
+
+
+def release_plans() -> None:
+    """Drop the plan kept between solves (its device memory is freed once no
+    result refers to it)."""
+    with _PLAN_LOCK:
+        _PLAN_CACHE.clear()
 
 
 def _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, initial_guess,
@@ -444,8 +477,14 @@ def _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, ini
     if initial_guess not in ("born", "zero"):
         raise ValueError(f"Invalid initial_guess '{initial_guess}'. Must be 'born' or 'zero'.")
     _incident_vector(omega, material.ref, k_hat, wave_type)
-    plan = SlabPlan(geometry, periodic=periodic, pad=pad, device=device)
-    plan.build_g0(omega, material.ref, volume_averaged=volume_averaged, n_orders=n_orders)
+    plan, key = _take_plan(geometry, periodic, pad, device)
+    r = material.ref
+    g0_key = (float(omega), float(r.alpha), float(r.beta), float(r.rho), bool(volume_averaged),
+              int(n_orders))
+    if getattr(plan, "_g0_key", None) != g0_key:
+        plan._g0_key = None
+        plan.build_g0(omega, r, volume_averaged=volume_averaged, n_orders=n_orders)
+        plan._g0_key = g0_key
     plan.set_material(material, omega)
     psi0 = plan.incident_field(omega, material.ref, k_hat, wave_type)
     x, st = plan.gmres(psi0, initial_guess, tol=gmres_tol, max_iter=max_iter, restart=restart,
@@ -455,8 +494,8 @@ def _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, ini
                       f"after {st['iterations']} iterations", stacklevel=3)
     g = geometry
     res = SlabResult(
-        psi=x.reshape(g.N_z, g.M, g.M, 9).cpu().numpy(),
-        psi0=psi0.cpu().numpy(),
+        psi=_lib.to_host(x.reshape(g.N_z, g.M, g.M, 9)),
+        psi0=_lib.to_host(psi0),
         geometry=geometry,
         material=material,
         omega=omega,
@@ -467,7 +506,11 @@ def _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, ini
         periodic=periodic,
         stats=st,
     )
-    res._plan = plan  # keeps T0 on the device for the output functions
+    # the output functions use the plan's T0 while no later solve has replaced it
+    token = object()
+    plan._material_token = token
+    res._plan, res._plan_token = plan, token
+    _give_plan(key, plan)
     return res
 
 
@@ -519,6 +562,9 @@ def _source_sums(result: SlabResult, T_local, kP: float, kS: float, r_hat):
         if np.allclose([kP, kS], k2, rtol=1e-14, atol=0.0) and np.allclose(r_hat, rh, rtol=0, atol=1e-14):
             return layer_sum, phased
     plan = getattr(result, "_plan", None)
+    if plan is not None and getattr(plan, "_material_token", None) is not getattr(
+            result, "_plan_token", None):
+        plan = None  # a later solve has reused the plan
     g = result.geometry
     if plan is None or T_local is not None:
         plan = SlabPlan(g)

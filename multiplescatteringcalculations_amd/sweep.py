@@ -124,8 +124,8 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
         if not st["converged"]:
             warnings.warn(f"sweep case {i} did not converge: rel_res={st['rel_residual']:.2e} "
                           f"after {st['iterations']} iterations", stacklevel=2)
-        res = SlabResult(psi=psi if keep_on_device else psi.cpu().numpy(),
-                         psi0=psi0 if keep_on_device else psi0.cpu().numpy(),
+        res = SlabResult(psi=psi if keep_on_device else _lib.to_host(psi),
+                         psi0=psi0 if keep_on_device else _lib.to_host(psi0),
                          geometry=c.geometry, material=c.material, omega=float(c.omega),
                          k_hat=np.asarray(c.k_hat, dtype=float), wave_type=c.wave_type,
                          n_gmres_iter=int(st["iterations"]),

@@ -40,7 +40,7 @@ def to_torch(arr: NDArray[np.complexfloating], device: torch.device, dtype: torc
 
 
 def to_numpy(tensor: torch.Tensor) -> NDArray[np.complexfloating]:
-    return tensor.detach().cpu().to(torch.complex128).numpy()
+    return _lib.to_host(tensor.detach().to(torch.complex128))
 
 
 def _complex_norm(x: torch.Tensor) -> torch.Tensor:

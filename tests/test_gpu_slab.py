@@ -373,3 +373,68 @@ def test_copied_g0_gives_the_same_operator(hip_device, va):
         other.copy_g0_from(src)
     with pytest.raises(RuntimeError, match="no G0"):
         other.copy_g0_from(ss.SlabPlan(ss.SlabGeometry(M=M, N_z=Nz + 1, a=1.0), device=hip_device))
+
+
+def test_reused_plan_solves_and_outputs(hip_device):
+    """Repeated solves of one slab shape reuse a cached plan: each result equals
+    the same solve on a fresh plan, and an earlier result's outputs stay those
+    of its own material after a later solve has reused the plan."""
+    msc, ss = _mods()
+    M, Nz = 8, 3
+    g = msc.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    r = msc.ReferenceMedium(REF_C.alpha, REF_C.beta, REF_C.rho)
+    om, kh = 692.8, np.array([0.893, 0.45, 0.0])
+    mats = [msc.SlabMaterial(*_iid(M, Nz, REF_C, 0.5, 0.3, seed=s), ref=r) for s in (1, 2)]
+    ss._PLAN_CACHE.clear()
+    fresh = []
+    for m in mats:
+        ss._PLAN_CACHE.clear()
+        res = msc.compute_slab_scattering_gpu(g, m, om, kh, "P", gmres_tol=1e-12, restart=20,
+                                              periodic=True)
+        fresh.append((res.psi.copy(), msc.slab_rpp_periodic(res, p=1e-4)))
+    ss._PLAN_CACHE.clear()
+    a = msc.compute_slab_scattering_gpu(g, mats[0], om, kh, "P", gmres_tol=1e-12, restart=20,
+                                        periodic=True)
+    b = msc.compute_slab_scattering_gpu(g, mats[1], om, kh, "P", gmres_tol=1e-12, restart=20,
+                                        periodic=True)
+    assert b._plan is a._plan  # the plan was reused
+    # (a fresh plan tunes its own tiles: agreement to rounding, not bitwise)
+    assert _rel(a.psi, fresh[0][0]) < 1e-12 and _rel(b.psi, fresh[1][0]) < 1e-12
+    assert abs(msc.slab_rpp_periodic(b, p=1e-4) - fresh[1][1]) <= 1e-12 * abs(fresh[1][1])
+    # a's plan now holds b's T0: a's outputs come from its own material
+    assert abs(msc.slab_rpp_periodic(a, p=1e-4) - fresh[0][1]) <= 1e-12 * abs(fresh[0][1])
+    # a different frequency on the cached plan rebuilds G0
+    c = msc.compute_slab_scattering_gpu(g, mats[0], 1.3 * om, kh, "P", gmres_tol=1e-12,
+                                        restart=20, periodic=True)
+    ss._PLAN_CACHE.clear()
+    d = msc.compute_slab_scattering_gpu(g, mats[0], 1.3 * om, kh, "P", gmres_tol=1e-12,
+                                        restart=20, periodic=True)
+    assert _rel(c.psi, d.psi) < 1e-12
+    ss.release_plans()
+    assert not ss._PLAN_CACHE
+
+
+def test_volume_averaged_toggle_on_one_plan(hip_device):
+    """Switching the near-neighbour correction on or off on a plan that has
+    already solved gives the solve of a fresh plan (the captured Arnoldi steps
+    are dropped when the correction switches)."""
+    msc, ss = _mods()
+    M, Nz = 8, 3
+    g = msc.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    r = msc.ReferenceMedium(REF_C.alpha, REF_C.beta, REF_C.rho)
+    mat = msc.SlabMaterial(*_iid(M, Nz, REF_C, 0.5, 0.3, seed=4), ref=r)
+    om = 692.8
+
+    def solve(plan, va):
+        plan.build_g0(om, r, volume_averaged=va, n_orders=2)
+        plan.set_material(mat, om)
+        psi0 = plan.incident_field(om, r, [0.893, 0.45, 0.0], "P")
+        x, st = plan.gmres(psi0, "born", tol=1e-12, max_iter=200, restart=20)
+        assert st["converged"]
+        return x.cpu().numpy()
+
+    shared = ss.SlabPlan(g, device=hip_device)
+    seq = [solve(shared, va) for va in (False, True, False, True)]
+    for va, x in zip((False, True, False, True), seq):
+        assert _rel(x, solve(ss.SlabPlan(g, device=hip_device), va)) < 1e-10
+    assert _rel(seq[1], seq[0]) > 1e-6  # the correction changes the field
