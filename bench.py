@@ -97,7 +97,7 @@ def train_preconditioner(plan, mat0, omega, device):
 
 
 def time_solves(plan, psi0, precision, steps, warmup, precond=None, label=None):
-    """K timed solves (HIP-graph Arnoldi steps), after W untimed ones."""
+    """K timed solves, after W untimed ones."""
     import torch
 
     label = label or precision
@@ -121,8 +121,7 @@ def time_solves(plan, psi0, precision, steps, warmup, precond=None, label=None):
 
 def time_pass3(plan, psi0, precision, steps):
     """Per-launch duration of the block-multiply pass from HIP events recorded
-    around each launch on its stream, over `steps` solves (event timing runs
-    the Arnoldi steps as individual launches instead of graphs)."""
+    around each launch on its stream, over `steps` solves."""
     import torch
 
     plan.timing(True)

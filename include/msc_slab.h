@@ -170,8 +170,9 @@ int msc_plan_reshape(msc_plan* plan, int M, int N_z, double a);
 int msc_plan_copy_g0(msc_plan* dst, const msc_plan* src, void* stream);
 
 /*
- * Run each Arnoldi step of msc_gmres as one instantiated HIP graph (enable = 1,
- * the default) or as individual launches (0).  Graph capture must not overlap
+ * Run each Arnoldi step of msc_gmres as one instantiated HIP graph (enable = 1)
+ * or as individual launches (0, the default: as fast on MI355X, with no capture
+ * on a plan's first solve).  Graph capture must not overlap
  * another host thread's allocation or synchronous copy calls, so callers that
  * drive several plans from several threads turn graphs off (msc_sweep does).
  * No reference counterpart.

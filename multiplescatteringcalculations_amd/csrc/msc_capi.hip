@@ -167,7 +167,7 @@ struct msc_plan {
   float2* b64 = nullptr;    // right-hand side narrowed to complex64
   float2* r64 = nullptr;    // complex64 residual
   hipStream_t cap = nullptr;  // private stream for HIP graph capture
-  bool use_graphs = true;
+  bool use_graphs = false;  // direct launches measured as fast (msc_plan_use_graphs)
   double* scal = nullptr;     // device scalars of msc_net_input
 };
 
@@ -293,10 +293,13 @@ int check_opts(const msc_gmres_opts* o) {
 
 namespace {
 
-// Launch the work of Arnoldi step j (operator on B = G0 T0 and the
-// orthogonalisation) on stream s.
-// Flexible GMRES (a learned preconditioner is set): z_j = P(v_j) (v_j has unit
-// norm, so the network's input scale is sqrt(n)), then V[j+1] = A z_j.
+// Launch the work of Arnoldi step j (operator and orthogonalisation) on stream
+// s.  The basis is built from w with A (step vector) = v_j - w, so that w does
+// not carry the identity part of A = I - B along v_j (whose removal by the
+// orthogonalisation cancels and asks for a second Gram-Schmidt pass):
+//   plain GMRES:    w = B v_j
+//   flexible GMRES: z_j = P(v_j) (v_j has unit norm, so the network's input
+//                   scale is sqrt(n)), w = v_j - A z_j (the fused residual).
 template <typename R>
 int step_launches(msc_plan* pl, Krylov& k, int j, int orth, hipStream_t s) {
   cx<R>* V = reinterpret_cast<cx<R>*>(k.V);
@@ -306,7 +309,7 @@ int step_launches(msc_plan* pl, Krylov& k, int j, int orth, hipStream_t s) {
   if (pl->net != nullptr) {
     cx<R>* zj = reinterpret_cast<cx<R>*>(k.Z) + (size_t)j * k.ldv;
     rc = net_apply<R>(*pl->net, pl->p, vj, zj, std::sqrt((double)pl->p.n), s, &k.st->done);
-    if (!rc) rc = matvec<R>(pl->p, zj, w, s, &k.st->done);
+    if (!rc) rc = residual<R>(pl->p, vj, zj, w, s, &k.st->done);
   } else {
     rc = scattered<R>(pl->p, vj, w, s, &k.st->done);
   }
@@ -429,14 +432,13 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
     ~NetScope() { pl->net = nullptr; }
   } scope{pl};
   pl->net = net;
-  // plain GMRES builds its basis from B = G0 T0 (A v = v - B v); flexible GMRES
-  // needs A z_j itself
-  k.shifted = net == nullptr;
+  // both forms build the basis from w with A (step vector) = v_j - w
+  k.shifted = true;
   bool mixed = o.mode == MSC_SOLVE_MIXED;
   if (mixed) {
     rc = pl->k64.ensure(n, m, sizeof(float2));
     if (rc) return rc;
-    pl->k64.shifted = net == nullptr;
+    pl->k64.shifted = true;
     if (net != nullptr) {
       rc = pl->k64.ensure_z();
       if (rc) return rc;

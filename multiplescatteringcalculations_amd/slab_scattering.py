@@ -440,8 +440,7 @@ def _build_slab_incident_field(geometry: SlabGeometry, omega: float, ref: Refere
 
 
 # One plan is kept between solves of the same slab shape: a new plan pays its
-# buffer allocations, the tile autotuning and the HIP-graph captures of its
-# Arnoldi steps again.  A solve takes the plan out of the cache while it runs.
+# buffer allocations and the tile autotuning again.  A solve takes the plan out of the cache while it runs.
 _PLAN_CACHE: dict = {}
 _PLAN_LOCK = threading.Lock()
 

@@ -325,8 +325,9 @@ def test_every_tile_variant_matches_oracle(hip_device, tiles):
 
 
 def test_autotune_and_graph_solves_are_reproducible(hip_device):
-    """Autotuned tiles stay valid, and repeated graph-replayed solves give
-    bit-identical results (deterministic reductions, replay-safe graphs)."""
+    """Autotuned tiles stay valid, and repeated solves give bit-identical
+    results with direct launches and with graph-replayed steps (deterministic
+    reductions, replay-safe graphs)."""
     msc, ss = _mods()
     M, Nz = 16, 6
     Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=5)
@@ -343,6 +344,15 @@ def test_autotune_and_graph_solves_are_reproducible(hip_device):
         outs.append(x.cpu().numpy())
     assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
     assert _rel(outs[1], outs[0]) < 1e-6
+    # the same steps as instantiated HIP graphs (captured on the first solve,
+    # replayed on the second) give bit-identical solves
+    from multiplescatteringcalculations_amd import _lib
+
+    _lib.check(plan.lib.msc_plan_use_graphs(plan._h, 1))
+    for prec, ref_x in (("fp64", outs[0]), ("mixed", outs[1]), ("fp64", outs[0]), ("mixed", outs[1])):
+        x, st = plan.gmres(psi0, "born", tol=1e-10, max_iter=200, restart=30, precision=prec)
+        assert np.array_equal(x.cpu().numpy(), ref_x)
+    _lib.check(plan.lib.msc_plan_use_graphs(plan._h, 0))
 
 
 @pytest.mark.parametrize("va", [False, True])
