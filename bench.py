@@ -242,7 +242,7 @@ def time_sphere(n_sub: int = 32):
     res = msc.compute_sphere_foldy_lax_fft_gpu(om, R, ref, c, n_sub, gmres_tol=TOL, restart=30)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    out = {"n_sub": n_sub, "cells": int(res.n_cells), "seconds_9_rhs": round(dt, 3)}
+    out = {"n_sub": n_sub, "cells": int(res.n_cells), "seconds_9_rhs": round(dt, 4)}
     log(f"sphere: {out}")
     return out
 
