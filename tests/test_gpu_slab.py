@@ -108,6 +108,9 @@ def test_incident_matches_oracle(hip_device, wave, khat):
 
 
 @pytest.mark.parametrize("M,Nz,contrast,pad", [
+    (1, 1, "iid", None),  # FFT grid 2 x 2 x 2 (table build's degenerate octant)
+    (1, 3, "iid", None),
+    (4, 1, "iid", None),
     (2, 2, "uniform", None),
     (3, 2, "iid", None),
     (3, 2, "iid", (8, 16)),
@@ -139,10 +142,14 @@ def test_matvec_matches_oracle(hip_device, M, Nz, contrast, pad):
         assert _rel(x - y, x - y_dir) < 1e-11
     # complex64 operator
     y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
-    assert _rel(x - y64, x - y_ref) < 2e-5
+    # relative to the scattered part, plus the input's complex64 rounding, which
+    # enters x - y64 directly (and is all of it where a single voxel has no
+    # coupling)
+    tol = 2e-5 * np.abs(x - y_ref).max() + 1e-7 * np.abs(x).max()
+    assert np.abs((x - y64) - (x - y_ref)).max() < tol
 
 
-@pytest.mark.parametrize("M,Nz", [(4, 2), (8, 3)])
+@pytest.mark.parametrize("M,Nz", [(2, 1), (2, 2), (4, 2), (8, 3)])
 def test_matvec_periodic_matches_oracle(hip_device, M, Nz):
     ref, om = REF_C, 692.8
     Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=7)
@@ -321,7 +328,11 @@ def test_every_tile_variant_matches_oracle(hip_device, tiles):
     y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
     assert _rel(x - y, x - y_ref) < 1e-11
     y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
-    assert _rel(x - y64, x - y_ref) < 2e-5
+    # relative to the scattered part, plus the input's complex64 rounding, which
+    # enters x - y64 directly (and is all of it where a single voxel has no
+    # coupling)
+    tol = 2e-5 * np.abs(x - y_ref).max() + 1e-7 * np.abs(x).max()
+    assert np.abs((x - y64) - (x - y_ref)).max() < tol
 
 
 def test_autotune_and_graph_solves_are_reproducible(hip_device):
