@@ -885,7 +885,8 @@ __global__ __launch_bounds__(kNT) void k_fft_axis(const cx<R>* __restrict__ in,
 template <typename R, int N>
 static int launch_fft_axis(const cx<R>* in, cx<R>* out, int outer, size_t inner, int nout,
                            bool inv, const cx<R>* tw, hipStream_t st) {
-  constexpr int NCOL = (N <= 64) ? 16 : ((N * 16 * sizeof(cx<R>) <= 80 * 1024) ? 16 : 8);
+  // 8 columns above 64 points: 4-5 workgroups per CU instead of 2 (LDS)
+  constexpr int NCOL = (N <= 64) ? 16 : (N * 8 * sizeof(cx<R>) <= 80 * 1024 ? 8 : 4);
   const size_t nchunk = (inner + NCOL - 1) / NCOL;
   const size_t nblk = (size_t)outer * nchunk;
   MSC_REQUIRE(nblk < (1ull << 31), "FFT batch too large");

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kFT) void k_fft_mirror(const double2* __restrict__ 
 template <int N, unsigned FLIP>
 int launch_fft_mirror(const double2* in, double2* out, int outer, size_t inner, int nout,
                       const double2* tw, hipStream_t st) {
-  constexpr int NCOL = (N <= 64) ? 16 : ((N * 16 * sizeof(double2) <= 80 * 1024) ? 16 : 8);
+  constexpr int NCOL = (N <= 64) ? 16 : (N * 8 * sizeof(double2) <= 80 * 1024 ? 8 : 4);
   const size_t nblk = (size_t)outer * ((inner + NCOL - 1) / NCOL);
   MSC_REQUIRE(nblk < (1ull << 31), "FFT batch too large");
   hipLaunchKernelGGL((k_fft_mirror<N, NCOL, FLIP>), dim3((unsigned)nblk), dim3(kFT), 0, st, in,
