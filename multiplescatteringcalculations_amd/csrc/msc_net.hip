@@ -344,24 +344,31 @@ __global__ __launch_bounds__(kNetNT) void k_conv_wgrad(const uint16_t* __restric
   v4f acc[TPW];
 #pragma unroll
   for (int q = 0; q < TPW; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
+  // staging moves 8 channels (16 bytes) per load and transposes them into LDS
+  static_assert(NI % 8 == 0 && NO % 8 == 0, "channel counts must be multiples of 8");
+  typedef short v8s __attribute__((ext_vector_type(8)));
   for (size_t c0 = vbeg; c0 < vend; c0 += kWgVox) {
-    for (int e = threadIdx.x; e < kWgVox * NO; e += kNetNT) {
-      const int v = e / NO, c = e - v * NO;
+    for (int e = threadIdx.x; e < kWgVox * (NO / 8); e += kNetNT) {
+      const int v = e / (NO / 8), c = (e - v * (NO / 8)) * 8;
       const size_t gv = c0 + v;
-      sg[c * kWgLd + v] = gv < vend ? g[gv * NO + c] : (uint16_t)0;
+      v8s val = v8s{0, 0, 0, 0, 0, 0, 0, 0};
+      if (gv < vend) val = *reinterpret_cast<const v8s*>(g + gv * NO + c);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sg[(c + k) * kWgLd + v] = (uint16_t)val[k];
     }
-    for (int e = threadIdx.x; e < kWgVox * NI; e += kNetNT) {
-      const int v = e / NI, c = e - v * NI;
+    for (int e = threadIdx.x; e < kWgVox * (NI / 8); e += kNetNT) {
+      const int v = e / (NI / 8), c = (e - v * (NI / 8)) * 8;
       const size_t gv = c0 + v;
-      uint16_t val = 0;
+      v8s val = v8s{0, 0, 0, 0, 0, 0, 0, 0};
       if (gv < vend) {
         const int gw = (int)(gv % W);
         const size_t q = gv / W;
         const int nh = (int)(q % H) + dx, nd = (int)(q / H) + dz, nw = gw + dy;
         if (nd >= 0 && nd < D && nh >= 0 && nh < H && nw >= 0 && nw < W)
-          val = in[(((size_t)nd * H + nh) * W + nw) * NI + c];
+          val = *reinterpret_cast<const v8s*>(in + (((size_t)nd * H + nh) * W + nw) * NI + c);
       }
-      sx[c * kWgLd + v] = val;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sx[(c + k) * kWgLd + v] = (uint16_t)val[k];
     }
     __syncthreads();
 #pragma unroll
