@@ -407,13 +407,29 @@ template <int NO>
 __global__ __launch_bounds__(kNetNT) void k_bias_grad(const uint16_t* __restrict__ g,
                                                       float* __restrict__ db, int cout,
                                                       size_t nvox) {
+  // Each thread sums one 8-channel group (16-byte loads) in registers: the
+  // thread stride S is a multiple of NO/8, so a thread's group never changes.
+  static_assert(NO % 8 == 0, "channel count must be a multiple of 8");
+  constexpr int G8 = NO / 8;
+  typedef short v8s __attribute__((ext_vector_type(8)));
   __shared__ float s[NO];
   for (int c = threadIdx.x; c < NO; c += kNetNT) s[c] = 0.f;
   __syncthreads();
-  const size_t tot = nvox * NO;
-  for (size_t e = (size_t)blockIdx.x * kNetNT + threadIdx.x; e < tot;
-       e += (size_t)gridDim.x * kNetNT)
-    atomicAdd(&s[e % NO], bf2f(g[e]));
+  const size_t T = (size_t)gridDim.x * kNetNT;
+  const size_t S = T - T % G8;
+  const size_t tid = (size_t)blockIdx.x * kNetNT + threadIdx.x;
+  if (tid < S) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const size_t tot = nvox * G8;
+    for (size_t e = tid; e < tot; e += S) {
+      const v8s val = *reinterpret_cast<const v8s*>(g + e * 8);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += bf2f((uint16_t)val[k]);
+    }
+    const int c0 = (int)(tid % G8) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) atomicAdd(&s[c0 + k], acc[k]);
+  }
   __syncthreads();
   for (int c = threadIdx.x; c < cout; c += kNetNT) atomicAdd(&db[c], s[c]);
 }
