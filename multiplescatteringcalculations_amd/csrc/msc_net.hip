@@ -403,35 +403,42 @@ __global__ __launch_bounds__(kNetNT) void k_conv_wgrad(const uint16_t* __restric
 }
 
 // db[o] += sum_v g[v][o]
+constexpr int kBiasGrid = 256;  // blocks: few global atomics per channel
 template <int NO>
 __global__ __launch_bounds__(kNetNT) void k_bias_grad(const uint16_t* __restrict__ g,
                                                       float* __restrict__ db, int cout,
                                                       size_t nvox) {
   // Each thread sums one 8-channel group (16-byte loads) in registers: the
   // thread stride S is a multiple of NO/8, so a thread's group never changes.
+  // The block's sums meet in LDS (no atomics there); one global atomic per
+  // channel and block.
   static_assert(NO % 8 == 0, "channel count must be a multiple of 8");
   constexpr int G8 = NO / 8;
   typedef short v8s __attribute__((ext_vector_type(8)));
-  __shared__ float s[NO];
-  for (int c = threadIdx.x; c < NO; c += kNetNT) s[c] = 0.f;
-  __syncthreads();
+  __shared__ float red[kNetNT][9];
   const size_t T = (size_t)gridDim.x * kNetNT;
   const size_t S = T - T % G8;
   const size_t tid = (size_t)blockIdx.x * kNetNT + threadIdx.x;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (tid < S) {
-    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const size_t tot = nvox * G8;
     for (size_t e = tid; e < tot; e += S) {
       const v8s val = *reinterpret_cast<const v8s*>(g + e * 8);
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc[k] += bf2f((uint16_t)val[k]);
     }
-    const int c0 = (int)(tid % G8) * 8;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) atomicAdd(&s[c0 + k], acc[k]);
   }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[threadIdx.x][k] = acc[k];
   __syncthreads();
-  for (int c = threadIdx.x; c < cout; c += kNetNT) atomicAdd(&db[c], s[c]);
+  // thread t's group is (blockIdx.x * kNetNT + t) % G8
+  const int gb = (int)(((size_t)blockIdx.x * kNetNT) % G8);
+  for (int c = threadIdx.x; c < cout; c += kNetNT) {
+    const int grp = c / 8, k = c - grp * 8;
+    float sum = 0.f;
+    for (int t = (grp - gb + G8) % G8; t < kNetNT; t += G8) sum += red[t][k];
+    atomicAdd(&db[c], sum);
+  }
 }
 
 // Mean-square loss of the first kNetIO output channels against target
@@ -801,7 +808,7 @@ struct NetOps {
     // layer 4 (HID -> IO)
     hipLaunchKernelGGL((k_conv_wgrad<HID, kNetIOPad>), wg, dim3(kNetNT), 0, s, net.act[3], g4,
                        Gr + net.woff[3], net.cin[3], net.cout[3], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<kNetIOPad>), dim3(gb), dim3(kNetNT), 0, s, g4,
+    hipLaunchKernelGGL((k_bias_grad<kNetIOPad>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g4,
                        Gr + net.boff[3], net.cout[3], nv);
     hipLaunchKernelGGL((k_conv<kNetIOPad, HID, kEpiBackward, true, kIOStaged>), G, dim3(kNetNT), 0,
                        s,
@@ -810,7 +817,7 @@ struct NetOps {
     // layer 3
     hipLaunchKernelGGL((k_conv_wgrad<HID, HID>), wg, dim3(kNetNT), 0, s, net.act[2], g3,
                        Gr + net.woff[2], net.cin[2], net.cout[2], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g3, Gr + net.boff[2],
+    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g3, Gr + net.boff[2],
                        net.cout[2], nv);
     hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), G, dim3(kNetNT), 0, s,
                        conv_args(net, g3, net.wtr + net.wpad_off[2], nullptr, net.act[2], g2,
@@ -818,7 +825,7 @@ struct NetOps {
     // layer 2
     hipLaunchKernelGGL((k_conv_wgrad<HID, HID>), wg, dim3(kNetNT), 0, s, net.act[1], g2,
                        Gr + net.woff[1], net.cin[1], net.cout[1], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g2, Gr + net.boff[1],
+    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g2, Gr + net.boff[1],
                        net.cout[1], nv);
     hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), G, dim3(kNetNT), 0, s,
                        conv_args(net, g2, net.wtr + net.wpad_off[1], nullptr, net.act[1], g1,
@@ -826,7 +833,7 @@ struct NetOps {
     // layer 1 (IO -> HID): weights and bias only
     hipLaunchKernelGGL((k_conv_wgrad<kNetIOPad, HID>), wg, dim3(kNetNT), 0, s, in, g1,
                        Gr + net.woff[0], net.cin[0], net.cout[0], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(gb), dim3(kNetNT), 0, s, g1, Gr + net.boff[0],
+    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g1, Gr + net.boff[0],
                        net.cout[0], nv);
   }
 };
