@@ -297,6 +297,15 @@ def test_sphere_config_validated_and_cpu_backend_refused(cfg_file):
         load_config(cfg_file(text.replace("{radius: 10.0, n_sub: 4}", "{n_sub: 4}")))
 
 
+def test_copy_g0_validates_arguments():
+    """msc_plan_copy_g0 rejects NULL plans before any device work."""
+    from multiplescatteringcalculations_amd import _lib
+
+    lib = _lib.load()
+    assert lib.msc_plan_copy_g0(None, None, None) == _lib.MSC_ERR_INVALID
+    assert b"NULL" in lib.msc_last_error()
+
+
 def test_sweep_and_reshape_validate_arguments():
     """msc_sweep and msc_plan_reshape reject bad arguments before any device work."""
     from multiplescatteringcalculations_amd import _lib
