@@ -13,6 +13,7 @@ Coordinate system: z = 0 (down), x = 1, y = 2.  Voigt order
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 import threading
 import warnings
@@ -468,6 +469,10 @@ def release_plans() -> None:
     result refers to it)."""
     with _PLAN_LOCK:
         _PLAN_CACHE.clear()
+
+
+# free the cached plan while the HIP runtime is still up
+atexit.register(release_plans)
 
 
 def _solve(geometry, material, omega, k_hat, wave_type, gmres_tol, max_iter, initial_guess,
