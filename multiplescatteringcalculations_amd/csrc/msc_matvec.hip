@@ -140,6 +140,16 @@ __device__ __forceinline__ void kmul45_stream(const cx<R>* __restrict__ kp, int 
   }
 }
 
+// Dispatch slot b runs on XCD b % 8: give each XCD a contiguous run of
+// logical blocks, so the column segments that neighbouring blocks of the x
+// passes share (rows of 9Q complex, not whole cache lines) sit in one L2
+// (matvec 0.384-0.406 -> 0.365 ms in complex64, 0.699 -> 0.654 ms in
+// complex128 at 128x128x32).
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  if (nblk % 8 != 0) return b;
+  return (b % 8) * (nblk / 8) + b / 8;
+}
+
 // ── pass 1: T0 gather + zero pad + forward FFT along y ─────────────────────
 template <typename R, int P, int B>
 __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
@@ -222,8 +232,9 @@ __global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
   constexpr int nck = P / Q;
-  const int z = blockIdx.x / nck;
-  const int ky0 = (blockIdx.x - z * nck) * Q;
+  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int z = lb / nck;
+  const int ky0 = (lb - z * nck) * Q;
   staged<cx<R>>(
       P * NTR,
       [&](int w) {
@@ -320,8 +331,9 @@ __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
   constexpr int nck = P / Q;
-  const int z = blockIdx.x / nck;
-  const int ky0 = (blockIdx.x - z * nck) * Q;
+  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int z = lb / nck;
+  const int ky0 = (lb - z * nck) * Q;
   staged<cx<R>>(
       P * NTR,
       [&](int w) {
