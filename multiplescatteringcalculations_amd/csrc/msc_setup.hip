@@ -26,11 +26,10 @@ namespace {
 // d mod P.  Aperiodic: |dz| < Nz, |dx|,|dy| < M.  Periodic: Pxy = M and the
 // lateral offsets dx in {j, j - M} (|dx| < M) are summed (fold without Bloch
 // phase, slab_scattering.py:337-346).  The self term is zero.  Each thread
-// evaluates one point of the octant 0 <= iz <= Pz/2, 0 <= ix, iy <= Pxy/2 and
-// writes it and its y reflection into work[iz][ix][Pxy][45]: the z and x
-// reflections are not stored (the FFT passes along x and z read them from the
-// stored half, k_fft_mirror).  Entry (i, j) of the block changes sign under a
-// reflection when exactly one of components i, j flips.
+// evaluates one point of the octant 0 <= iz <= Pz/2, 0 <= ix, iy <= Pxy/2 into
+// work[iz][ix][iy][45]: the reflections are not stored (the FFT passes read
+// them from the stored half, k_fft_mirror).  Entry (i, j) of the block changes
+// sign under a reflection when exactly one of components i, j flips.
 __device__ __forceinline__ unsigned entry_pair(int e) {
   // component pair (i, j) of entry e in add_prop45's layout, as a bit mask
   if (e < 6) {
@@ -79,19 +78,9 @@ __global__ __launch_bounds__(128) void k_g0_spatial(double2* __restrict__ work, 
         }
       }
     }
-    double2* line = work + ((size_t)iz * (hxy + 1) + ix) * Pxy * kNE;
-    double2* o = line + (size_t)iy * kNE;
+    double2* o = work + idx * kNE;
 #pragma unroll
     for (int e = 0; e < kNE; ++e) o[e] = acc[e];
-    // the y reflection, unless it maps the index onto itself (0 or Pxy/2)
-    if (iy != 0 && iy != hxy) {
-      double2* om = line + (size_t)(Pxy - iy) * kNE;
-#pragma unroll
-      for (int e = 0; e < kNE; ++e) {
-        const bool neg = (__popc(kFlipY & entry_pair(e)) & 1) != 0;
-        om[e] = neg ? make_double2(-acc[e].x, -acc[e].y) : acc[e];
-      }
-    }
   }
 }
 
@@ -376,7 +365,7 @@ int build_table(Plan& p, double omega, double alpha, double beta, double rho, hi
   const size_t P = (size_t)p.Pz * p.Pxy * p.Pxy;
   const int P2 = p.Pxy, H = p.Pxy / 2 + 1, NC = p.NC;  // NC = Pz/2 + 1
   if (p.work == nullptr)
-    MSC_HIP_CHECK(hipMalloc(&p.work, (size_t)NC * H * P2 * kNE * sizeof(double2)));
+    MSC_HIP_CHECK(hipMalloc(&p.work, (size_t)NC * H * H * kNE * sizeof(double2)));
   if (p.work2 == nullptr)
     MSC_HIP_CHECK(hipMalloc(&p.work2, (size_t)NC * H * H * kNE * sizeof(double2)));
   double2* work = reinterpret_cast<double2*>(p.work);
@@ -389,10 +378,10 @@ int build_table(Plan& p, double omega, double alpha, double beta, double rho, hi
   int rc = kOk;
   // Only the stored half of each axis is transformed or kept (the table holds
   // k_y, k_x <= Pxy/2 and k_z <= Pz/2):
-  //   y: work [NC*H][Pxy][45] -> work2 [NC*H][H][45]
+  //   y: work [NC*H][H of Pxy rows][45] -> work2 [NC*H][H][45]   (y mirror)
   //   x: work2 [NC][H of Pxy rows][H*45] -> work [NC][H][H*45]   (x mirror)
   //   z: work [1][NC of Pz rows][H*H*45] -> work2 [NC][H*H*45]   (z mirror)
-  rc = fft_axis_pruned<double>(work, work2, NC * H, P2, kNE, H, false, p.tw128, st);
+  rc = fft_mirror<kFlipY>(work, work2, NC * H, P2, kNE, H, p.tw128, st);
   if (!rc) rc = fft_mirror<kFlipX>(work2, work, NC, P2, (size_t)H * kNE, H, p.tw128, st);
   if (!rc) rc = fft_mirror<kFlipZ>(work, work2, 1, p.Pz, (size_t)H * H * kNE, NC, p.tw128, st);
   if (!rc) {
