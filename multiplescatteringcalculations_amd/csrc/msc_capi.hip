@@ -3,10 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
@@ -183,6 +185,30 @@ hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 bool autotune_enabled() {
   const char* v = std::getenv("MSC_AUTOTUNE");
   return v == nullptr || std::strcmp(v, "0") != 0;
+}
+
+// Tiles tuned once per (device, FFT grid, precision) in the process: every
+// later plan on that grid takes them (the kernels take any slab size on the
+// grid), so a sweep's workers and new plans skip the timing runs.
+std::mutex g_tile_mu;
+std::map<std::array<int, 4>, std::array<int, 5>> g_tiles;
+
+std::array<int, 4> tile_key(const Plan& p, int pr) { return {p.device, p.Pz, p.Pxy, pr}; }
+
+bool lookup_tiles(Plan& p, int pr) {
+  std::lock_guard<std::mutex> lock(g_tile_mu);
+  const auto it = g_tiles.find(tile_key(p, pr));
+  if (it == g_tiles.end()) return false;
+  for (int i = 0; i < 5; ++i) p.tile[pr][i] = it->second[i];
+  p.tuned[pr] = true;
+  return true;
+}
+
+void store_tiles(const Plan& p, int pr) {
+  std::lock_guard<std::mutex> lock(g_tile_mu);
+  std::array<int, 5> t{};
+  for (int i = 0; i < 5; ++i) t[i] = p.tile[pr][i];
+  g_tiles[tile_key(p, pr)] = t;
 }
 
 int alloc_plan_buffers(Plan& p) {
@@ -451,14 +477,20 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
   // tile autotuning once per plan and precision (MSC_AUTOTUNE=0 keeps the defaults)
   if (autotune_enabled()) {
     if (!p.tuned[0]) {
-      rc = autotune<double>(p, b, r, st);
-      if (rc) return rc;
+      if (!lookup_tiles(p, 0)) {
+        rc = autotune<double>(p, b, r, st);
+        if (rc) return rc;
+        store_tiles(p, 0);
+      }
       k.drop_graphs();
     }
     if (mixed && !p.tuned[1]) {
-      rc = krylov_narrow(b, pl->b64, n, st);
-      if (!rc) rc = autotune<float>(p, pl->b64, pl->r64, st);
-      if (rc) return rc;
+      if (!lookup_tiles(p, 1)) {
+        rc = krylov_narrow(b, pl->b64, n, st);
+        if (!rc) rc = autotune<float>(p, pl->b64, pl->r64, st);
+        if (rc) return rc;
+        store_tiles(p, 1);
+      }
       pl->k64.drop_graphs();
     }
   }
@@ -784,6 +816,7 @@ int msc_plan_autotune(msc_plan* plan, int precision, void* stream) {
   (void)hipFree(ob);
   plan->k128.drop_graphs();
   plan->k64.drop_graphs();
+  if (!rc) store_tiles(plan->p, precision == MSC_C128 ? 0 : 1);
   return rc;
 }
 
