@@ -89,56 +89,59 @@ struct Halo {
   static constexpr int ELEMS = 3 * ROWS * COLS * LDV;
 };
 
-// One output plane d of a workgroup's tile (rows h0..h0+3, voxels w0..w0+63)
-// from the three staged input planes d-1, d, d+1, which sit in LDS plane
-// slots (sbase + 0, 1, 2) mod 3 of sh.  MFMA over the 27 taps, then the
-// epilogue EPI.
-template <int NI, int NO, int EPI, bool MIRROR, int NS>
-__device__ __forceinline__ void conv_plane(const ConvArgs& a, const uint16_t* sh, int sbase,
+// Weight fragments of one tap: lane l holds W[tap][o*16 + l%16][ib*16 + 4(l/16) ..].
+template <int NI, int NO>
+__device__ __forceinline__ void load_wf(v4s (&wf)[NO / 16][NI / 16], const uint16_t* w, int tap) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 0; o < NO / 16; ++o)
+#pragma unroll
+    for (int ib = 0; ib < NI / 16; ++ib)
+      wf[o][ib] = *reinterpret_cast<const v4s*>(
+          w + ((size_t)tap * NO + o * 16 + (lane & 15)) * NI + ib * 16 + (lane >> 4) * 4);
+}
+
+// The lane's first B-fragment element for tap offset (dz, dx, dy) in the
+// staged halo whose plane d sits in slot (sbase + 1) mod 3.
+template <int NS>
+__device__ __forceinline__ const uint16_t* tap_base(const uint16_t* sh, int sbase, int dz, int dx,
+                                                    int dy) {
+  constexpr int LDV = Halo<NS>::LDV, COLS = Halo<NS>::COLS;
+  constexpr int PLANE = Halo<NS>::ROWS * COLS * LDV;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  return sh + ((sbase + 1 + dz) % 3) * PLANE +
+         ((1 + wave + dx) * COLS + (1 + dy + (lane & 15))) * LDV + (lane >> 4) * 4;
+}
+
+// The MFMAs of one tap over the wave's kVT voxel tiles.
+template <int NI, int NO, int NS>
+__device__ __forceinline__ void tap_mfma(v4f (&acc)[NO / 16][kVT],
+                                         const v4s (&wf)[NO / 16][NI / 16],
+                                         const uint16_t* base) {
+  constexpr int LDV = Halo<NS>::LDV;
+  const int grp = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int t = 0; t < kVT; ++t) {
+#pragma unroll
+    for (int ib = 0; ib < NI / 16; ++ib) {
+      v4s bf = v4s{0, 0, 0, 0};
+      if (ib * 16 + 16 <= NS || ib * 16 + grp * 4 < NS)
+        bf = *reinterpret_cast<const v4s*>(base + t * 16 * LDV + ib * 16);
+#pragma unroll
+      for (int o = 0; o < NO / 16; ++o)
+        acc[o][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[o][ib], bf, acc[o][t], 0, 0, 0);
+    }
+  }
+}
+
+// Epilogue EPI of one output plane d: rows h0 + wave, voxels w0 .. w0 + 63.
+template <int NO, int EPI>
+__device__ __forceinline__ void conv_store(const ConvArgs& a, const v4f (&acc)[NO / 16][kVT],
                                            int d, int h0, int w0) {
-  constexpr int NOT = NO / 16, NIB = NI / 16, LDV = Halo<NS>::LDV;
-  constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
-  constexpr int PLANE = ROWS * COLS * LDV;
+  constexpr int NOT = NO / 16;
   const int H = a.H, W = a.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = lane & 15, grp = lane >> 4;
-  v4f acc[NOT][kVT];
-#pragma unroll
-  for (int o = 0; o < NOT; ++o)
-#pragma unroll
-    for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
-  // one z offset per unrolled group: a full unroll holds every tap's weight
-  // fragments in registers across the plane loop of k_conv_zm
-#pragma unroll 9
-  for (int tap = 0; tap < 27; ++tap) {
-    int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
-    if (MIRROR) {
-      dz = -dz;
-      dx = -dx;
-      dy = -dy;
-    }
-    v4s wf[NOT][NIB];
-#pragma unroll
-    for (int o = 0; o < NOT; ++o)
-#pragma unroll
-      for (int ib = 0; ib < NIB; ++ib)
-        wf[o][ib] = *reinterpret_cast<const v4s*>(a.w + ((size_t)tap * NO + o * 16 + col) * NI +
-                                                  ib * 16 + grp * 4);
-    const uint16_t* base = sh + ((sbase + 1 + dz) % 3) * PLANE +
-                           ((1 + wave + dx) * COLS + (1 + dy + col)) * LDV + grp * 4;
-#pragma unroll
-    for (int t = 0; t < kVT; ++t) {
-#pragma unroll
-      for (int ib = 0; ib < NIB; ++ib) {
-        v4s bf = v4s{0, 0, 0, 0};
-        if (ib * 16 + 16 <= NS || ib * 16 + grp * 4 < NS)
-          bf = *reinterpret_cast<const v4s*>(base + t * 16 * LDV + ib * 16);
-#pragma unroll
-        for (int o = 0; o < NOT; ++o)
-          acc[o][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[o][ib], bf, acc[o][t], 0, 0, 0);
-      }
-    }
-  }
   const int h = h0 + wave;
   if (h >= H) return;
   float inv_s = 1.f;
@@ -195,6 +198,52 @@ __device__ __forceinline__ void conv_plane(const ConvArgs& a, const uint16_t* sh
       }
     }
   }
+}
+
+// One output plane d of a workgroup's tile (rows h0..h0+3, voxels w0..w0+63)
+// from the three staged input planes d-1, d, d+1, which sit in LDS plane
+// slots (sbase + 0, 1, 2) mod 3 of sh.  MFMA over the 27 taps, then the
+// epilogue EPI.  The weight fragments are loaded per tap.
+template <int NI, int NO, int EPI, bool MIRROR, int NS>
+__device__ __forceinline__ void conv_plane(const ConvArgs& a, const uint16_t* sh, int sbase,
+                                           int d, int h0, int w0) {
+  constexpr int NOT = NO / 16, NIB = NI / 16;
+  v4f acc[NOT][kVT];
+#pragma unroll
+  for (int o = 0; o < NOT; ++o)
+#pragma unroll
+    for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 9
+  for (int tap = 0; tap < 27; ++tap) {
+    int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
+    if (MIRROR) {
+      dz = -dz;
+      dx = -dx;
+      dy = -dy;
+    }
+    v4s wf[NOT][NIB];
+    load_wf<NI, NO>(wf, a.w, tap);
+    tap_mfma<NI, NO, NS>(acc, wf, tap_base<NS>(sh, sbase, dz, dx, dy));
+  }
+  conv_store<NO, EPI>(a, acc, d, h0, w0);
+}
+
+// conv_plane with the 27 taps' weight fragments already in registers (wr).
+template <int NI, int NO, int EPI, int NS>
+__device__ __forceinline__ void conv_plane_reg(const ConvArgs& a,
+                                               const v4s (&wr)[27][NO / 16][NI / 16],
+                                               const uint16_t* sh, int sbase, int d, int h0,
+                                               int w0) {
+  v4f acc[NO / 16][kVT];
+#pragma unroll
+  for (int o = 0; o < NO / 16; ++o)
+#pragma unroll
+    for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int tap = 0; tap < 27; ++tap)
+    tap_mfma<NI, NO, NS>(acc, wr[tap],
+                         tap_base<NS>(sh, sbase, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1));
+  conv_store<NO, EPI>(a, acc, d, h0, w0);
 }
 
 // NS: channels of the input that can be non-zero (staged); NI: channels stored.
@@ -264,6 +313,18 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
   constexpr int PT = (NPL + kNetNT - 1) / kNetNT;
   __shared__ __align__(16) uint16_t sh[3 * PLANE];
   if (a.done != nullptr && *a.done) return;
+  // With one output block and at most two input blocks per tap, the 27 taps'
+  // weight fragments (<= 108 VGPRs) are loaded once, before the plane loop, at
+  // no cost in resident waves.  Global loads complete in order, so a weight
+  // load issued inside the loop would also wait for the next plane's halo
+  // loads issued before it.
+  constexpr int NOT = NO / 16, NIB = NI / 16;
+  constexpr bool WREG = NOT == 1 && NIB <= 2;
+  v4s wr[WREG ? 27 : 1][NOT][NIB];
+  if constexpr (WREG) {
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap) load_wf<NI, NO>(wr[tap], a.w, tap);
+  }
   const int D = a.D, H = a.H, W = a.W;
   const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
   int b = blockIdx.x;
@@ -312,7 +373,10 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
     store((d + 1) % 3);  // its slot held plane d-2, last read before the barrier below
     __syncthreads();
     if (d + 1 < d1) load(d + 2);
-    conv_plane<NI, NO, EPI, false, NS>(a, sh, (d + 2) % 3, d, h0, w0);
+    if constexpr (WREG)
+      conv_plane_reg<NI, NO, EPI, NS>(a, wr, sh, (d + 2) % 3, d, h0, w0);
+    else
+      conv_plane<NI, NO, EPI, false, NS>(a, sh, (d + 2) % 3, d, h0, w0);
     __syncthreads();
   }
 }
