@@ -61,6 +61,7 @@ struct ConvArgs {
   void* out;            // [D][H][W][NO] (or [D][H][W][9] complex for kEpiApply*)
   const void* r;        // kEpiApply*: the preconditioned vector [D][H][W][9] complex
   const double* scal;   // kEpiApply*: input scale (n, sum |r|^2)
+  float scale;          // kEpiApply*: the input scale s when given (> 0), else read scal
   const int* done;      // early exit while *done (may be NULL)
   int D, H, W;
 };
@@ -145,7 +146,8 @@ __device__ __forceinline__ void conv_store(const ConvArgs& a, const v4f (&acc)[N
   const int h = h0 + wave;
   if (h >= H) return;
   float inv_s = 1.f;
-  if constexpr (EPI == kEpiApply64 || EPI == kEpiApply128) inv_s = 1.f / scale_of(a.scal);
+  if constexpr (EPI == kEpiApply64 || EPI == kEpiApply128)
+    inv_s = 1.f / (a.scale > 0.f ? a.scale : scale_of(a.scal));
 #pragma unroll
   for (int t = 0; t < kVT; ++t) {
     const int w = w0 + t * 16 + col;
@@ -246,6 +248,33 @@ __device__ __forceinline__ void conv_plane_reg(const ConvArgs& a,
   conv_store<NO, EPI>(a, acc, d, h0, w0);
 }
 
+// conv_plane with the weight fragments staged in LDS in lane order
+// (wl[((tap * NOT + o) * NIB + ib) * 64 + lane]), one conflict-free 8-byte
+// read per fragment.
+template <int NI, int NO, int EPI, int NS>
+__device__ __forceinline__ void conv_plane_lds(const ConvArgs& a, const v4s* wl,
+                                               const uint16_t* sh, int sbase, int d, int h0,
+                                               int w0) {
+  constexpr int NOT = NO / 16, NIB = NI / 16;
+  const int lane = threadIdx.x & 63;
+  v4f acc[NOT][kVT];
+#pragma unroll
+  for (int o = 0; o < NOT; ++o)
+#pragma unroll
+    for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 9
+  for (int tap = 0; tap < 27; ++tap) {
+    v4s wf[NOT][NIB];
+#pragma unroll
+    for (int o = 0; o < NOT; ++o)
+#pragma unroll
+      for (int ib = 0; ib < NIB; ++ib) wf[o][ib] = wl[((tap * NOT + o) * NIB + ib) * 64 + lane];
+    tap_mfma<NI, NO, NS>(acc, wf,
+                         tap_base<NS>(sh, sbase, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1));
+  }
+  conv_store<NO, EPI>(a, acc, d, h0, w0);
+}
+
 // NS: channels of the input that can be non-zero (staged); NI: channels stored.
 template <int NI, int NO, int EPI, bool MIRROR, int NS = NI>
 __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
@@ -325,6 +354,23 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
 #pragma unroll
     for (int tap = 0; tap < 27; ++tap) load_wf<NI, NO>(wr[tap], a.w, tap);
   }
+  // Otherwise the fragments go to LDS once, where that still leaves room for
+  // two workgroups per CU; the plane loop's first barrier publishes them.
+  constexpr int NWF = 27 * NOT * NIB * 64;
+  constexpr bool WLDS = !WREG && 3 * PLANE * 2 + NWF * 8 <= 80 * 1024;
+  __shared__ v4s wl[WLDS ? NWF : 1];
+  if constexpr (WLDS) {
+    for (int f = (int)threadIdx.x; f < NWF; f += kNetNT) {
+      const int lane = f & 63;
+      int r = f >> 6;
+      const int ib = r % NIB;
+      r /= NIB;
+      const int o = r % NOT;
+      const int tap = r / NOT;
+      wl[f] = *reinterpret_cast<const v4s*>(
+          a.w + ((size_t)tap * NO + o * 16 + (lane & 15)) * NI + ib * 16 + (lane >> 4) * 4);
+    }
+  }
   const int D = a.D, H = a.H, W = a.W;
   const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
   int b = blockIdx.x;
@@ -375,6 +421,8 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
     if (d + 1 < d1) load(d + 2);
     if constexpr (WREG)
       conv_plane_reg<NI, NO, EPI, NS>(a, wr, sh, (d + 2) % 3, d, h0, w0);
+    else if constexpr (WLDS)
+      conv_plane_lds<NI, NO, EPI, NS>(a, wl, sh, (d + 2) % 3, d, h0, w0);
     else
       conv_plane<NI, NO, EPI, false, NS>(a, sh, (d + 2) % 3, d, h0, w0);
     __syncthreads();
@@ -592,12 +640,13 @@ template <typename R>
 __global__ __launch_bounds__(kNetNT) void k_net_gather(const cx<R>* __restrict__ r,
                                                        const cx<R>* __restrict__ tp,
                                                        const double* __restrict__ scal,
+                                                       float scale,
                                                        const unsigned int* __restrict__ tmax,
                                                        uint16_t* __restrict__ out, size_t nvox,
                                                        const int* __restrict__ done) {
   if (done != nullptr && *done) return;
   const float tm = __uint_as_float(*tmax);
-  const float s0 = scale_of(scal);
+  const float s0 = scale > 0.f ? scale : scale_of(scal);
   const float s = tm > 0.f ? s0 / tm : 0.f;
   for (size_t v = (size_t)blockIdx.x * kNetNT + threadIdx.x; v < nvox;
        v += (size_t)gridDim.x * kNetNT) {
@@ -849,6 +898,7 @@ struct NetOps {
                            nullptr, z, done);
     a.r = r;
     a.scal = net.scal;
+    a.scale = net.scale_arg;
     constexpr int EPI = sizeof(R) == 4 ? kEpiApply64 : kEpiApply128;
     const int zc = conv_zc();
     hipLaunchKernelGGL((k_conv_zm<HID, kNetIOPad, EPI>), conv_grid_zm(net, zc), dim3(kNetNT), 0,
@@ -1001,13 +1051,23 @@ int net_adam(Net& net, double lr, hipStream_t s) {
   return net_pack(net, s);
 }
 
+// The float kernel argument for a given scale, or 0 when the scale is measured
+// (scale <= 0) or not a positive normal float.
+static float given_scale(double scale) {
+  const float f = (float)scale;
+  return scale > 0.0 && std::isnormal(f) ? f : 0.f;
+}
+
 template <typename R>
 int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_t* out,
               hipStream_t s, const int* done) {
   MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
-  if (scale > 0.0) {
+  // A given scale that is a positive normal float travels as a kernel
+  // argument; otherwise it (or the measured one) is stored in scal.
+  const float sarg = given_scale(scale);
+  if (sarg == 0.f && scale > 0.0) {
     hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, scale * scale, 1.0);
-  } else {
+  } else if (sarg == 0.f) {
     hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, (double)p.n, 0.0);
     hipLaunchKernelGGL((k_norm_scale<R>), dim3(grid_of(p.n, 2048)), dim3(kNetNT), 0, s, r, p.n,
                        scal);
@@ -1019,7 +1079,7 @@ int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_
     tp = p.tp64;
   }
   hipLaunchKernelGGL((k_net_gather<R>), dim3(grid_of(p.nvox)), dim3(kNetNT), 0, s, r, tp, scal,
-                     p.tmax, out, p.nvox, done);
+                     sarg, p.tmax, out, p.nvox, done);
   MSC_LAUNCH_CHECK();
   return kOk;
 }
@@ -1032,6 +1092,7 @@ int net_apply(Net& net, const Plan& p, const cx<R>* r, cx<R>* z, double scale, h
   if (rc) return rc;
   rc = net_input<R>(p, r, scale, net.scal, net.act[0], s, done);
   if (rc) return rc;
+  net.scale_arg = given_scale(scale);
   if constexpr (sizeof(R) == 4) {
     by_width<Apply64>(net.hidden, net, r, z, s, done);
   } else {

@@ -47,6 +47,7 @@ struct Net {
   uint16_t* gbuf[2] = {};          // gradient ping-pong (bf16), sized for the widest layer
   double* loss = nullptr;          // device loss accumulator (+ partials)
   double* scal = nullptr;          // device (n, sum |r|^2) of the current apply: s = sqrt(n/sum)
+  float scale_arg = 0.f;           // host: a given s of the current apply, 0 = read scal
   bool packed = false;
   // changes whenever device buffers are (re)allocated; captured graphs that
   // reference this network are valid only for one generation
