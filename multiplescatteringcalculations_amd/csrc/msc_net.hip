@@ -9,10 +9,12 @@
 // One v_mfma_f32_16x16x16_bf16 multiplies a 16 (out channel) x 16 (in
 // channel) weight block by 16 (in channel) x 16 (voxel) activations.  Lane l
 // of a wave holds A[l%16][4(l/16)+0..3], B[4(l/16)+0..3][l%16] and
-// D[4(l/16)+0..3][l%16].  A workgroup stages its input halo in LDS; each wave
-// owns 64 consecutive voxels of one row and every output channel, and loads
-// the weight fragments of a tap once for its four voxel tiles.  The last
-// layer of a preconditioner application writes z = r + N / s directly.
+// D[4(l/16)+0..3][l%16].  The forward layers of the preconditioner use
+// v_mfma_f32_16x16x32_bf16 over (tap, channel) instead (conv_plane32).  A
+// workgroup stages its input halo in LDS; each wave owns 64 consecutive voxels
+// of one row and every output channel, and loads the weight fragments of a tap
+// once for its four voxel tiles.  The last layer of a preconditioner
+// application writes z = r + N / s directly.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -24,6 +26,8 @@
 namespace msc {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8b __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kNetNT = 256;  // threads per workgroup (4 waves)
@@ -230,47 +234,93 @@ __device__ __forceinline__ void conv_plane(const ConvArgs& a, const uint16_t* sh
   conv_store<NO, EPI>(a, acc, d, h0, w0);
 }
 
-// conv_plane with the 27 taps' weight fragments already in registers (wr).
-template <int NI, int NO, int EPI, int NS>
-__device__ __forceinline__ void conv_plane_reg(const ConvArgs& a,
-                                               const v4s (&wr)[27][NO / 16][NI / 16],
-                                               const uint16_t* sh, int sbase, int d, int h0,
-                                               int w0) {
-  v4f acc[NO / 16][kVT];
+// Forward layers contract over K = (tap, input channel), 27 NI long, in steps
+// of 32 with v_mfma_f32_16x16x32_bf16 (lane l holds A[l%16][8(l/16) + 0..7],
+// B[8(l/16) + 0..7][l%16]).  Lane group g of step s covers the 8 channels at
+// K index 32 s + 8 g: one tap per step for NI = 32, two taps for NI = 16; K
+// beyond 27 NI (the second half of the last step when 27 NI is an odd
+// multiple of 16) is zero on both operands.
+template <int NI>
+constexpr int k32_steps() { return (27 * NI + 31) / 32; }
+
+// Weight fragments of step s (zero beyond the 27th tap).
+template <int NI, int NO>
+__device__ __forceinline__ void load_wf32(v8s (&wf)[NO / 16], const uint16_t* w, int s) {
+  const int lane = threadIdx.x & 63;
+  const int k = 32 * s + 8 * (lane >> 4);
+  const int tap = k / NI, ch = k - tap * NI;
 #pragma unroll
-  for (int o = 0; o < NO / 16; ++o)
-#pragma unroll
-    for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int tap = 0; tap < 27; ++tap)
-    tap_mfma<NI, NO, NS>(acc, wr[tap],
-                         tap_base<NS>(sh, sbase, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1));
-  conv_store<NO, EPI>(a, acc, d, h0, w0);
+  for (int o = 0; o < NO / 16; ++o) {
+    wf[o] = v8s{0, 0, 0, 0, 0, 0, 0, 0};
+    if (tap < 27)
+      wf[o] = *reinterpret_cast<const v8s*>(w + ((size_t)tap * NO + o * 16 + (lane & 15)) * NI +
+                                            ch);
+  }
 }
 
-// conv_plane with the weight fragments staged in LDS in lane order
-// (wl[((tap * NOT + o) * NIB + ib) * 64 + lane]), one conflict-free 8-byte
-// read per fragment.
-template <int NI, int NO, int EPI, int NS>
-__device__ __forceinline__ void conv_plane_lds(const ConvArgs& a, const v4s* wl,
-                                               const uint16_t* sh, int sbase, int d, int h0,
-                                               int w0) {
-  constexpr int NOT = NO / 16, NIB = NI / 16;
+// The MFMAs of step s over the wave's kVT voxel tiles; B fragments from the
+// staged halo (two 8-byte reads; channels at or above NS are zero).
+template <int NI, int NO, int NS>
+__device__ __forceinline__ void step_mfma32(v4f (&acc)[NO / 16][kVT], const v8s (&wf)[NO / 16],
+                                            const uint16_t* sh, int sbase, int s) {
+  constexpr int LDV = Halo<NS>::LDV;
+  const int lane = threadIdx.x & 63;
+  const int k = 32 * s + 8 * (lane >> 4);
+  const int tap = k / NI, ch = k - tap * NI;
+  const bool live = tap < 27;
+  const int tq = live ? tap : 26;  // an address inside the halo for the zero half
+  const uint16_t* base =
+      tap_base<NS>(sh, sbase, tq / 9 - 1, (tq / 3) % 3 - 1, tq % 3 - 1) - (lane >> 4) * 4 + ch;
+  const bool lo_ok = live && ch < NS, hi_ok = live && ch + 4 < NS;
+#pragma unroll
+  for (int t = 0; t < kVT; ++t) {
+    v4s lo = v4s{0, 0, 0, 0}, hi = v4s{0, 0, 0, 0};
+    if (lo_ok) lo = *reinterpret_cast<const v4s*>(base + t * 16 * LDV);
+    if (hi_ok) hi = *reinterpret_cast<const v4s*>(base + t * 16 * LDV + 4);
+    const v8s bf = v8s{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+#pragma unroll
+    for (int o = 0; o < NO / 16; ++o)
+      acc[o][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          __builtin_bit_cast(v8b, wf[o]), __builtin_bit_cast(v8b, bf), acc[o][t], 0, 0, 0);
+  }
+}
+
+// Where a forward layer's weight fragments live during the plane loop.
+enum WeightSrc : int {
+  kWGlobal = 0,  // loaded per step
+  kWReg = 1,     // registers, loaded once per workgroup
+  kWLds = 2      // LDS in lane order, staged once per workgroup
+};
+
+// One output plane of the forward (z-marching) convolution, K = 32 steps.
+template <int NI, int NO, int EPI, int NS, int WSRC>
+__device__ __forceinline__ void conv_plane32(const ConvArgs& a,
+                                             const v8s (&wr)[WSRC == kWReg ? k32_steps<NI>() : 1]
+                                                            [NO / 16],
+                                             const v8s* wl, const uint16_t* sh, int sbase, int d,
+                                             int h0, int w0) {
+  constexpr int NOT = NO / 16, S = k32_steps<NI>();
   const int lane = threadIdx.x & 63;
   v4f acc[NOT][kVT];
 #pragma unroll
   for (int o = 0; o < NOT; ++o)
 #pragma unroll
     for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  if constexpr (WSRC == kWReg) {
+#pragma unroll
+    for (int st = 0; st < S; ++st) step_mfma32<NI, NO, NS>(acc, wr[st], sh, sbase, st);
+  } else {
 #pragma unroll 9
-  for (int tap = 0; tap < 27; ++tap) {
-    v4s wf[NOT][NIB];
+    for (int st = 0; st < S; ++st) {
+      v8s wf[NOT];
+      if constexpr (WSRC == kWLds) {
 #pragma unroll
-    for (int o = 0; o < NOT; ++o)
-#pragma unroll
-      for (int ib = 0; ib < NIB; ++ib) wf[o][ib] = wl[((tap * NOT + o) * NIB + ib) * 64 + lane];
-    tap_mfma<NI, NO, NS>(acc, wf,
-                         tap_base<NS>(sh, sbase, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1));
+        for (int o = 0; o < NOT; ++o) wf[o] = wl[(st * NOT + o) * 64 + lane];
+      } else {
+        load_wf32<NI, NO>(wf, a.w, st);
+      }
+      step_mfma32<NI, NO, NS>(acc, wf, sh, sbase, st);
+    }
   }
   conv_store<NO, EPI>(a, acc, d, h0, w0);
 }
@@ -342,33 +392,36 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
   constexpr int PT = (NPL + kNetNT - 1) / kNetNT;
   __shared__ __align__(16) uint16_t sh[3 * PLANE];
   if (a.done != nullptr && *a.done) return;
-  // With one output block and at most two input blocks per tap, the 27 taps'
-  // weight fragments (<= 108 VGPRs) are loaded once, before the plane loop, at
-  // no cost in resident waves.  Global loads complete in order, so a weight
-  // load issued inside the loop would also wait for the next plane's halo
-  // loads issued before it.
-  constexpr int NOT = NO / 16, NIB = NI / 16;
-  constexpr bool WREG = NOT == 1 && NIB <= 2;
-  v4s wr[WREG ? 27 : 1][NOT][NIB];
-  if constexpr (WREG) {
+  // With one output block and at most 27 steps (NI <= 32), the weight
+  // fragments (<= 108 VGPRs) are loaded once, before the plane loop, at no
+  // cost in resident waves.  Global loads complete in order, so a weight load
+  // issued inside the loop would also wait for the next plane's halo loads
+  // issued before it.  Otherwise the fragments go to LDS once, where that
+  // still leaves room for two workgroups per CU (the plane loop's first
+  // barrier publishes them), or are loaded per step.
+  // Layers of 48 input channels, whose steps straddle taps, keep the
+  // 16x16x16 form per tap (the K = 32 steps spill there).
+  constexpr bool K32 = NI % 32 == 0 || NI == 16;
+  constexpr int NOT = NO / 16, S = k32_steps<NI>();
+  constexpr int NWF = S * NOT * 64;
+  constexpr int WSRC = !K32                                        ? kWGlobal
+                       : (NOT == 1 && S <= 27)                     ? kWReg
+                       : (3 * PLANE * 2 + NWF * 16 <= 80 * 1024) ? kWLds
+                                                                 : kWGlobal;
+  v8s wr[WSRC == kWReg ? S : 1][NOT];
+  __shared__ v8s wl[WSRC == kWLds ? NWF : 1];
+  if constexpr (WSRC == kWReg) {
 #pragma unroll
-    for (int tap = 0; tap < 27; ++tap) load_wf<NI, NO>(wr[tap], a.w, tap);
-  }
-  // Otherwise the fragments go to LDS once, where that still leaves room for
-  // two workgroups per CU; the plane loop's first barrier publishes them.
-  constexpr int NWF = 27 * NOT * NIB * 64;
-  constexpr bool WLDS = !WREG && 3 * PLANE * 2 + NWF * 8 <= 80 * 1024;
-  __shared__ v4s wl[WLDS ? NWF : 1];
-  if constexpr (WLDS) {
+    for (int st = 0; st < S; ++st) load_wf32<NI, NO>(wr[st], a.w, st);
+  } else if constexpr (WSRC == kWLds) {
     for (int f = (int)threadIdx.x; f < NWF; f += kNetNT) {
-      const int lane = f & 63;
-      int r = f >> 6;
-      const int ib = r % NIB;
-      r /= NIB;
-      const int o = r % NOT;
-      const int tap = r / NOT;
-      wl[f] = *reinterpret_cast<const v4s*>(
-          a.w + ((size_t)tap * NO + o * 16 + (lane & 15)) * NI + ib * 16 + (lane >> 4) * 4);
+      const int lane = f & 63, r = f >> 6;
+      const int o = r % NOT, st = r / NOT;
+      const int k = 32 * st + 8 * (lane >> 4);
+      const int tap = k / NI, ch = k - tap * NI;
+      wl[f] = tap < 27 ? *reinterpret_cast<const v8s*>(
+                             a.w + ((size_t)tap * NO + o * 16 + (lane & 15)) * NI + ch)
+                       : v8s{0, 0, 0, 0, 0, 0, 0, 0};
     }
   }
   const int D = a.D, H = a.H, W = a.W;
@@ -419,10 +472,8 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
     store((d + 1) % 3);  // its slot held plane d-2, last read before the barrier below
     __syncthreads();
     if (d + 1 < d1) load(d + 2);
-    if constexpr (WREG)
-      conv_plane_reg<NI, NO, EPI, NS>(a, wr, sh, (d + 2) % 3, d, h0, w0);
-    else if constexpr (WLDS)
-      conv_plane_lds<NI, NO, EPI, NS>(a, wl, sh, (d + 2) % 3, d, h0, w0);
+    if constexpr (K32)
+      conv_plane32<NI, NO, EPI, NS, WSRC>(a, wr, wl, sh, (d + 2) % 3, d, h0, w0);
     else
       conv_plane<NI, NO, EPI, false, NS>(a, sh, (d + 2) % 3, d, h0, w0);
     __syncthreads();
