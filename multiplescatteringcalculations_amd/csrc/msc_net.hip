@@ -66,6 +66,8 @@ struct ConvArgs {
   const void* r;        // kEpiApply*: the preconditioned vector [D][H][W][9] complex
   const double* scal;   // kEpiApply*: input scale (n, sum |r|^2)
   float scale;          // kEpiApply*: the input scale s when given (> 0), else read scal
+  const void* tp;       // fused input: T0 parameters [D][H][W][4] complex (precision of r)
+  const unsigned int* tmax;  // fused input: max |t| over the slab (float bits)
   const int* done;      // early exit while *done (may be NULL)
   int D, H, W;
 };
@@ -377,13 +379,55 @@ __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
   conv_plane<NI, NO, EPI, MIRROR, NS>(a, sh, 0, d, h0, w0);
 }
 
+// The network's input at voxel v: (s / t_max) T0 r as kNetIO bf16 channels,
+// channel 2c = Re, 2c+1 = Im of component c (s / t_max passed as st).
+template <typename R>
+__device__ __forceinline__ void t0_input(const cx<R>* __restrict__ r,
+                                         const cx<R>* __restrict__ tp, size_t v, float st,
+                                         uint16_t* pk) {
+  const cx<R>* t = tp + v * 4;
+  const cx<R> tu = t[0], td = t[1], to = t[2], ts = t[3];
+  cx<R> x[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) x[c] = r[v * 9 + c];
+  const cx<R> sum = cadd<R>(cadd<R>(x[3], x[4]), x[5]);
+  const cx<R> dmo = csub<R>(td, to);
+  const cx<R> tos = cmul<R>(to, sum);
+  cx<R> y[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[c] = cmul<R>(tu, x[c]);
+#pragma unroll
+  for (int c = 3; c < 6; ++c) y[c] = cadd<R>(cmul<R>(dmo, x[c]), tos);
+#pragma unroll
+  for (int c = 6; c < 9; ++c) y[c] = cmul<R>(ts, x[c]);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    pk[2 * c] = f2bf(st * (float)y[c].x);
+    pk[2 * c + 1] = f2bf(st * (float)y[c].y);
+  }
+}
+
+// s / t_max of the current application (0 when T0 vanishes everywhere).
+__device__ __forceinline__ float input_scale(const double* scal, float scale,
+                                             const unsigned int* tmax) {
+  const float tm = __uint_as_float(*tmax);
+  const float s0 = scale > 0.f ? scale : scale_of(scal);
+  return tm > 0.f ? s0 / tm : 0.f;
+}
+
 // Forward convolution marching along z: a workgroup computes planes
 // d0 .. d0+zc-1 of its tile with a ring of three staged input planes in LDS,
 // and the loads of plane d+2 are in flight while plane d is computed.  Each
-// output plane stages one new input plane (three in k_conv).
-template <int NI, int NO, int EPI, int NS = NI>
+// output plane stages one new input plane (three in k_conv).  SRC 0 stages
+// the bf16 grid a.in; SRC 1 / 2 (the first layer of a preconditioner
+// application) stages the network input computed from the complex64 /
+// complex128 vector a.r and T0 parameters a.tp, as k_net_gather would
+// write it, so the application needs no separate gather pass.
+template <int NI, int NO, int EPI, int NS = NI, int SRC = 0>
 __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
   static_assert(NS % 4 == 0 && NS <= NI, "staged channels must be a multiple of 4");
+  static_assert(SRC == 0 || (NS == 20 && NI == kNetIOPad), "the fused input stages 20 channels");
+  using RS = typename std::conditional<SRC == 2, double, float>::type;
   constexpr int LDV = Halo<NS>::LDV;
   constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
   constexpr int PLANE = ROWS * COLS * LDV;
@@ -433,32 +477,71 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
   const int d0 = (b / nbh) * zc;
   const int d1 = min(D, d0 + zc);
   const int h0 = bh * kTH, w0 = bw * kTW;
-  v4s val[PT];
+  // per thread: PT groups of 4 channels (bf16 grid) or PV voxels of 5 groups
+  constexpr int NVX = ROWS * COLS, PV = (NVX + kNetNT - 1) / kNetNT;
+  v4s val[SRC == 0 ? PT : PV * 5];
+  float st = 0.f;
+  if constexpr (SRC != 0) st = input_scale(a.scal, a.scale, a.tmax);
   auto load = [&](int gd) {
+    if constexpr (SRC == 0) {
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-      const int e = k * kNetNT + (int)threadIdx.x;
-      val[k] = v4s{0, 0, 0, 0};
-      if (e < NPL && gd >= 0 && gd < D) {
-        const int g = e % G4;
-        const int q = e / G4;
-        const int c = q % COLS, rr = q / COLS;
-        const int gh = h0 + rr - 1, gw = w0 + c - 1;
-        if (gh >= 0 && gh < H && gw >= 0 && gw < W)
-          val[k] = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI +
-                                                 g * 4);
+      for (int k = 0; k < PT; ++k) {
+        const int e = k * kNetNT + (int)threadIdx.x;
+        val[k] = v4s{0, 0, 0, 0};
+        if (e < NPL && gd >= 0 && gd < D) {
+          const int g = e % G4;
+          const int q = e / G4;
+          const int c = q % COLS, rr = q / COLS;
+          const int gh = h0 + rr - 1, gw = w0 + c - 1;
+          if (gh >= 0 && gh < H && gw >= 0 && gw < W)
+            val[k] = *reinterpret_cast<const v4s*>(a.in + (((size_t)gd * H + gh) * W + gw) * NI +
+                                                   g * 4);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PV; ++k) {
+        const int q = k * kNetNT + (int)threadIdx.x;
+#pragma unroll
+        for (int g = 0; g < 5; ++g) val[5 * k + g] = v4s{0, 0, 0, 0};
+        if (q < NVX && gd >= 0 && gd < D) {
+          const int c = q % COLS, rr = q / COLS;
+          const int gh = h0 + rr - 1, gw = w0 + c - 1;
+          if (gh >= 0 && gh < H && gw >= 0 && gw < W) {
+            uint16_t pk[kNetIO];
+            t0_input<RS>(reinterpret_cast<const cx<RS>*>(a.r),
+                         reinterpret_cast<const cx<RS>*>(a.tp), ((size_t)gd * H + gh) * W + gw,
+                         st, pk);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+              val[5 * k + g] = v4s{(short)pk[4 * g], (short)pk[4 * g + 1], (short)pk[4 * g + 2],
+                                   (short)pk[4 * g + 3]};
+            val[5 * k + 4] = v4s{(short)pk[16], (short)pk[17], 0, 0};
+          }
+        }
       }
     }
   };
   auto store = [&](int slot) {
+    if constexpr (SRC == 0) {
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-      const int e = k * kNetNT + (int)threadIdx.x;
-      if (e < NPL) {
-        const int g = e % G4;
-        const int q = e / G4;
-        const int c = q % COLS, rr = q / COLS;
-        *reinterpret_cast<v4s*>(&sh[slot * PLANE + (rr * COLS + c) * LDV + g * 4]) = val[k];
+      for (int k = 0; k < PT; ++k) {
+        const int e = k * kNetNT + (int)threadIdx.x;
+        if (e < NPL) {
+          const int g = e % G4;
+          const int q = e / G4;
+          *reinterpret_cast<v4s*>(&sh[slot * PLANE + q * LDV + g * 4]) = val[k];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PV; ++k) {
+        const int q = k * kNetNT + (int)threadIdx.x;
+        if (q < NVX) {
+#pragma unroll
+          for (int g = 0; g < 5; ++g)
+            *reinterpret_cast<v4s*>(&sh[slot * PLANE + q * LDV + g * 4]) = val[5 * k + g];
+        }
       }
     }
   };
@@ -696,32 +779,11 @@ __global__ __launch_bounds__(kNetNT) void k_net_gather(const cx<R>* __restrict__
                                                        uint16_t* __restrict__ out, size_t nvox,
                                                        const int* __restrict__ done) {
   if (done != nullptr && *done) return;
-  const float tm = __uint_as_float(*tmax);
-  const float s0 = scale > 0.f ? scale : scale_of(scal);
-  const float s = tm > 0.f ? s0 / tm : 0.f;
+  const float s = input_scale(scal, scale, tmax);
   for (size_t v = (size_t)blockIdx.x * kNetNT + threadIdx.x; v < nvox;
        v += (size_t)gridDim.x * kNetNT) {
-    const cx<R>* t = tp + v * 4;
-    const cx<R> tu = t[0], td = t[1], to = t[2], ts = t[3];
-    cx<R> x[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) x[c] = r[v * 9 + c];
-    const cx<R> sum = cadd<R>(cadd<R>(x[3], x[4]), x[5]);
-    const cx<R> dmo = csub<R>(td, to);
-    const cx<R> tos = cmul<R>(to, sum);
-    cx<R> y[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) y[c] = cmul<R>(tu, x[c]);
-#pragma unroll
-    for (int c = 3; c < 6; ++c) y[c] = cadd<R>(cmul<R>(dmo, x[c]), tos);
-#pragma unroll
-    for (int c = 6; c < 9; ++c) y[c] = cmul<R>(ts, x[c]);
     uint16_t pk[kNetIOPad];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      pk[2 * c] = f2bf(s * (float)y[c].x);
-      pk[2 * c + 1] = f2bf(s * (float)y[c].y);
-    }
+    t0_input<R>(r, tp, v, s, pk);
 #pragma unroll
     for (int c = kNetIO; c < kNetIOPad; ++c) pk[c] = 0;
     uint4* dst = reinterpret_cast<uint4*>(out + v * kNetIOPad);
@@ -810,7 +872,7 @@ int Net::ensure_grid(int D_, int H_, int W_) {
   release_grid();
   gen = ++g_net_gen;
   const size_t nv = (size_t)D_ * H_ * W_;
-  for (int l = 0; l < kNetLayers; ++l)
+  for (int l = 1; l < kNetLayers; ++l)
     MSC_HIP_CHECK(hipMalloc(&act[l], std::max<size_t>(nv, 1) * pin[l] * sizeof(uint16_t)));
   const int widest = std::max(hidden, kNetIOPad);
   for (int i = 0; i < 2; ++i)
@@ -917,11 +979,18 @@ template <int HID>
 struct NetOps {
   static void hidden_layers(Net& net, const uint16_t* in, hipStream_t s, const int* done) {
     const int zc = conv_zc();
-    const dim3 G = conv_grid_zm(net, zc);
-    hipLaunchKernelGGL((k_conv_zm<kNetIOPad, HID, kEpiHidden, kIOStaged>), G, dim3(kNetNT), 0, s,
+    hipLaunchKernelGGL((k_conv_zm<kNetIOPad, HID, kEpiHidden, kIOStaged>), conv_grid_zm(net, zc),
+                       dim3(kNetNT), 0, s,
                        conv_args(net, in, net.wpk + net.wpad_off[0], net.bpk + net.bpad_off[0],
                                  nullptr, net.act[1], done),
                        zc);
+    hidden_2_3(net, s, done);
+  }
+
+  // The second and third layers (act[1] -> act[2] -> act[3]).
+  static void hidden_2_3(Net& net, hipStream_t s, const int* done) {
+    const int zc = conv_zc();
+    const dim3 G = conv_grid_zm(net, zc);
     hipLaunchKernelGGL((k_conv_zm<HID, HID, kEpiHidden>), G, dim3(kNetNT), 0, s,
                        conv_args(net, net.act[1], net.wpk + net.wpad_off[1],
                                  net.bpk + net.bpad_off[1], nullptr, net.act[2], done),
@@ -942,16 +1011,29 @@ struct NetOps {
                        zc);
   }
 
+  // z = r + N((s / t_max) T0 r) / s; the first layer computes its input
+  // from r and the T0 parameters tp itself.
   template <typename R>
-  static void apply(Net& net, const cx<R>* r, cx<R>* z, hipStream_t s, const int* done) {
-    hidden_layers(net, net.act[0], s, done);
+  static void apply(Net& net, const cx<R>* r, const cx<R>* tp, const unsigned int* tmax,
+                    cx<R>* z, hipStream_t s, const int* done) {
+    const int zc = conv_zc();
+    ConvArgs a0 = conv_args(net, nullptr, net.wpk + net.wpad_off[0], net.bpk + net.bpad_off[0],
+                            nullptr, net.act[1], done);
+    a0.r = r;
+    a0.tp = tp;
+    a0.tmax = tmax;
+    a0.scal = net.scal;
+    a0.scale = net.scale_arg;
+    constexpr int SRC = sizeof(R) == 4 ? 1 : 2;
+    hipLaunchKernelGGL((k_conv_zm<kNetIOPad, HID, kEpiHidden, kIOStaged, SRC>),
+                       conv_grid_zm(net, zc), dim3(kNetNT), 0, s, a0, zc);
+    hidden_2_3(net, s, done);
     ConvArgs a = conv_args(net, net.act[3], net.wpk + net.wpad_off[3], net.bpk + net.bpad_off[3],
                            nullptr, z, done);
     a.r = r;
     a.scal = net.scal;
     a.scale = net.scale_arg;
     constexpr int EPI = sizeof(R) == 4 ? kEpiApply64 : kEpiApply128;
-    const int zc = conv_zc();
     hipLaunchKernelGGL((k_conv_zm<HID, kNetIOPad, EPI>), conv_grid_zm(net, zc), dim3(kNetNT), 0,
                        s, a, zc);
   }
@@ -1025,14 +1107,16 @@ struct Bwd {
 };
 template <int HID>
 struct Apply64 {
-  static void run(Net& n, const float2* r, float2* z, hipStream_t s, const int* done) {
-    NetOps<HID>::template apply<float>(n, r, z, s, done);
+  static void run(Net& n, const float2* r, const float2* tp, const unsigned int* tmax, float2* z,
+                  hipStream_t s, const int* done) {
+    NetOps<HID>::template apply<float>(n, r, tp, tmax, z, s, done);
   }
 };
 template <int HID>
 struct Apply128 {
-  static void run(Net& n, const double2* r, double2* z, hipStream_t s, const int* done) {
-    NetOps<HID>::template apply<double>(n, r, z, s, done);
+  static void run(Net& n, const double2* r, const double2* tp, const unsigned int* tmax,
+                  double2* z, hipStream_t s, const int* done) {
+    NetOps<HID>::template apply<double>(n, r, tp, tmax, z, s, done);
   }
 };
 
@@ -1109,12 +1193,12 @@ static float given_scale(double scale) {
   return scale > 0.0 && std::isnormal(f) ? f : 0.f;
 }
 
+// A given scale that is a positive normal float travels as a kernel argument
+// (returned); otherwise it (or the measured one, scale <= 0) is stored in scal
+// and 0 is returned.
 template <typename R>
-int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_t* out,
-              hipStream_t s, const int* done) {
-  MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
-  // A given scale that is a positive normal float travels as a kernel
-  // argument; otherwise it (or the measured one) is stored in scal.
+static float set_input_scale(const Plan& p, const cx<R>* r, double scale, double* scal,
+                             hipStream_t s) {
   const float sarg = given_scale(scale);
   if (sarg == 0.f && scale > 0.0) {
     hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, scale * scale, 1.0);
@@ -1123,6 +1207,14 @@ int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_
     hipLaunchKernelGGL((k_norm_scale<R>), dim3(grid_of(p.n, 2048)), dim3(kNetNT), 0, s, r, p.n,
                        scal);
   }
+  return sarg;
+}
+
+template <typename R>
+int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_t* out,
+              hipStream_t s, const int* done) {
+  MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
+  const float sarg = set_input_scale<R>(p, r, scale, scal, s);
   const cx<R>* tp;
   if constexpr (sizeof(R) == 8) {
     tp = p.tp128;
@@ -1141,13 +1233,12 @@ int net_apply(Net& net, const Plan& p, const cx<R>* r, cx<R>* z, double scale, h
   MSC_REQUIRE(net.packed, "network parameters are not set");
   int rc = net.ensure_grid(p.Nz, p.M, p.M);
   if (rc) return rc;
-  rc = net_input<R>(p, r, scale, net.scal, net.act[0], s, done);
-  if (rc) return rc;
-  net.scale_arg = given_scale(scale);
+  MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
+  net.scale_arg = set_input_scale<R>(p, r, scale, net.scal, s);
   if constexpr (sizeof(R) == 4) {
-    by_width<Apply64>(net.hidden, net, r, z, s, done);
+    by_width<Apply64>(net.hidden, net, r, p.tp64, p.tmax, z, s, done);
   } else {
-    by_width<Apply128>(net.hidden, net, r, z, s, done);
+    by_width<Apply128>(net.hidden, net, r, p.tp128, p.tmax, z, s, done);
   }
   MSC_LAUNCH_CHECK();
   return kOk;

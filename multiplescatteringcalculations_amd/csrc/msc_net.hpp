@@ -42,7 +42,8 @@ struct Net {
   // activation workspace for one grid
   int D = 0, H = 0, W = 0;
   size_t nvox = 0;
-  uint16_t* act[kNetLayers] = {};  // act[0] input (pin[0]); act[l] = output of layer l-1
+  uint16_t* act[kNetLayers] = {};  // act[l] = output of layer l-1 (act[0] unused: the input is
+                                   // the caller's, or computed by an application's first layer)
   float* outf = nullptr;           // last layer output, fp32 [vox][pout[3]]
   uint16_t* gbuf[2] = {};          // gradient ping-pong (bf16), sized for the widest layer
   double* loss = nullptr;          // device loss accumulator (+ partials)
