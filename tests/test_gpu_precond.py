@@ -26,8 +26,9 @@ REF_C = otm.Medium(3000.0, 1732.0, 2500.0)
 
 
 def _nrel(a, b):
-    a = torch.as_tensor(a).double().cpu()
-    b = torch.as_tensor(b).double().cpu()
+    a, b = torch.as_tensor(a).cpu(), torch.as_tensor(b).cpu()
+    dt = torch.complex128 if (a.is_complex() or b.is_complex()) else torch.float64
+    a, b = a.to(dt), b.to(dt)
     return float(torch.linalg.vector_norm(a - b) / max(float(torch.linalg.vector_norm(b)), 1e-300))
 
 
@@ -167,6 +168,56 @@ def test_random_network_fgmres_reaches_same_solution(hip_device, precision):
     # and a plain solve after it is unaffected by the flexible one
     x3, s3 = plan.gmres(psi0, "born", tol=1e-11, max_iter=300, restart=20, precision="fp64")
     assert torch.equal(x3, x0) and s3["iterations"] == s0["iterations"]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-5)])
+def test_fgmres_steps_match_host_fgmres(hip_device, precision, tol):
+    """Three flexible GMRES steps on the device (where the network's input
+    scale is passed as sqrt(n) for the unit-norm v_j) against the same three
+    steps on the host, with z_j = net.apply(v_j) measuring the scale of v_j.
+    The device builds its basis from w = v_j - A z_j, so its v_{j+1} is the
+    textbook one times -1; the network is not odd (biases, leaky ReLU), so the
+    host follows the same basis: H[:, j] = e_j - V^H w, H[j+1, j] = -||w_perp||,
+    x_3 = x_0 + Z y, y = argmin ||beta e_1 - H y||.  A wrong input or output
+    scale on either path, or the other sign, changes x_3 at order one."""
+    import warnings
+
+    plan, psi0, *_ = _small_plan(seed=4)
+    net, _ = _net(16, seed=6, scale=0.4)
+    k = 3
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        xd, sd = plan.gmres(psi0, "born", tol=1e-14, max_iter=k, restart=20, precision=precision,
+                            precond=net)
+    assert sd["iterations"] == k and sd["precond_applies"] == k and not sd["fell_back"]
+    b = psi0.to(hip_device, torch.complex128)
+    x0 = b.clone()
+    r0 = b - plan.matvec(x0)
+    beta = float(torch.linalg.vector_norm(r0))
+    V, Z = [r0 / beta], []
+    H = torch.zeros(k + 1, k, dtype=torch.complex128)
+    for j in range(k):
+        z = net.apply(plan, V[j])
+        Z.append(z)
+        w = V[j] - plan.matvec(z)
+        h = torch.zeros(j + 1, dtype=torch.complex128)
+        for _ in range(2):  # classical Gram-Schmidt, twice
+            for i in range(j + 1):
+                c = torch.vdot(V[i], w)
+                h[i] += c.cpu()
+                w = w - c * V[i]
+        H[: j + 1, j] = -h
+        H[j, j] += 1.0
+        hn = float(torch.linalg.vector_norm(w))
+        H[j + 1, j] = -hn
+        V.append(w / hn)
+    e1 = torch.zeros(k + 1, 1, dtype=torch.complex128)
+    e1[0, 0] = beta
+    y = torch.linalg.lstsq(H, e1).solution.reshape(-1)
+    xh = x0 + sum(y[j].item() * Z[j] for j in range(k))
+    xd = xd.reshape(-1).to(hip_device)
+    assert _nrel(xh - x0, x0) > 1e-3  # the steps move the iterate
+    assert _nrel(xd - x0, xh - x0) < tol, _nrel(xd - x0, xh - x0)
 
 
 def test_useless_preconditioner_hands_over_to_plain_gmres(hip_device):
