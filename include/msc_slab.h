@@ -131,6 +131,26 @@ int msc_resonance_tmatrices(double omega, double a, double alpha, double beta, d
                             int neumann_order, double neumann_tol, void* tcomp, void* psi_exc,
                             double* pivot_ratio, int* neumann_iters, void* stream);
 
+/*
+ * msc_resonance_tmatrices without any device allocation.  No counterpart in
+ * the reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and errors as
+ * msc_resonance_tmatrices; the workspace is a 64 MiB device array inside the
+ * library that every GPU it runs on holds until the process ends, so n_sub
+ * is limited to 6 (larger n_sub is MSC_ERR_UNSUPPORTED; any count is taken,
+ * in chunks of at most 64 cubes).  Calls no hipMalloc/hipFree.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (other threads,
+ * other streams) race and must be serialised by the caller.  Synchronises
+ * `stream` before returning.
+ */
+int msc_resonance_tmatrices_noalloc(double omega, double a, double alpha, double beta, double rho,
+                                    int n_sub, const double* khat3, int wave_s, int count,
+                                    const double* dlambda, const double* dmu, const double* drho,
+                                    int neumann_order, double neumann_tol, void* tcomp,
+                                    void* psi_exc, double* pivot_ratio, int* neumann_iters,
+                                    void* stream);
+
 /* Last error message of the calling thread ("" if none). */
 const char* msc_last_error(void);
 

@@ -38,6 +38,10 @@ SIGNATURES = {
                                         c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                         c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p]),
+    "msc_resonance_tmatrices_noalloc": (c_int, [c_double, c_double, c_double, c_double, c_double,
+                                                c_int, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                                c_void_p, c_int, c_double, c_void_p, c_void_p,
+                                                c_void_p, c_void_p, c_void_p]),
     "msc_plan_info": (c_int, [c_void_p, POINTER(c_longlong)]),
     "msc_plan_autotune": (c_int, [c_void_p, c_int, c_void_p]),
     "msc_plan_tiles": (c_int, [c_void_p, POINTER(c_int)]),

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/msc_slab.h"
@@ -460,6 +461,59 @@ std::vector<double2> incident_patterns(int n, double a, double k, const double* 
   return psi;
 }
 
+// The same 9 incident patterns as incident_patterns(), written on the device
+// (column-major (9N, 9)): thread m owns rows 9m..9m+8 of every column, zeros
+// included, so the output needs no clearing.  The sub-cell coordinates and
+// their mean follow the host's arithmetic step by step; only cos/sin of the
+// phase come from the device's math library.
+__global__ __launch_bounds__(64) void k_res_incident(double2* __restrict__ psi, int n, double a,
+                                                    double k, double kh0, double kh1,
+                                                    double kh2) {
+  const int N = n * n * n, n9 = 9 * N;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  const double as = a / n;
+  auto coord = [&](int i) {
+    if (n == 1) return 0.0;
+    if (i == n - 1) return a - as;
+    return (-a + as) + i * ((a - as) - (-a + as)) / (n - 1);
+  };
+  double mean[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < N; ++q) {
+    mean[0] += coord(q / (n * n));
+    mean[1] += coord((q / n) % n);
+    mean[2] += coord(q % n);
+  }
+  for (int d = 0; d < 3; ++d) mean[d] /= N;
+  const double c0 = coord(m / (n * n)), c1 = coord((m / n) % n), c2 = coord(m % n);
+  const double ph = k * (c0 * kh0 + c1 * kh1 + c2 * kh2);
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  const double2 e = make_double2(cs, sn);
+  const double dx[3] = {c0 - mean[0], c1 - mean[1], c2 - mean[2]};
+  const int VP[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+  for (int col = 0; col < 9; ++col) {
+    double2 v[9];
+    for (int i = 0; i < 9; ++i) v[i] = make_double2(0.0, 0.0);
+    if (col < 3) {
+      v[col] = e;
+    } else {
+      const int al = col - 3, p = VP[al][0], q = VP[al][1];
+      double u[3] = {0.0, 0.0, 0.0};
+      if (p == q) {
+        u[p] = dx[p];
+      } else {
+        u[p] = 0.5 * dx[q];
+        u[q] = 0.5 * dx[p];
+      }
+      for (int i = 0; i < 3; ++i) v[i] = make_double2(u[i] * e.x, u[i] * e.y);
+      v[3 + al] = e;
+    }
+    double2* out = psi + (size_t)col * n9 + 9 * m;
+    for (int i = 0; i < 9; ++i) out[i] = v[i];
+  }
+}
+
 // Dynamic LDS of the elimination kernel for panel width nb.
 size_t lu_lds(int nb, int n, int nc) { return (size_t)nb * std::max(n, nc) * sizeof(double2); }
 constexpr size_t kLuLdsCap = 150 * 1024;  // leaves room for the static arrays (NB = 16)
@@ -643,5 +697,237 @@ extern "C" int msc_resonance_tmatrices(double omega, double a, double alpha, dou
   RES_CHECK(hipStreamSynchronize(st));
 #undef RES_CHECK
   cleanup();
+  return kOk;
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// No-allocation variant (msc_resonance_tmatrices_noalloc).
+//
+// Its device workspace is `scratch`, a file-scope device array of
+// kScratchBytes that every GPU the library runs on holds until the process
+// ends.  Host code never passes `scratch` itself (its host-side name is only a
+// handle): on the first call on a device, with that device current, the
+// entry point asks the runtime for the array's address on that device
+// (hipGetSymbolAddress) and keeps it per device; kernels, copies and fills
+// take that address.  Calls that overlap on one GPU share the array and race;
+// every call writes what it reads before reading it, so nothing of an earlier
+// call's contents is used.
+// ─────────────────────────────────────────────────────────────────────────────
+
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxChunk = 64;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_address(void** out) {
+  int dev = -1;
+  MSC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    msc::set_error("device ordinal out of range for the no-allocation scratch");
+    return msc::kUnsupported;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_addr[dev] == nullptr) {
+    void* p = nullptr;
+    MSC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) {
+      msc::set_error("hipGetSymbolAddress returned no address for the scratch array");
+      return msc::kHip;
+    }
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return msc::kOk;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Byte offsets of the workspace arrays inside the scratch for `chunk` systems.
+struct ScratchLayout {
+  size_t P, psi, tp, ratio, sing, A, t0, t1, acc, norm, act, total;
+};
+
+ScratchLayout scratch_layout(int n, int chunk, bool neumann) {
+  const size_t N = (size_t)n * n * n, n9 = 9 * N, nc = n9 + 9, no = 2 * n - 1;
+  const size_t c = (size_t)chunk;
+  ScratchLayout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  };
+  L.P = take(no * no * no * 81 * sizeof(double2));
+  L.psi = take(n9 * 9 * sizeof(double2));
+  L.tp = take(c * 4 * sizeof(double2));
+  L.ratio = take(c * sizeof(double));
+  L.sing = take(c * sizeof(int));
+  L.A = take(c * n9 * nc * sizeof(double2));
+  if (neumann) {
+    L.t0 = take(c * n9 * 9 * sizeof(double2));
+    L.t1 = take(c * n9 * 9 * sizeof(double2));
+    L.acc = take(c * n9 * 9 * sizeof(double2));
+    L.norm = take(c * 2 * sizeof(double));
+    L.act = take(c * sizeof(int));
+  }
+  L.total = off;
+  return L;
+}
+
+}  // namespace
+
+extern "C" int msc_resonance_tmatrices_noalloc(double omega, double a, double alpha, double beta,
+                                               double rho, int n_sub, const double* khat3,
+                                               int wave_s, int count, const double* dlambda,
+                                               const double* dmu, const double* drho,
+                                               int neumann_order, double neumann_tol,
+                                               void* tcomp, void* psi_exc, double* pivot_ratio,
+                                               int* neumann_iters, void* stream) {
+  set_error("");
+  MSC_REQUIRE(omega > 0.0 && a > 0.0 && alpha > 0.0 && beta > 0.0 && rho > 0.0,
+              "omega, a, alpha, beta, rho must be positive");
+  MSC_REQUIRE(n_sub >= 1, "n_sub must be >= 1");
+  MSC_REQUIRE(count >= 0, "count must be >= 0");
+  MSC_REQUIRE(count == 0 || (dlambda && dmu && drho && tcomp), "null contrast or output pointer");
+  MSC_REQUIRE(khat3 != nullptr, "khat must not be NULL");
+  MSC_REQUIRE(wave_s == 0 || wave_s == 1, "wave_type must be 0 (P) or 1 (S)");
+  MSC_REQUIRE(neumann_order >= 0, "neumann_order must be >= 0");
+  const double kn = std::sqrt(khat3[0] * khat3[0] + khat3[1] * khat3[1] + khat3[2] * khat3[2]);
+  MSC_REQUIRE(kn > 0.0, "khat must be non-zero");
+  const int n = n_sub, N = n * n * n, n9 = 9 * N, nc = n9 + 9;
+  int nb = 16;
+  while (nb > 1 && lu_lds(nb, n9, nc) > kLuLdsCap) nb >>= 1;
+  if (n_sub > 8 || lu_lds(nb, n9, nc) > kLuLdsCap) {
+    set_error("n_sub too large for the resonance solver (at most 8)");
+    return kUnsupported;
+  }
+  const bool neumann = neumann_order > 0;
+  if (scratch_layout(n, 1, neumann).total > kScratchBytes) {
+    set_error("n_sub = " + std::to_string(n_sub) +
+              " needs more than the 64 MiB no-allocation scratch for one cube (at most n_sub = 6); "
+              "use msc_resonance_tmatrices");
+    return kUnsupported;
+  }
+  if (count == 0) return kOk;
+  int chunk = std::min(count, kNoAllocMaxChunk);
+  while (chunk > 1 && scratch_layout(n, chunk, neumann).total > kScratchBytes) --chunk;
+  const ScratchLayout L = scratch_layout(n, chunk, neumann);
+  void* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  unsigned char* sb = static_cast<unsigned char*>(base);
+  double2* dP = reinterpret_cast<double2*>(sb + L.P);
+  double2* dpsi = reinterpret_cast<double2*>(sb + L.psi);
+  double2* dtp = reinterpret_cast<double2*>(sb + L.tp);
+  double* dratio = reinterpret_cast<double*>(sb + L.ratio);
+  int* dsing = reinterpret_cast<int*>(sb + L.sing);
+  double2* dA = reinterpret_cast<double2*>(sb + L.A);
+  double2* dt0 = neumann ? reinterpret_cast<double2*>(sb + L.t0) : nullptr;
+  double2* dt1 = neumann ? reinterpret_cast<double2*>(sb + L.t1) : nullptr;
+  double2* dacc = neumann ? reinterpret_cast<double2*>(sb + L.acc) : nullptr;
+  double* dnorm = neumann ? reinterpret_cast<double*>(sb + L.norm) : nullptr;
+  int* dact = neumann ? reinterpret_cast<int*>(sb + L.act) : nullptr;
+
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const double as = a / n, kP = omega / alpha, kS = omega / beta;
+  double coef[8];
+  rc = msc_cube_coefficients(omega, as, alpha, beta, rho, coef);
+  if (rc) return rc;
+  const size_t stride = (size_t)n9 * nc;
+  const int no = 2 * n - 1;
+  hipLaunchKernelGGL(k_res_incident, dim3((N + 63) / 64), dim3(64), 0, st, dpsi, n, a,
+                     wave_s ? kS : kP, khat3[0] / kn, khat3[1] / kn, khat3[2] / kn);
+  MSC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_res_offsets, dim3((no * no * no + 63) / 64), dim3(64), 0, st, dP, n,
+                     2.0 * as, kP, kS, 1.0 / (4.0 * M_PI * rho * omega * omega));
+  MSC_LAUNCH_CHECK();
+  double hratio[kNoAllocMaxChunk], hnorm[2 * kNoAllocMaxChunk];
+  int hsing[kNoAllocMaxChunk], conv[kNoAllocMaxChunk], act[kNoAllocMaxChunk];
+  double2* out = static_cast<double2*>(tcomp);
+  double2* pex = static_cast<double2*>(psi_exc);
+  for (int b0 = 0; b0 < count; b0 += chunk) {
+    const int cb = std::min(chunk, count - b0);
+    rc = tparams_batch(dlambda + b0, dmu + b0, drho + b0, (size_t)cb, omega, coef,
+                       8.0 * as * as * as, dtp, st);
+    if (rc) return rc;
+    const int gx = (int)std::min<size_t>((stride + kRT - 1) / kRT, 2048);
+    hipLaunchKernelGGL(k_res_assemble, dim3(gx, cb), dim3(kRT), 0, st, dA, stride, dP, dtp, dpsi,
+                       n);
+    MSC_LAUNCH_CHECK();
+    if (!neumann) {
+      switch (nb) {
+        case 16: rc = launch_lu<16>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+        case 8: rc = launch_lu<8>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+        case 4: rc = launch_lu<4>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+        case 2: rc = launch_lu<2>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+        default: rc = launch_lu<1>(dA, stride, n9, nc, cb, dratio, dsing, st); break;
+      }
+      if (rc) return rc;
+      MSC_HIP_CHECK(hipMemcpyAsync(hratio, dratio, cb * sizeof(double), hipMemcpyDeviceToHost, st));
+      MSC_HIP_CHECK(hipMemcpyAsync(hsing, dsing, cb * sizeof(int), hipMemcpyDeviceToHost, st));
+      MSC_HIP_CHECK(hipStreamSynchronize(st));
+      for (int i = 0; i < cb; ++i) {
+        if (hsing[i]) {
+          set_error("the internal Foldy-Lax matrix is singular");
+          return kInvalid;
+        }
+        if (pivot_ratio) pivot_ratio[b0 + i] = hratio[i];
+        if (neumann_iters) neumann_iters[b0 + i] = -1;
+      }
+    } else {
+      for (int b = 0; b < cb; ++b) {
+        MSC_HIP_CHECK(hipMemcpyAsync(dt0 + (size_t)b * n9 * 9, dpsi,
+                                     (size_t)n9 * 9 * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        MSC_HIP_CHECK(hipMemcpyAsync(dacc + (size_t)b * n9 * 9, dpsi,
+                                     (size_t)n9 * 9 * sizeof(double2), hipMemcpyDeviceToDevice, st));
+      }
+      for (int i = 0; i < cb; ++i) {
+        conv[i] = -1;
+        act[i] = 1;
+      }
+      double2 *tin = dt0, *tout = dt1;
+      for (int k = 1; k <= neumann_order; ++k) {
+        MSC_HIP_CHECK(hipMemcpyAsync(dact, act, cb * sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, st, dnorm, 2 * cb);
+        const int gxn = (n9 * 9 + kRT - 1) / kRT;
+        hipLaunchKernelGGL(k_res_neumann, dim3(gxn, cb), dim3(kRT), 0, st, dA, stride, n9, tin,
+                           tout, dacc, dnorm, dact);
+        MSC_LAUNCH_CHECK();
+        MSC_HIP_CHECK(hipMemcpyAsync(hnorm, dnorm, 2 * cb * sizeof(double), hipMemcpyDeviceToHost,
+                                     st));
+        MSC_HIP_CHECK(hipStreamSynchronize(st));
+        bool all = true;
+        for (int i = 0; i < cb; ++i) {
+          const double rel = std::sqrt(hnorm[2 * i]) / (std::sqrt(hnorm[2 * i + 1]) + 1e-300);
+          if (conv[i] < 0 && act[i] && rel < neumann_tol) conv[i] = k;
+          if (conv[i] >= 0) act[i] = 0;
+          all = all && conv[i] >= 0;
+        }
+        std::swap(tin, tout);
+        if (all) break;
+      }
+      for (int b = 0; b < cb; ++b)
+        MSC_HIP_CHECK(hipMemcpyAsync(dA + (size_t)b * stride + (size_t)n9 * n9,
+                                     dacc + (size_t)b * n9 * 9, (size_t)n9 * 9 * sizeof(double2),
+                                     hipMemcpyDeviceToDevice, st));
+      for (int i = 0; i < cb; ++i) {
+        if (pivot_ratio) pivot_ratio[b0 + i] = NAN;
+        if (neumann_iters) neumann_iters[b0 + i] = conv[i];
+      }
+    }
+    hipLaunchKernelGGL(k_res_tcomp, dim3(cb), dim3(kRT), 0, st, dA, stride, dtp, n9,
+                       out + (size_t)b0 * 81, pex ? pex + (size_t)b0 * n9 * 9 : nullptr);
+    MSC_LAUNCH_CHECK();
+  }
+  MSC_HIP_CHECK(hipStreamSynchronize(st));
   return kOk;
 }
