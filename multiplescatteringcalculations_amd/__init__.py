@@ -20,6 +20,7 @@ from .slab_scattering import (
     SlabResult,
     compute_slab_scattering,
     compute_slab_tmatrices,
+    kennett_reference_rpp,
     random_iid_slab_material,
     random_slab_material,
     slab_reflected_field,
@@ -32,8 +33,23 @@ from .cpa_iteration import (
     Phase,
     compute_cpa,
     compute_cpa_two_phase,
+    phases_from_two_phase,
+    voigt_average,
 )
-from .kennett_layers import FluidLayer, IsotropicLayer, KennettResult, LayerStack
+from .kennett_layers import (
+    FluidLayer,
+    IsotropicLayer,
+    KennettResult,
+    LayerStack,
+    PSVCoefficients,
+    SHCoefficients,
+    cpa_stack_from_phases,
+    cubic_to_isotropic_layer,
+    kennett_layers,
+    psv_fluid_solid,
+    random_heterogeneous_stack,
+    random_velocity_stack,
+)
 from .learned_precond import LearnedPreconditioner, harvest_pairs, merge_pairs
 from .ocean_bottom import (
     OceanBottomConfig,
@@ -47,7 +63,12 @@ from .resonance_tmatrix import (
     ResonanceTmatrixResult,
     compute_resonance_tmatrices,
     compute_resonance_tmatrix,
+    scattering_order_decomposition,
+    strain_from_displacement_traction,
+    sub_cell_centres,
     suggest_n_subcells,
+    validate_rayleigh_limit,
+    voigt_tmatrix_from_resonance_result,
 )
 from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
 from .sphere_scattering_fft import (
@@ -56,7 +77,7 @@ from .sphere_scattering_fft import (
     compute_sphere_foldy_lax_fft,
     compute_sphere_foldy_lax_fft_gpu,
 )
-from .solver_config import load_config, run_from_config
+from .solver_config import ScatteringConfig, load_config, run_from_config, validate_config
 from .sweep import SweepCase, compute_slab_sweep
 from .torch_gmres import get_device, select_dtype, to_numpy, to_torch, torch_gmres
 
@@ -65,14 +86,18 @@ __all__ = [
     "effective_stiffness_voigt", "SlabGeometry", "SlabMaterial", "SlabPlan", "SlabResult",
     "compute_slab_scattering", "compute_slab_scattering_gpu", "compute_slab_tmatrices",
     "random_iid_slab_material", "random_slab_material", "slab_reflected_field",
-    "slab_rpp_periodic", "uniform_slab_material", "SlabOperator", "load_config",
-    "run_from_config", "get_device", "select_dtype", "to_numpy", "to_torch", "torch_gmres",
+    "slab_rpp_periodic", "kennett_reference_rpp", "uniform_slab_material", "SlabOperator",
+    "ScatteringConfig", "load_config", "run_from_config", "validate_config", "get_device", "select_dtype", "to_numpy", "to_torch", "torch_gmres",
     "LearnedPreconditioner", "harvest_pairs", "merge_pairs", "SweepCase", "compute_slab_sweep",
-    "FluidLayer", "IsotropicLayer", "KennettResult", "LayerStack",
+    "FluidLayer", "IsotropicLayer", "KennettResult", "LayerStack", "PSVCoefficients",
+    "SHCoefficients", "kennett_layers", "psv_fluid_solid", "cubic_to_isotropic_layer",
+    "cpa_stack_from_phases", "random_heterogeneous_stack", "random_velocity_stack",
     "OceanBottomConfig", "OceanBottomResult", "compute_ocean_bottom_reflection",
     "load_ocean_bottom_config", "ricker_source_spectrum", "write_log",
     "ResonanceTmatrixResult", "compute_resonance_tmatrices", "compute_resonance_tmatrix",
-    "suggest_n_subcells", "CPAResult", "CubicEffectiveMedium", "Phase", "compute_cpa",
-    "compute_cpa_two_phase", "SphereDecompositionResult", "compute_sphere_foldy_lax",
+    "suggest_n_subcells", "validate_rayleigh_limit", "scattering_order_decomposition",
+    "voigt_tmatrix_from_resonance_result", "sub_cell_centres", "strain_from_displacement_traction",
+    "CPAResult", "CubicEffectiveMedium", "Phase", "compute_cpa", "compute_cpa_two_phase",
+    "phases_from_two_phase", "voigt_average", "SphereDecompositionResult", "compute_sphere_foldy_lax",
     "compute_sphere_foldy_lax_fft", "compute_sphere_foldy_lax_fft_gpu",
 ]

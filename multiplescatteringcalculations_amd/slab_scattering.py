@@ -645,6 +645,30 @@ def slab_reflected_field(result: SlabResult, T_local: NDArray | None = None) -> 
     return R_PP, complex(np.dot(sv, u_S)), R_PP
 
 
+def kennett_reference_rpp(ref: ReferenceMedium, contrast: MaterialContrast, H: float,
+                          omega: float) -> complex:
+    """Normal-incidence R_PP of a uniform layer of thickness H (slab_scattering.py:738-783).
+
+    The layer (background moduli plus ``contrast``) sits between two half-spaces
+    of the background medium; the reflection is the Kennett recursion at p = 0
+    over that three-layer stack (the top layer's thickness does not enter R_PP
+    at its lower interface), evaluated by ``kennett_layers``.
+    """
+    from .kennett_layers import IsotropicLayer, LayerStack, kennett_layers
+
+    lam = ref.rho * (ref.alpha**2 - 2.0 * ref.beta**2) + contrast.Dlambda
+    mu = ref.rho * ref.beta**2 + contrast.Dmu
+    rho = ref.rho + contrast.Drho
+    layer = IsotropicLayer(alpha=float(np.sqrt((lam + 2.0 * mu) / rho)),
+                           beta=float(np.sqrt(mu / rho)), rho=rho, thickness=H)
+    stack = LayerStack(layers=[
+        IsotropicLayer(alpha=ref.alpha, beta=ref.beta, rho=ref.rho, thickness=100.0),
+        layer,
+        IsotropicLayer(alpha=ref.alpha, beta=ref.beta, rho=ref.rho, thickness=np.inf),
+    ])
+    return complex(kennett_layers(stack, p=0.0, omega=np.array([omega])).RPP[0])
+
+
 # ═══════════════════════════════════════════════════════════════
 #  Material generators (slab_scattering.py:791-852)
 # ═══════════════════════════════════════════════════════════════
@@ -698,7 +722,7 @@ def random_iid_slab_material(geometry: SlabGeometry, ref: ReferenceMedium, phi: 
 
 __all__ = [
     "SlabGeometry", "SlabMaterial", "SlabResult", "SlabPlan", "compute_slab_tmatrices",
-    "compute_slab_scattering", "slab_rpp_periodic", "slab_reflected_field",
+    "compute_slab_scattering", "slab_rpp_periodic", "slab_reflected_field", "kennett_reference_rpp",
     "random_slab_material", "uniform_slab_material", "random_iid_slab_material",
     "tparams_to_tmatrices", "tmatrices_to_tparams", "compute_cube_tmatrix",
     "sub_cell_tmatrix_9x9", "MaterialContrast", "ReferenceMedium",

@@ -8,7 +8,10 @@ import pytest
 
 import multiplescatteringcalculations_amd as msc
 from multiplescatteringcalculations_amd import cpa_iteration as cp
-from multiplescatteringcalculations_amd import kennett_layers as kl
+import multiplescatteringcalculations_amd.kennett_layers  # noqa: F401  (the package root exports the function of that name)
+import sys
+
+kl = sys.modules["multiplescatteringcalculations_amd.kennett_layers"]
 from oracle import cpa as ocpa
 from oracle import tmatrix as otm
 

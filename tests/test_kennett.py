@@ -9,7 +9,10 @@ and the homogeneous stack (cubic_scattering/tests/test_kennett_layers.py:
 import numpy as np
 import pytest
 
-from multiplescatteringcalculations_amd import kennett_layers as kl
+import multiplescatteringcalculations_amd.kennett_layers  # noqa: F401  (the package root exports the function of that name)
+import sys
+
+kl = sys.modules["multiplescatteringcalculations_amd.kennett_layers"]
 from oracle import kennett as ok
 from oracle import slab as osl
 from oracle import tmatrix as otm
@@ -179,3 +182,35 @@ def test_planted_sign_slip_in_oracle_is_caught():
     got = kl.psv_solid_solid(p, e1, n1, r1, c1, e2, n2, r2, c2)
     assert np.abs(got.Rd - Rd).max() < 1e-12
     assert np.abs(got.Rd - np.array([[Rd[0, 0], -Rd[0, 1]], [-Rd[1, 0], Rd[1, 1]]])).max() > 1e-3
+
+
+@pytest.mark.parametrize("H,omega,scale", [(8.0, 346.4, 0.1), (2.0, 692.8, -0.2), (32.0, 100.0, 0.4),
+                                           (0.5, 1500.0, 0.05)])
+def test_kennett_reference_rpp_matches_airy_layer(H, omega, scale):
+    """kennett_reference_rpp (slab_scattering.py:738-783) equals the oracle's
+    closed-form normal-incidence reflection of one uniform layer."""
+    import multiplescatteringcalculations_amd as msc
+    from oracle import slab as osl
+    from oracle import tmatrix as otm
+
+    r = msc.ReferenceMedium(3000.0, 1732.0, 2500.0)
+    c = msc.MaterialContrast(scale * r.lam, -0.5 * scale * r.mu, 0.7 * scale * r.rho)
+    got = msc.kennett_reference_rpp(r, c, H, omega)
+    ref = osl.kennett_layer_rpp_normal(otm.Medium(3000.0, 1732.0, 2500.0), c.Dlambda, c.Dmu,
+                                       c.Drho, H, omega)
+    assert abs(got - ref) <= 1e-12 * max(abs(ref), 1e-300)
+    assert msc.kennett_reference_rpp(r, msc.MaterialContrast(0.0, 0.0, 0.0), H, omega) == 0
+
+
+def test_top_level_names_of_the_reference_are_exported():
+    """The reference package's top-level names on this path
+    (cubic_scattering/__init__.py:138-307) import from the package root."""
+    import multiplescatteringcalculations_amd as msc
+
+    for name in ("ScatteringConfig", "validate_config", "load_config", "run_from_config",
+                 "kennett_layers", "kennett_reference_rpp", "cpa_stack_from_phases",
+                 "cubic_to_isotropic_layer", "psv_fluid_solid", "voigt_tmatrix_from_resonance_result",
+                 "validate_rayleigh_limit", "scattering_order_decomposition", "sub_cell_centres",
+                 "phases_from_two_phase", "voigt_average", "random_heterogeneous_stack",
+                 "random_velocity_stack", "PSVCoefficients", "SHCoefficients"):
+        assert hasattr(msc, name) and name in msc.__all__, name
