@@ -119,6 +119,19 @@ def time_solves(plan, psi0, precision, steps, warmup, precond=None, label=None):
     return times, st, x
 
 
+def rpp_of(plan, x, mat, omega):
+    """Specular R_PP (slab_rpp_periodic) of a device solution of the timed slab."""
+    from multiplescatteringcalculations_amd.slab_scattering import SlabResult, slab_rpp_periodic
+
+    g = plan.geometry
+    res = SlabResult(psi=None, psi0=None, geometry=g, material=mat, omega=omega,
+                     k_hat=np.array([np.sqrt(1.0 - (P_SLOW * VP) ** 2), P_SLOW * VP, 0.0]),
+                     wave_type="P", n_gmres_iter=0, gmres_residual=0.0)
+    layer_sum, _ = plan.source_sums(x, 0.0, 0.0, np.zeros(3))
+    res._sums = (layer_sum, None, (0.0, 0.0), np.zeros(3))
+    return slab_rpp_periodic(res, p=P_SLOW)
+
+
 def time_pass3(plan, psi0, precision, steps):
     """Per-launch duration of the block-multiply pass from HIP events recorded
     around each launch on its stream, over `steps` solves."""
@@ -293,31 +306,161 @@ def time_sphere(n_sub: int = 32):
     return out
 
 
-def cpu_baseline(mat, omega, n_matvec: int):
-    """The oracle's restatement of the reference CPU matvec (slab_scattering.py:
-    359-418: per-layer 2-D FFTs and the N_z^2 layer-pair 9x9 loop) timed on one
-    matvec of this workload and scaled by the plain solve's operator count.
-    The spectral table is a synthetic complex array of the reference's shape
-    (2N_z-1, 2M-1, 2M-1, 9, 9): the matvec's time does not depend on its values,
-    and building the real one takes minutes of Python loops."""
+def host_info():
+    """CPU model, core counts and thread settings of the host the CPU legs run on."""
+    import torch
+
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OPENBLAS_NUM_THREADS": os.environ.get("OPENBLAS_NUM_THREADS"),
+            "torch_threads": torch.get_num_threads()}
+
+
+def _median_time(fn, repeats=3):
+    fn()  # warm-up
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), [round(t, 4) for t in ts]
+
+
+def cpu_baseline(plan, n_matvec: int):
+    """CPU legs, all on this host's cores, 1 warm-up + 3 timed repeats (median):
+
+    * the oracle's restatement of the reference CPU matvec (slab_scattering.py:
+      359-418: per-layer 2-D FFTs and the N_z^2 layer-pair 9x9 loop, NumPy,
+      one core) on the 128 x 128 x 32 slab with its real per-voxel T0 (the
+      device's, expanded to 9x9) and a spectral table of the reference's shape
+      (2N_z-1, 2M-1, 2M-1, 9, 9) holding constant values (the loop's time does
+      not depend on them; the oracle builds the real one in minutes of Python
+      loops), scaled by the plain FP64 GPU solve's operator count: the
+      reported s/solve (setup excluded, labelled extrapolated);
+    * the full 32 x 32 x 8 solve of the oracle (ka 0.2, +-0.1, normal
+      incidence; SciPy GMRES(30) to 1e-8 from the Born guess) with its real
+      T0 and table: setup once, then the solve timed;
+    * the torch-CPU 3-D-FFT formula of slab_scattering_gpu.py:95-155 (fftn,
+      81-entry einsum, ifftn over the padded (9, 64, 256, 256) grid) at
+      torch.get_num_threads() threads, per matvec, scaled the same way.
+    """
+    import torch
+
+    from multiplescatteringcalculations_amd.slab_scattering import tparams_to_tmatrices
     from oracle import slab as osl
     from oracle import tmatrix as otm
 
+    info = host_info()
     rng = np.random.default_rng(1)
+    n = NZ * M * M * 9
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     H = 2 * M - 1
     kh = np.full((2 * NZ - 1, H, H, 9, 9), 1e-12 + 1e-13j, dtype=np.complex128)
-    ref = otm.Medium(VP, VS, RHO)
-    # per-voxel T0 through the oracle's closed form (one evaluation per inclusion)
-    T = np.zeros((NZ, M, M, 9, 9), dtype=complex)
-    idx = np.argwhere(mat.Drho != 0)
-    for (l, i, j) in idx[:64]:
-        T[l, i, j] = otm.tmatrix_9x9(otm.cube_tmatrix(omega, A, ref, mat.Dlambda[l, i, j],
-                                                      mat.Dmu[l, i, j], mat.Drho[l, i, j]), omega, A)
-    x = (rng.standard_normal(NZ * M * M * 9) + 1j * rng.standard_normal(NZ * M * M * 9))
+    T = tparams_to_tmatrices(plan.tparams().cpu().numpy())
+    t_np, reps_np = _median_time(lambda: osl.slab_matvec(x, T, kh, M, NZ))
+    del kh
+    log(f"  numpy 2-D matvec (128x128x32): median {t_np:.2f} s of {reps_np}")
+
+    # torch-CPU 3-D FFT formula (slab_scattering_gpu.py:95-155) on the padded grid
+    Pz, Pxy = plan.Pz, plan.Pxy
+    khat3 = torch.full((9, 9, Pz, Pxy, Pxy), 1e-12 + 1e-13j, dtype=torch.complex128)
+    Tt = torch.as_tensor(T)
+    xt = torch.as_tensor(x)
+
+    def torch_mv():
+        psi = xt.reshape(NZ, M, M, 9)
+        tau = torch.einsum("lmnab,lmnb->lmna", Tt, psi).permute(3, 0, 1, 2)
+        pad = torch.zeros((9, Pz, Pxy, Pxy), dtype=torch.complex128)
+        pad[:, :NZ, :M, :M] = tau
+        acc = torch.fft.ifftn(torch.einsum("ijzxy,jzxy->izxy", khat3,
+                                           torch.fft.fftn(pad, dim=(1, 2, 3))), dim=(1, 2, 3))
+        return psi - acc[:, :NZ, :M, :M].permute(1, 2, 3, 0)
+
+    t_th, reps_th = _median_time(torch_mv)
+    del khat3, Tt
+    log(f"  torch-CPU 3-D matvec ({torch.get_num_threads()} threads): median {t_th:.3f} s "
+        f"of {reps_th}")
+
+    # config-1 full solve of the oracle
+    import multiplescatteringcalculations_amd as msc
+
+    ref = msc.ReferenceMedium(VP, VS, RHO)
+    g1 = msc.SlabGeometry(M=32, N_z=8, a=A)
+    m1 = msc.random_iid_slab_material(g1, ref, 0.3, 0.1, seed=0)
+    om1 = 0.2 * VS / A
+    ref_o = otm.Medium(VP, VS, RHO)
     t0 = time.perf_counter()
-    osl.slab_matvec(x, T, kh, M, NZ)
-    t_mv = time.perf_counter() - t0
-    return t_mv, t_mv * n_matvec
+    T1 = osl.slab_tmatrices(32, 8, A, m1.Dlambda, m1.Dmu, m1.Drho, om1, ref_o)
+    kh1 = osl.slab_kernels(32, 8, A, om1, ref_o)
+    psi01 = osl.incident_field(32, 8, A, om1, ref_o, [1.0, 0.0, 0.0], "P").ravel()
+    t_setup1 = time.perf_counter() - t0
+    from scipy.sparse.linalg import LinearOperator, gmres
+
+    n1 = psi01.size
+    count = [0]
+
+    def mv1(v):
+        count[0] += 1
+        return osl.slab_matvec(v, T1, kh1, 32, 8)
+
+    def solve1():
+        count[0] = 0
+        gmres(LinearOperator((n1, n1), matvec=mv1, dtype=complex), psi01, x0=psi01.copy(),
+              rtol=TOL, atol=0.0, restart=RESTART, maxiter=50)
+
+    t_s1, reps_s1 = _median_time(solve1)
+    log(f"  oracle 32x32x8 solve: setup {t_setup1:.2f} s, solve median {t_s1:.3f} s "
+        f"({count[0]} matvecs)")
+    return {
+        "value": round(t_np * n_matvec, 2), "unit": "s/solve", "cores": 1, "kind": "port",
+        "sample": (f"1 warm-up + 3 timed CPU matvecs of the 128x128x32 slab (oracle restatement "
+                   f"of slab_scattering.py:359-418, NumPy, 1 core; median {t_np:.2f} s) x {n_matvec} "
+                   f"operator applications of the plain FP64 GPU solve = extrapolated s/solve, "
+                   f"setup excluded"),
+        "extrapolated": True,
+        "host": info,
+        "numpy_matvec_s": round(t_np, 3), "numpy_matvec_repeats_s": reps_np,
+        "torch_cpu_3d_matvec_s": round(t_th, 4), "torch_cpu_3d_repeats_s": reps_th,
+        "torch_cpu_3d_s_per_solve": round(t_th * n_matvec, 3),
+        "torch_cpu_3d_threads": info["torch_threads"],
+        "config1_32x32x8": {"setup_s": round(t_setup1, 3), "solve_s": round(t_s1, 4),
+                            "solve_repeats_s": reps_s1, "matvecs": count[0],
+                            "s_per_solve_incl_setup": round(t_setup1 + t_s1, 3)},
+    }
+
+
+def time_matvecs(plan, reps: int = 20):
+    """Per-operator-application time of the whole five-pass matvec (both
+    precisions) from torch events on the plan's stream."""
+    import torch
+
+    out = {}
+    for dt in (torch.complex64, torch.complex128):
+        x = torch.randn(plan.n, dtype=dt, device=plan.device)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            plan.matvec(x, y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.matvec(x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        out["c64" if dt == torch.complex64 else "c128"] = e0.elapsed_time(e1) / reps
+    return out
 
 
 def main():
@@ -358,6 +501,12 @@ def main():
                                  label="learned+mixed")
     diff = float((xmx - x64).abs().max() / x64.abs().max())
     diff_ln = float((xln - x64).abs().max() / x64.abs().max())
+    # R_PP of each timed solve against an FP64 solve to 1e-12
+    xt, stt = plan.gmres(psi0, "born", tol=1e-12, max_iter=600, restart=RESTART)
+    rpp_t = rpp_of(plan, xt, mat, omega)
+    rpp_err = {k: abs(rpp_of(plan, xx, mat, omega) - rpp_t) / abs(rpp_t)
+               for k, xx in (("fp64", x64), ("mixed", xmx), ("learned_mixed", xln))}
+    log(f"R_PP {rpp_t:.6e}; relative errors vs FP64 at 1e-12: {rpp_err}")
     log(f"tiles (y-fwd, x-fwd, z, x-inv, y-inv): {plan.tiles()}")
     timmx = time_pass3(plan, psi0, "mixed", args.steps)
     tim64 = time_pass3(plan, psi0, "fp64", args.steps)
@@ -372,11 +521,31 @@ def main():
     avg_ms = ms64 / max(n64, 1)
     achieved = bytes_pass3 / (avg_ms * 1e-3) / 1e9
     kname = f"k_z_mult<float,64,{plan.tiles()['c64'][2]}>"
+    # whole five-pass operator application against the per-matvec bytes of a
+    # dense-table formulation, B_mv = s (E P + 2 n + 4 N) with E = 81 or 45
+    # table entries per padded point, and against its compulsory bytes (psi
+    # in, y out, T0 and the compressed table once each)
+    mv = time_matvecs(plan)
+    P = plan.Pz * plan.Pxy * plan.Pxy
+    nvox = NZ * M * M
+    whole = {}
+    for key, sb in (("c64", 8), ("c128", 16)):
+        t = mv[key] * 1e-3
+        b81 = sb * (81 * P + 2 * 9 * nvox + 4 * nvox)
+        b45 = sb * (45 * P + 2 * 9 * nvox + 4 * nvox)
+        comp = sb * (2 * 9 * nvox + 4 * nvox) + nuv * 45 * nc * sb
+        whole[key] = {"ms": round(mv[key], 4), "B_mv_81_GB": round(b81 / 1e9, 3),
+                      "frac_B_mv_81": round(b81 / t / 8e12, 3),
+                      "B_mv_45_GB": round(b45 / 1e9, 3), "frac_B_mv_45": round(b45 / t / 8e12, 3),
+                      "compulsory_GB": round(comp / 1e9, 4),
+                      "frac_compulsory": round(comp / t / 8e12, 3)}
+    log(f"whole matvec: {whole}")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 4), "traffic": measured_traffic(kname),
                 "kernel": f"{kname} (z FFT + 9x9 block multiply "
                           "+ inverse z FFT), complex64 launches of the mixed solves",
-                "launches": n64, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_pass3}
+                "launches": n64, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_pass3,
+                "whole_matvec": whole}
 
     mean_mx = float(np.mean(tmx))
     mean_64 = float(np.mean(t64))
@@ -387,16 +556,9 @@ def main():
                                           "learned_strong_contrast": time_strong_contrast()}
     cpu = None
     if not args.no_cpu_baseline:
-        log("CPU baseline (oracle restatement of the reference CPU matvec, 1 matvec) ...")
-        n_mv = st64["matvecs_c128"]
-        t_mv, t_solve = cpu_baseline(mat, omega, n_mv)
-        cpu = {"value": round(t_solve, 2), "unit": "s/solve", "cores": 1, "kind": "port",
-               "sample": (f"1 CPU matvec of the 128x128x32 slab ({t_mv:.2f} s: numpy 2-D FFTs + "
-                          f"N_z^2 layer-pair einsum loop, synthetic table) x {n_mv} operator "
-                          f"applications of the plain FP64 GPU solve; setup excluded; host "
-                          f"{platform.processor() or platform.machine()}, {os.cpu_count()} cpus "
-                          f"visible, 1 used")}
-        log(f"  cpu matvec {t_mv:.2f} s -> {t_solve:.1f} s/solve (extrapolated)")
+        log("CPU baseline (oracle restatement of the reference CPU path) ...")
+        cpu = cpu_baseline(plan, st64["matvecs_c128"])
+        log(f"  -> {cpu['value']} s/solve (extrapolated)")
 
     out = {
         "metric": "s/solve to 1e-8 residual, 128\u00d7128\u00d732 slab: learned+mixed vs plain FP64 GMRES",
@@ -429,7 +591,11 @@ def main():
             "fp64_iterations": st64["iterations"], "mixed_iterations": stmx["iterations"],
             "mixed_matvecs_c64": stmx["matvecs_c64"], "mixed_matvecs_c128": stmx["matvecs_c128"],
             "fp64_rel_residual": st64["rel_residual"], "mixed_rel_residual": stmx["rel_residual"],
-            "mixed_vs_fp64_max_rel_diff": diff, "setup_s": round(t_setup, 3),
+            "mixed_vs_fp64_max_rel_diff": diff,
+            "rpp_tight_fp64": [rpp_t.real, rpp_t.imag], "rpp_rel_err": rpp_err,
+            "psi_rel_err_vs_tight": {k: float((xx - xt).abs().max() / xt.abs().max())
+                                     for k, xx in (("fp64", x64), ("mixed", xmx),
+                                                   ("learned_mixed", xln))}, "setup_s": round(t_setup, 3),
             "pass3_c128_avg_us": round(tim64["c128"][1] / max(tim64["c128"][0], 1) * 1e3, 2),
             "sweep_256_uneven_slabs_mixed": sweep,
             "other_paths": extras,

@@ -5,10 +5,14 @@
  * The network maps the induced source of a residual r to a correction:
  *     z = P(r) = r + N((s / t_max) T0 r) / s,   s = sqrt(n) / ||r||
  * (n = 9 N_z M^2, t_max = max over the slab of the 4 T0 parameters' moduli).
- * N is four 3x3x3 zero-padded convolutions 18 -> h -> h -> h -> 18 over the
- * (N_z, M, M) voxel grid with leaky-ReLU (slope 0.1) between them; its 18
- * channels are the real and imaginary parts of the 9 components, interleaved
- * (channel 2c = Re, 2c+1 = Im of component c).  Convolutions run as BF16
+ * N is either four 3x3x3 zero-padded convolutions 18 -> h -> h -> h -> 18 over
+ * the (N_z, M, M) voxel grid with leaky-ReLU (slope 0.1) between them
+ * (msc_net_create), or a four-level network (msc_net_create_unet) whose
+ * levels 1-3 are 2x, 4x, 8x coarser grids reached by 2x2x2 stride-2
+ * convolutions and left by 2x2x2 stride-2 transposed convolutions added to
+ * the finer level's activations.  Its 18 channels are the real and imaginary
+ * parts of the 9 components, interleaved (channel 2c = Re, 2c+1 = Im of
+ * component c).  Convolutions run as BF16
  * MFMA with FP32 accumulation; activations are BF16.  It is used as the right
  * preconditioner of flexible GMRES (msc_gmres_opts.precond), and is trained
  * on the GPU by msc_net_train_step from residual -> correction pairs.
@@ -17,8 +21,10 @@
  * (slab_scattering.py:532-552) or torch_gmres (torch_gmres.py:83-205), and
  * these entry points extend that solve.
  *
- * Parameters are one flat float32 vector, layer by layer: weights in
- * PyTorch's Conv3d layout [out][in][3][3][3] followed by the bias [out].
+ * Parameters are one flat float32 vector, layer by layer in forward order:
+ * weights [out][in][k][k][k] (k = 3, or 2 for the stride-2 layers; the
+ * kernel offset (kz, kx, ky) within a 2x2x2 block is 4 kz + 2 kx + ky)
+ * followed by the bias [out].
  * Training inputs are BF16 [sample][voxel][32] (msc_net_input; channels
  * 18..31 zero), targets float32 [sample][voxel][18].
  */
@@ -34,9 +40,19 @@ extern "C" {
 /* Create a network of hidden width 16, 32, 48 or 64 (parameters zero:
  * P(r) = r until parameters are set) on the current device. */
 int msc_net_create(msc_net** net, int hidden);
+
+/* Create the four-level network with level widths c4[0..3] (c4[0] = 16,
+ * c4[1..3] in {16, 32, 48}; parameters zero) on the current device.  Layers:
+ *   x0 = conv3(conv3(in)),  x1 = conv3(down(x0)),  x2 = conv3(down(x1)),
+ *   x3 = conv3(down(x2)),   y2 = conv3(up(x3) + x2),  y1 = conv3(up(y2) + x1),
+ *   out = conv3(conv3(up(y1) + x0)) -> 18 channels
+ * with leaky-ReLU after every conv3 and down layer except the last conv3;
+ * up layers are linear.  Level l has ceil(D / 2^l) x ceil(H / 2^l) x
+ * ceil(W / 2^l) voxels.  (16, 16, 32, 48) has 197,938 parameters. */
+int msc_net_create_unet(msc_net** net, const int* c4);
 int msc_net_destroy(msc_net* net);
 
-/* Number of float32 parameters (27*(18h + 2h^2 + 18h) + 3h + 18). */
+/* Number of float32 parameters (local network: 27*(18h + 2h^2 + 18h) + 3h + 18). */
 int msc_net_num_params(const msc_net* net, long long* n);
 
 /* Copy the flat float32 parameter vector in / out (device pointers). */

@@ -73,6 +73,7 @@ SIGNATURES = {
     "msc_timing_read": (c_int, [c_void_p, POINTER(c_double)]),
     # learned preconditioner (include/msc_precond.h)
     "msc_net_create": (c_int, [POINTER(c_void_p), c_int]),
+    "msc_net_create_unet": (c_int, [POINTER(c_void_p), POINTER(c_int)]),
     "msc_net_destroy": (c_int, [c_void_p]),
     "msc_net_num_params": (c_int, [c_void_p, POINTER(c_longlong)]),
     "msc_net_set_params": (c_int, [c_void_p, c_void_p, c_void_p]),

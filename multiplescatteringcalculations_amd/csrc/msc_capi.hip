@@ -1109,6 +1109,28 @@ int msc_net_create(msc_net** net, int hidden) {
   return kOk;
 }
 
+int msc_net_create_unet(msc_net** net, const int* c4) {
+  MSC_REQUIRE(net != nullptr && c4 != nullptr, "NULL argument");
+  *net = nullptr;
+  msc_net* h = new (std::nothrow) msc_net();
+  MSC_REQUIRE(h != nullptr, "out of host memory");
+  int rc = net_init_unet(h->n, c4);
+  if (!rc) rc = net_pack(h->n, nullptr);
+  if (!rc) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      set_error(std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+      rc = kHip;
+    }
+  }
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *net = h;
+  return kOk;
+}
+
 int msc_net_destroy(msc_net* net) {
   if (net == nullptr) return kOk;
   (void)hipDeviceSynchronize();

@@ -30,8 +30,14 @@ typedef short v8s __attribute__((ext_vector_type(8)));
 typedef __bf16 v8b __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kNetNT = 256;  // threads per workgroup (4 waves)
-constexpr int kVT = 4;       // 16-voxel tiles per wave
+#ifndef MSC_NET_VT
+#define MSC_NET_VT 4
+#endif
+#ifndef MSC_NET_ZC
+#define MSC_NET_ZC 4
+#endif
+constexpr int kNetNT = 256;         // threads per workgroup (4 waves)
+constexpr int kVT = MSC_NET_VT;     // 16-voxel tiles per wave of the row-tiled kernels
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   // round to nearest even
@@ -54,7 +60,8 @@ enum ConvEpi : int {
   kEpiLast = 1,      // out fp32 = acc + bias
   kEpiBackward = 2,  // out bf16 = acc * lrelu'(act)  (data gradient through act)
   kEpiApply64 = 3,   // out complex64  z = r + (acc + bias) / s  (preconditioner output)
-  kEpiApply128 = 4   // out complex128 z = r + (acc + bias) / s
+  kEpiApply128 = 4,  // out complex128 z = r + (acc + bias) / s
+  kEpiBackwardLin = 5  // out bf16 = acc  (data gradient into a linear node)
 };
 
 struct ConvArgs {
@@ -191,6 +198,9 @@ __device__ __forceinline__ void conv_store(const ConvArgs& a, const v4f (&acc)[N
             const float z = acc[o][t][i] + a.bias[c0 + i];
             r[i] = z > 0.f ? z : kNetSlope * z;
           }
+        } else if constexpr (EPI == kEpiBackwardLin) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) r[i] = acc[o][t][i];
         } else {
           const v4s av = *reinterpret_cast<const v4s*>(a.act + v * NO + c0);
 #pragma unroll
@@ -563,33 +573,58 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
   }
 }
 
-// Weight gradient of one tap (blockIdx.y): dW[o][i][tap] += sum_v g[v][o] in[v+off][i]
-// over the block's voxel range.  Both operands are staged transposed in LDS so
-// that a lane's four consecutive voxels are one 8-byte read.  The four waves
-// share the (NO/16) x (NI/16) output tiles; partial tiles are added to the
-// FP32 gradient with atomics (real channels only).
+// Weight gradient of one tap (blockIdx.y) over the block's voxel range.  Both
+// operands are staged transposed in LDS so that a lane's four consecutive
+// voxels are one 8-byte read.  The four waves share the (NO/16) x (NI/16)
+// output tiles; partial tiles are added to the FP32 gradient with atomics
+// (real channels only), dW laid out [o][i][tap].
+//   MODE 0 (3x3x3 conv, 27 taps): dW[o][i][tap] += sum_v g[v][o] in[v + off(tap)][i]
+//     over the voxels v of the (D, H, W) grid;
+//   MODE 1 (2x2x2 stride-2 down, 8 taps): dW[o][i][s] += sum_c g[c][o] in[f(c, s)][i]
+//     over the voxels c of the coarse (D, H, W) grid, `in` on the fine
+//     (Df, Hf, Wf) grid, f(c, s) = 2c + (s >> 2, (s >> 1) & 1, s & 1);
+//   MODE 2 (2x2x2 transposed up, 8 taps): dW[o][i][s] += sum_c g[f(c, s)][o] in[c][i]
+//     with g on the fine grid.
+// Fine voxels outside the fine grid contribute zero.
 constexpr int kWgVox = 64;       // voxels staged per iteration
 constexpr int kWgLd = kWgVox + 4;  // LDS row stride (elements)
 
-template <int NI, int NO>
+struct WgradGrid {
+  int D, H, W;     // the grid the voxel loop runs over (coarse for MODE 1, 2)
+  int Df, Hf, Wf;  // fine grid of MODE 1, 2
+};
+
+// Fine voxel f(c, s) of coarse voxel c, or -1 outside the fine grid.
+__device__ __forceinline__ long long fine_of(size_t c, int s, const WgradGrid& q) {
+  const int cw = (int)(c % q.W);
+  const size_t t = c / q.W;
+  const int ch = (int)(t % q.H), cd = (int)(t / q.H);
+  const int fd = 2 * cd + (s >> 2), fh = 2 * ch + ((s >> 1) & 1), fw = 2 * cw + (s & 1);
+  if (fd >= q.Df || fh >= q.Hf || fw >= q.Wf) return -1;
+  return ((long long)fd * q.Hf + fh) * q.Wf + fw;
+}
+
+template <int NI, int NO, int MODE = 0>
 __global__ __launch_bounds__(kNetNT) void k_conv_wgrad(const uint16_t* __restrict__ in,
                                                        const uint16_t* __restrict__ g,
                                                        float* __restrict__ dW, int cin, int cout,
-                                                       int D, int H, int W, int vox_per_block) {
+                                                       WgradGrid q, int vox_per_block) {
   constexpr int NOT = NO / 16, NIT = NI / 16, NTILE = NOT * NIT;
   constexpr int TPW = (NTILE + 3) / 4;
+  constexpr int TAPS = MODE == 0 ? 27 : 8;
   __shared__ __align__(16) uint16_t sg[NO * kWgLd];
   __shared__ __align__(16) uint16_t sx[NI * kWgLd];
   const int tap = blockIdx.y;
   const int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
+  const int D = q.D, H = q.H, W = q.W;
   const size_t nvox = (size_t)D * H * W;
   const size_t vbeg = (size_t)blockIdx.x * vox_per_block;
   const size_t vend = std::min(nvox, vbeg + (size_t)vox_per_block);
   v4f acc[TPW];
 #pragma unroll
-  for (int q = 0; q < TPW; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < TPW; ++k) acc[k] = v4f{0.f, 0.f, 0.f, 0.f};
   // staging moves 8 channels (16 bytes) per load and transposes them into LDS
   static_assert(NI % 8 == 0 && NO % 8 == 0, "channel counts must be multiples of 8");
   typedef short v8s __attribute__((ext_vector_type(8)));
@@ -598,7 +633,14 @@ __global__ __launch_bounds__(kNetNT) void k_conv_wgrad(const uint16_t* __restric
       const int v = e / (NO / 8), c = (e - v * (NO / 8)) * 8;
       const size_t gv = c0 + v;
       v8s val = v8s{0, 0, 0, 0, 0, 0, 0, 0};
-      if (gv < vend) val = *reinterpret_cast<const v8s*>(g + gv * NO + c);
+      if (gv < vend) {
+        if constexpr (MODE == 2) {
+          const long long f = fine_of(gv, tap, q);
+          if (f >= 0) val = *reinterpret_cast<const v8s*>(g + (size_t)f * NO + c);
+        } else {
+          val = *reinterpret_cast<const v8s*>(g + gv * NO + c);
+        }
+      }
 #pragma unroll
       for (int k = 0; k < 8; ++k) sg[(c + k) * kWgLd + v] = (uint16_t)val[k];
     }
@@ -607,19 +649,26 @@ __global__ __launch_bounds__(kNetNT) void k_conv_wgrad(const uint16_t* __restric
       const size_t gv = c0 + v;
       v8s val = v8s{0, 0, 0, 0, 0, 0, 0, 0};
       if (gv < vend) {
-        const int gw = (int)(gv % W);
-        const size_t q = gv / W;
-        const int nh = (int)(q % H) + dx, nd = (int)(q / H) + dz, nw = gw + dy;
-        if (nd >= 0 && nd < D && nh >= 0 && nh < H && nw >= 0 && nw < W)
-          val = *reinterpret_cast<const v8s*>(in + (((size_t)nd * H + nh) * W + nw) * NI + c);
+        if constexpr (MODE == 0) {
+          const int gw = (int)(gv % W);
+          const size_t t = gv / W;
+          const int nh = (int)(t % H) + dx, nd = (int)(t / H) + dz, nw = gw + dy;
+          if (nd >= 0 && nd < D && nh >= 0 && nh < H && nw >= 0 && nw < W)
+            val = *reinterpret_cast<const v8s*>(in + (((size_t)nd * H + nh) * W + nw) * NI + c);
+        } else if constexpr (MODE == 1) {
+          const long long f = fine_of(gv, tap, q);
+          if (f >= 0) val = *reinterpret_cast<const v8s*>(in + (size_t)f * NI + c);
+        } else {
+          val = *reinterpret_cast<const v8s*>(in + gv * NI + c);
+        }
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k) sx[(c + k) * kWgLd + v] = (uint16_t)val[k];
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < TPW; ++q) {
-      const int tile = wave + 4 * q;
+    for (int k = 0; k < TPW; ++k) {
+      const int tile = wave + 4 * k;
       if (tile < NTILE) {
         const int ot = tile / NIT, it = tile - (tile / NIT) * NIT;
 #pragma unroll
@@ -628,23 +677,226 @@ __global__ __launch_bounds__(kNetNT) void k_conv_wgrad(const uint16_t* __restric
               *reinterpret_cast<const v4s*>(&sg[(ot * 16 + col) * kWgLd + ks * 16 + grp * 4]);
           const v4s b =
               *reinterpret_cast<const v4s*>(&sx[(it * 16 + col) * kWgLd + ks * 16 + grp * 4]);
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[k], 0, 0, 0);
         }
       }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < TPW; ++q) {
-    const int tile = wave + 4 * q;
+  for (int k = 0; k < TPW; ++k) {
+    const int tile = wave + 4 * k;
     if (tile >= NTILE) continue;
     const int ot = tile / NIT, it = tile - (tile / NIT) * NIT;
     const int ci = it * 16 + col;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int co = ot * 16 + grp * 4 + r;
-      if (co < cout && ci < cin) atomicAdd(&dW[((size_t)co * cin + ci) * 27 + tap], acc[q][r]);
+      if (co < cout && ci < cin)
+        atomicAdd(&dW[((size_t)co * cin + ci) * TAPS + tap], acc[k][r]);
     }
+  }
+}
+
+// ── small-grid convolution and the 2x2x2 stride-2 down / transposed up layers ──
+// On the coarse levels of the multi-resolution network a grid has a few
+// hundred to a few ten thousand voxels: the halo-staged row tiles above would
+// leave most of the chip idle.  These kernels give each wave one tile of 16
+// consecutive voxels (flattened over the whole grid) and one 16-channel output
+// block, and load the B fragments straight from global memory (the grids are
+// L2-resident): lane l holds voxel v0 + l%16 and channels 4(l/16)..+3 of each
+// 16-channel input block, as v_mfma_f32_16x16x16_bf16 expects.
+
+struct DirArgs {
+  const uint16_t* in;    // [input voxels][NI]
+  const uint16_t* w;     // [taps][NO][NI] bf16
+  const float* bias;     // [NO] (forward epilogues)
+  const uint16_t* skip;  // up layers: [fine voxels][NO] added to the output (or NULL)
+  const uint16_t* act;   // gradient epilogues: [output voxels][NO]
+  uint16_t* out;         // [output voxels][NO]
+  const int* done;       // early exit while *done (may be NULL)
+  int D, H, W;           // grid of the convolution / coarse grid of a resampling layer
+  int Df, Hf, Wf;        // fine grid of a resampling layer
+};
+
+enum DirEpi : int {
+  kDirHidden = 0,    // out = lrelu(acc + bias)
+  kDirBackward = 1,  // out = acc * lrelu'(act)
+  kDirBackLin = 2,   // out = acc
+  kDirUp = 3,        // out = acc + bias + skip   (transposed up layer)
+  kDirDownBack = 4   // out = (acc + skip) * lrelu'(act)  (data gradient of a down layer)
+};
+
+// Epilogue of 4 channels c0..c0+3 of output voxel v.
+template <int NO, int EPI>
+__device__ __forceinline__ void dir_store(const DirArgs& a, const v4f& acc, size_t v, int c0) {
+  float r[4];
+  if constexpr (EPI == kDirHidden || EPI == kDirUp) {
+    float sk[4] = {0.f, 0.f, 0.f, 0.f};
+    if (EPI == kDirUp && a.skip != nullptr) {
+      const v4s sv = *reinterpret_cast<const v4s*>(a.skip + v * NO + c0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sk[i] = bf2f((uint16_t)sv[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float z = acc[i] + a.bias[c0 + i];
+      r[i] = EPI == kDirUp ? z + sk[i] : (z > 0.f ? z : kNetSlope * z);
+    }
+  } else if constexpr (EPI == kDirBackLin) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = acc[i];
+  } else {
+    float sk[4] = {0.f, 0.f, 0.f, 0.f};
+    if (EPI == kDirDownBack && a.skip != nullptr) {
+      const v4s sv = *reinterpret_cast<const v4s*>(a.skip + v * NO + c0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sk[i] = bf2f((uint16_t)sv[i]);
+    }
+    const v4s av = *reinterpret_cast<const v4s*>(a.act + v * NO + c0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float gsum = acc[i] + sk[i];
+      r[i] = bf2f((uint16_t)av[i]) > 0.f ? gsum : kNetSlope * gsum;
+    }
+  }
+  v4s pk;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pk[i] = (short)f2bf(r[i]);
+  *reinterpret_cast<v4s*>(a.out + v * NO + c0) = pk;
+}
+
+// 3x3x3 "same" convolution (MIRROR: negated taps, the data gradient with
+// transposed weights); one wave per (16-voxel tile, 16-channel output block).
+template <int NI, int NO, int EPI, bool MIRROR>
+__global__ __launch_bounds__(kNetNT) void k_conv_dir(DirArgs a) {
+  if (a.done != nullptr && *a.done) return;
+  constexpr int NOT = NO / 16, NIB = NI / 16;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const size_t nv = (size_t)a.D * a.H * a.W;
+  const size_t ntile = (nv + 15) / 16;
+  const size_t wid = (size_t)blockIdx.x * (kNetNT / 64) + (threadIdx.x >> 6);
+  if (wid >= ntile * NOT) return;
+  const int ob = (int)(wid % NOT);
+  const size_t v = (wid / NOT) * 16 + col;
+  const bool live = v < nv;
+  int vw = 0, vh = 0, vd = 0;
+  if (live) {
+    vw = (int)(v % a.W);
+    const size_t t = v / a.W;
+    vh = (int)(t % a.H);
+    vd = (int)(t / a.H);
+  }
+  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 9
+  for (int tap = 0; tap < 27; ++tap) {
+    int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
+    if (MIRROR) {
+      dz = -dz;
+      dx = -dx;
+      dy = -dy;
+    }
+    const int nd = vd + dz, nh = vh + dx, nw = vw + dy;
+    const bool ok = live && nd >= 0 && nd < a.D && nh >= 0 && nh < a.H && nw >= 0 && nw < a.W;
+    const uint16_t* src = a.in + (((size_t)(ok ? nd : 0) * a.H + (ok ? nh : 0)) * a.W +
+                                  (ok ? nw : 0)) * NI + grp * 4;
+    const uint16_t* wsrc = a.w + ((size_t)tap * NO + ob * 16 + col) * NI + grp * 4;
+#pragma unroll
+    for (int ib = 0; ib < NIB; ++ib) {
+      v4s bf = v4s{0, 0, 0, 0};
+      if (ok) bf = *reinterpret_cast<const v4s*>(src + ib * 16);
+      const v4s af = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf, acc, 0, 0, 0);
+    }
+  }
+  // D[4 grp + i][col]: output channel ob*16 + 4 grp + i of voxel v
+  if (live) dir_store<NO, EPI>(a, acc, v, ob * 16 + grp * 4);
+}
+
+// Fine voxel f(c, s) of coarse voxel c (coords cd, ch, cw), or -1 outside.
+__device__ __forceinline__ long long fine_at(int cd, int ch, int cw, int s, const DirArgs& a) {
+  const int fd = 2 * cd + (s >> 2), fh = 2 * ch + ((s >> 1) & 1), fw = 2 * cw + (s & 1);
+  if (fd >= a.Df || fh >= a.Hf || fw >= a.Wf) return -1;
+  return ((long long)fd * a.Hf + fh) * a.Wf + fw;
+}
+
+// fine -> coarse: out[c][o] = epi(sum_{s, i} w[s][o][i] in[f(c, s)][i]); EPI
+// kDirHidden (a down layer) or kDirBackward (the data gradient of an up layer).
+template <int NI, int NO, int EPI>
+__global__ __launch_bounds__(kNetNT) void k_down_mfma(DirArgs a) {
+  if (a.done != nullptr && *a.done) return;
+  constexpr int NOT = NO / 16, NIB = NI / 16;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const size_t nc = (size_t)a.D * a.H * a.W;
+  const size_t ntile = (nc + 15) / 16;
+  const size_t wid = (size_t)blockIdx.x * (kNetNT / 64) + (threadIdx.x >> 6);
+  if (wid >= ntile * NOT) return;
+  const int ob = (int)(wid % NOT);
+  const size_t c = (wid / NOT) * 16 + col;
+  const bool live = c < nc;
+  int cw = 0, ch = 0, cd = 0;
+  if (live) {
+    cw = (int)(c % a.W);
+    const size_t t = c / a.W;
+    ch = (int)(t % a.H);
+    cd = (int)(t / a.H);
+  }
+  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const long long f = live ? fine_at(cd, ch, cw, s, a) : -1;
+    const uint16_t* src = a.in + (size_t)(f >= 0 ? f : 0) * NI + grp * 4;
+    const uint16_t* wsrc = a.w + ((size_t)s * NO + ob * 16 + col) * NI + grp * 4;
+#pragma unroll
+    for (int ib = 0; ib < NIB; ++ib) {
+      v4s bf = v4s{0, 0, 0, 0};
+      if (f >= 0) bf = *reinterpret_cast<const v4s*>(src + ib * 16);
+      const v4s af = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf, acc, 0, 0, 0);
+    }
+  }
+  if (live) dir_store<NO, EPI>(a, acc, c, ob * 16 + grp * 4);
+}
+
+// coarse -> fine: out[f(c, s)][o] = epi(sum_i w[s][o][i] in[c][i]); EPI
+// kDirUp (an up layer) or kDirDownBack (the data gradient of a down layer).
+// The wave's 16 coarse voxels are loaded once and reused for all 8 s.
+template <int NI, int NO, int EPI>
+__global__ __launch_bounds__(kNetNT) void k_up_mfma(DirArgs a) {
+  if (a.done != nullptr && *a.done) return;
+  constexpr int NOT = NO / 16, NIB = NI / 16;
+  const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const size_t nc = (size_t)a.D * a.H * a.W;
+  const size_t ntile = (nc + 15) / 16;
+  const size_t wid = (size_t)blockIdx.x * (kNetNT / 64) + (threadIdx.x >> 6);
+  if (wid >= ntile * NOT) return;
+  const int ob = (int)(wid % NOT);
+  const size_t c = (wid / NOT) * 16 + col;
+  const bool live = c < nc;
+  int cw = 0, ch = 0, cd = 0;
+  if (live) {
+    cw = (int)(c % a.W);
+    const size_t t = c / a.W;
+    ch = (int)(t % a.H);
+    cd = (int)(t / a.H);
+  }
+  v4s bf[NIB];
+#pragma unroll
+  for (int ib = 0; ib < NIB; ++ib) {
+    bf[ib] = v4s{0, 0, 0, 0};
+    if (live) bf[ib] = *reinterpret_cast<const v4s*>(a.in + c * NI + ib * 16 + grp * 4);
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* wsrc = a.w + ((size_t)s * NO + ob * 16 + col) * NI + grp * 4;
+#pragma unroll
+    for (int ib = 0; ib < NIB; ++ib) {
+      const v4s af = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[ib], acc, 0, 0, 0);
+    }
+    const long long f = live ? fine_at(cd, ch, cw, s, a) : -1;
+    if (f >= 0) dir_store<NO, EPI>(a, acc, (size_t)f, ob * 16 + grp * 4);
   }
 }
 
@@ -743,21 +995,22 @@ __global__ __launch_bounds__(kNetNT) void k_adam(float* __restrict__ p, float* _
   }
 }
 
-// Pack layer parameters (fp32 W [cout][cin][27], b [cout]) into BF16
-// wpk [27][pout][pin], wtr [27][pin][pout] and fp32 bpk [pout] (zero padded).
+// Pack layer parameters (fp32 W [cout][cin][taps], b [cout]) into BF16
+// wpk [taps][pout][pin], wtr [taps][pin][pout] and fp32 bpk [pout] (zero padded).
 __global__ __launch_bounds__(kNetNT) void k_pack(const float* __restrict__ W,
                                                  const float* __restrict__ b, int cin, int cout,
-                                                 int pin, int pout, uint16_t* __restrict__ wpk,
+                                                 int pin, int pout, int taps,
+                                                 uint16_t* __restrict__ wpk,
                                                  uint16_t* __restrict__ wtr,
                                                  float* __restrict__ bpk) {
-  const size_t tot = (size_t)27 * pout * pin;
+  const size_t tot = (size_t)taps * pout * pin;
   for (size_t e = (size_t)blockIdx.x * kNetNT + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * kNetNT) {
     const int i = (int)(e % pin);
     const size_t q = e / pin;
     const int o = (int)(q % pout);
     const int tap = (int)(q / pout);
-    const float val = (o < cout && i < cin) ? W[((size_t)o * cin + i) * 27 + tap] : 0.f;
+    const float val = (o < cout && i < cin) ? W[((size_t)o * cin + i) * taps + tap] : 0.f;
     const uint16_t bv = f2bf(val);
     wpk[e] = bv;
     wtr[((size_t)tap * pin + i) * pout + o] = bv;
@@ -851,13 +1104,11 @@ Net::~Net() {
 }
 
 void Net::release_grid() {
-  for (int l = 0; l < kNetLayers; ++l) {
-    if (act[l]) (void)hipFree(act[l]);
-    act[l] = nullptr;
-  }
-  for (int i = 0; i < 2; ++i) {
-    if (gbuf[i]) (void)hipFree(gbuf[i]);
-    gbuf[i] = nullptr;
+  for (int t = 0; t < kNetMaxTensors; ++t) {
+    if (act[t]) (void)hipFree(act[t]);
+    if (grad[t]) (void)hipFree(grad[t]);
+    act[t] = nullptr;
+    grad[t] = nullptr;
   }
   if (outf) (void)hipFree(outf);
   outf = nullptr;
@@ -871,45 +1122,73 @@ int Net::ensure_grid(int D_, int H_, int W_) {
   if (D_ == D && H_ == H && W_ == W && outf != nullptr) return kOk;
   release_grid();
   gen = ++g_net_gen;
-  const size_t nv = (size_t)D_ * H_ * W_;
-  for (int l = 1; l < kNetLayers; ++l)
-    MSC_HIP_CHECK(hipMalloc(&act[l], std::max<size_t>(nv, 1) * pin[l] * sizeof(uint16_t)));
-  const int widest = std::max(hidden, kNetIOPad);
-  for (int i = 0; i < 2; ++i)
-    MSC_HIP_CHECK(hipMalloc(&gbuf[i], std::max<size_t>(nv, 1) * widest * sizeof(uint16_t)));
-  MSC_HIP_CHECK(hipMalloc(&outf, std::max<size_t>(nv, 1) * kNetIOPad * sizeof(float)));
+  LD[0] = D_;
+  LH[0] = H_;
+  LW[0] = W_;
+  for (int l = 1; l < kNetLevels; ++l) {
+    LD[l] = (LD[l - 1] + 1) / 2;
+    LH[l] = (LH[l - 1] + 1) / 2;
+    LW[l] = (LW[l - 1] + 1) / 2;
+  }
+  for (int l = 0; l < kNetLevels; ++l) lvox[l] = (size_t)LD[l] * LH[l] * LW[l];
+  const int last = ntensors - 1;
+  for (int t = 1; t < ntensors; ++t) {
+    const size_t bytes = std::max<size_t>(lvox[T[t].level], 1) * T[t].ch * sizeof(uint16_t);
+    if (t != last) MSC_HIP_CHECK(hipMalloc(&act[t], bytes));
+    MSC_HIP_CHECK(hipMalloc(&grad[t], bytes));
+  }
+  MSC_HIP_CHECK(hipMalloc(&outf, std::max<size_t>(lvox[0], 1) * kNetIOPad * sizeof(float)));
   D = D_;
   H = H_;
   W = W_;
-  nvox = nv;
+  nvox = lvox[0];
   return kOk;
 }
 
-int net_init(Net& net, int hidden) {
-  MSC_REQUIRE(hidden == 16 || hidden == 32 || hidden == 48 || hidden == 64,
-              "hidden width must be 16, 32, 48 or 64");
-  net.hidden = hidden;
-  net.gen = ++g_net_gen;
-  const int real[kNetLayers + 1] = {kNetIO, hidden, hidden, hidden, kNetIO};
-  const int pad[kNetLayers + 1] = {kNetIOPad, hidden, hidden, hidden, kNetIOPad};
+namespace {
+
+int pad16(int c) { return (c + 15) / 16 * 16; }
+
+// Appends a layer (and its output tensor) to the graph; returns the output tensor id.
+int add_layer(Net& net, int kind, int in, int cout, bool act, int skip = -1) {
+  const int k = net.nlayers++;
+  NetLayer& l = net.L[k];
+  l.kind = kind;
+  l.in = in;
+  l.out = net.ntensors++;
+  l.skip = skip;
+  l.cin = in == 0 ? kNetIO : net.T[in].ch;
+  l.cout = cout;
+  l.pin = in == 0 ? kNetIOPad : net.T[in].ch;
+  l.pout = cout == kNetIO ? kNetIOPad : pad16(cout);
+  l.taps = kind == kConv3 ? 27 : 8;
+  l.act = act;
+  const int lvl = in == 0 ? 0 : net.T[in].level;
+  NetTensor& t = net.T[l.out];
+  t.level = kind == kDown2 ? lvl + 1 : (kind == kUp2 ? lvl - 1 : lvl);
+  t.ch = l.pout;
+  t.act = act;
+  if (skip >= 0) net.T[skip].skip_user = k;
+  return l.out;
+}
+
+int alloc_params(Net& net) {
   size_t off = 0, wp = 0, bp = 0;
-  for (int l = 0; l < kNetLayers; ++l) {
-    net.cin[l] = real[l];
-    net.cout[l] = real[l + 1];
-    net.pin[l] = pad[l];
-    net.pout[l] = pad[l + 1];
-    net.woff[l] = off;
-    off += (size_t)27 * net.cin[l] * net.cout[l];
-    net.boff[l] = off;
-    off += net.cout[l];
-    net.wpad_off[l] = wp;
-    wp += (size_t)27 * net.pin[l] * net.pout[l];
-    net.bpad_off[l] = bp;
-    bp += net.pout[l];
+  for (int k = 0; k < net.nlayers; ++k) {
+    NetLayer& l = net.L[k];
+    l.woff = off;
+    off += (size_t)l.taps * l.cin * l.cout;
+    l.boff = off;
+    off += l.cout;
+    l.wpad_off = wp;
+    wp += (size_t)l.taps * l.pin * l.pout;
+    l.bpad_off = bp;
+    bp += l.pout;
   }
   net.nparams = off;
   net.nwpad = wp;
   net.nbpad = bp;
+  net.gen = ++g_net_gen;
   MSC_HIP_CHECK(hipGetDevice(&net.device));
   MSC_HIP_CHECK(hipMalloc(&net.params, off * sizeof(float)));
   MSC_HIP_CHECK(hipMalloc(&net.adam_m, off * sizeof(float)));
@@ -927,13 +1206,57 @@ int net_init(Net& net, int hidden) {
   return kOk;
 }
 
+}  // namespace
+
+int net_init(Net& net, int hidden) {
+  MSC_REQUIRE(hidden == 16 || hidden == 32 || hidden == 48 || hidden == 64,
+              "hidden width must be 16, 32, 48 or 64");
+  net.arch = kArchLocal;
+  net.hidden = hidden;
+  net.nlayers = 0;
+  net.ntensors = 1;
+  int t = add_layer(net, kConv3, 0, hidden, true);
+  t = add_layer(net, kConv3, t, hidden, true);
+  t = add_layer(net, kConv3, t, hidden, true);
+  add_layer(net, kConv3, t, kNetIO, false);
+  return alloc_params(net);
+}
+
+int net_init_unet(Net& net, const int* c) {
+  MSC_REQUIRE(c != nullptr, "widths must not be NULL");
+  MSC_REQUIRE(c[0] == 16, "the full-resolution width c0 must be 16");
+  for (int l = 1; l < kNetLevels; ++l)
+    MSC_REQUIRE(c[l] == 16 || c[l] == 32 || c[l] == 48, "level widths c1..c3 must be 16, 32 or 48");
+  net.arch = kArchUNet;
+  net.hidden = c[0];
+  for (int l = 0; l < kNetLevels; ++l) net.widths[l] = c[l];
+  net.nlayers = 0;
+  net.ntensors = 1;
+  int x0 = add_layer(net, kConv3, 0, c[0], true);
+  x0 = add_layer(net, kConv3, x0, c[0], true);
+  int x1 = add_layer(net, kDown2, x0, c[1], true);
+  x1 = add_layer(net, kConv3, x1, c[1], true);
+  int x2 = add_layer(net, kDown2, x1, c[2], true);
+  x2 = add_layer(net, kConv3, x2, c[2], true);
+  int x3 = add_layer(net, kDown2, x2, c[3], true);
+  x3 = add_layer(net, kConv3, x3, c[3], true);
+  int y = add_layer(net, kUp2, x3, c[2], false, x2);
+  y = add_layer(net, kConv3, y, c[2], true);
+  y = add_layer(net, kUp2, y, c[1], false, x1);
+  y = add_layer(net, kConv3, y, c[1], true);
+  y = add_layer(net, kUp2, y, c[0], false, x0);
+  y = add_layer(net, kConv3, y, c[0], true);
+  add_layer(net, kConv3, y, kNetIO, false);
+  return alloc_params(net);
+}
+
 int net_pack(Net& net, hipStream_t s) {
-  for (int l = 0; l < kNetLayers; ++l) {
-    const size_t tot = (size_t)27 * net.pin[l] * net.pout[l];
-    hipLaunchKernelGGL(k_pack, dim3(grid_of(tot)), dim3(kNetNT), 0, s, net.params + net.woff[l],
-                       net.params + net.boff[l], net.cin[l], net.cout[l], net.pin[l],
-                       net.pout[l], net.wpk + net.wpad_off[l], net.wtr + net.wpad_off[l],
-                       net.bpk + net.bpad_off[l]);
+  for (int k = 0; k < net.nlayers; ++k) {
+    const NetLayer& l = net.L[k];
+    const size_t tot = (size_t)l.taps * l.pin * l.pout;
+    hipLaunchKernelGGL(k_pack, dim3(grid_of(tot)), dim3(kNetNT), 0, s, net.params + l.woff,
+                       net.params + l.boff, l.cin, l.cout, l.pin, l.pout, l.taps,
+                       net.wpk + l.wpad_off, net.wtr + l.wpad_off, net.bpk + l.bpad_off);
   }
   MSC_LAUNCH_CHECK();
   net.packed = true;
@@ -942,8 +1265,13 @@ int net_pack(Net& net, hipStream_t s) {
 
 namespace {
 
-ConvArgs conv_args(const Net& net, const uint16_t* in, const uint16_t* w, const float* bias,
-                   const uint16_t* act, void* out, const int* done) {
+// Planes per workgroup of the z-marching forward convolution: 4 stages 1.5
+// input planes per output plane and leaves 512 workgroups on a 128x128x32
+// grid (2 and 8 planes measured slower there).
+constexpr int kConvZc = MSC_NET_ZC;
+
+ConvArgs conv_args(const Net& net, int lvl, const uint16_t* in, const uint16_t* w,
+                   const float* bias, const uint16_t* act, void* out, const int* done) {
   ConvArgs a{};
   a.in = in;
   a.w = w;
@@ -951,190 +1279,266 @@ ConvArgs conv_args(const Net& net, const uint16_t* in, const uint16_t* w, const 
   a.act = act;
   a.out = out;
   a.done = done;
-  a.D = net.D;
-  a.H = net.H;
-  a.W = net.W;
+  a.D = net.LD[lvl];
+  a.H = net.LH[lvl];
+  a.W = net.LW[lvl];
   return a;
 }
 
-dim3 conv_grid(const Net& net) {
-  const int nbw = (net.W + kTW - 1) / kTW, nbh = (net.H + kTH - 1) / kTH;
-  return dim3((unsigned)(net.D * nbh * nbw));
+dim3 conv_grid(const ConvArgs& a) {
+  const int nbw = (a.W + kTW - 1) / kTW, nbh = (a.H + kTH - 1) / kTH;
+  return dim3((unsigned)(a.D * nbh * nbw));
 }
 
-// Planes per workgroup of the z-marching forward convolution: 4 stages 1.5
-// input planes per output plane and leaves 512 workgroups on a 128x128x32
-// grid (2 and 8 planes measured slower there).
-constexpr int kConvZc = 4;
-int conv_zc() { return kConvZc; }
-
-dim3 conv_grid_zm(const Net& net, int zc) {
-  const int nbw = (net.W + kTW - 1) / kTW, nbh = (net.H + kTH - 1) / kTH;
-  return dim3((unsigned)(((net.D + zc - 1) / zc) * nbh * nbw));
+dim3 conv_grid_zm(const ConvArgs& a, int zc) {
+  const int nbw = (a.W + kTW - 1) / kTW, nbh = (a.H + kTH - 1) / kTH;
+  return dim3((unsigned)(((a.D + zc - 1) / zc) * nbh * nbw));
 }
 
-// The four-layer pass per hidden width.  forward() ends in outf (fp32);
-// apply() ends in z = r + N/s (complex, R) with the scale of net.scal.
-template <int HID>
-struct NetOps {
-  static void hidden_layers(Net& net, const uint16_t* in, hipStream_t s, const int* done) {
-    const int zc = conv_zc();
-    hipLaunchKernelGGL((k_conv_zm<kNetIOPad, HID, kEpiHidden, kIOStaged>), conv_grid_zm(net, zc),
-                       dim3(kNetNT), 0, s,
-                       conv_args(net, in, net.wpk + net.wpad_off[0], net.bpk + net.bpad_off[0],
-                                 nullptr, net.act[1], done),
-                       zc);
-    hidden_2_3(net, s, done);
+// Run-time width -> template instantiation (hidden widths 16, 32, 48, 64).
+#define MSC_WIDTH_SWITCH(h, ...)   \
+  switch (h) {                     \
+    case 16: {                     \
+      constexpr int HW = 16;       \
+      __VA_ARGS__;                 \
+    } break;                       \
+    case 32: {                     \
+      constexpr int HW = 32;       \
+      __VA_ARGS__;                 \
+    } break;                       \
+    case 48: {                     \
+      constexpr int HW = 48;       \
+      __VA_ARGS__;                 \
+    } break;                       \
+    default: {                     \
+      constexpr int HW = 64;       \
+      __VA_ARGS__;                 \
+    } break;                       \
   }
 
-  // The second and third layers (act[1] -> act[2] -> act[3]).
-  static void hidden_2_3(Net& net, hipStream_t s, const int* done) {
-    const int zc = conv_zc();
-    const dim3 G = conv_grid_zm(net, zc);
-    hipLaunchKernelGGL((k_conv_zm<HID, HID, kEpiHidden>), G, dim3(kNetNT), 0, s,
-                       conv_args(net, net.act[1], net.wpk + net.wpad_off[1],
-                                 net.bpk + net.bpad_off[1], nullptr, net.act[2], done),
-                       zc);
-    hipLaunchKernelGGL((k_conv_zm<HID, HID, kEpiHidden>), G, dim3(kNetNT), 0, s,
-                       conv_args(net, net.act[2], net.wpk + net.wpad_off[2],
-                                 net.bpk + net.bpad_off[2], nullptr, net.act[3], done),
-                       zc);
-  }
+template <int NI, int NO, int EPI, int NS = NI, int SRC = 0>
+void launch_zm(const ConvArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_conv_zm<NI, NO, EPI, NS, SRC>), conv_grid_zm(a, kConvZc), dim3(kNetNT), 0,
+                     s, a, kConvZc);
+}
 
-  static void forward(Net& net, const uint16_t* in, hipStream_t s, const int* done) {
-    hidden_layers(net, in, s, done);
-    const int zc = conv_zc();
-    hipLaunchKernelGGL((k_conv_zm<HID, kNetIOPad, kEpiLast>), conv_grid_zm(net, zc),
-                       dim3(kNetNT), 0, s,
-                       conv_args(net, net.act[3], net.wpk + net.wpad_off[3],
-                                 net.bpk + net.bpad_off[3], nullptr, net.outf, done),
-                       zc);
-  }
-
-  // z = r + N((s / t_max) T0 r) / s; the first layer computes its input
-  // from r and the T0 parameters tp itself.
-  template <typename R>
-  static void apply(Net& net, const cx<R>* r, const cx<R>* tp, const unsigned int* tmax,
-                    cx<R>* z, hipStream_t s, const int* done) {
-    const int zc = conv_zc();
-    ConvArgs a0 = conv_args(net, nullptr, net.wpk + net.wpad_off[0], net.bpk + net.bpad_off[0],
-                            nullptr, net.act[1], done);
-    a0.r = r;
-    a0.tp = tp;
-    a0.tmax = tmax;
-    a0.scal = net.scal;
-    a0.scale = net.scale_arg;
-    constexpr int SRC = sizeof(R) == 4 ? 1 : 2;
-    hipLaunchKernelGGL((k_conv_zm<kNetIOPad, HID, kEpiHidden, kIOStaged, SRC>),
-                       conv_grid_zm(net, zc), dim3(kNetNT), 0, s, a0, zc);
-    hidden_2_3(net, s, done);
-    ConvArgs a = conv_args(net, net.act[3], net.wpk + net.wpad_off[3], net.bpk + net.bpad_off[3],
-                           nullptr, z, done);
-    a.r = r;
-    a.scal = net.scal;
-    a.scale = net.scale_arg;
-    constexpr int EPI = sizeof(R) == 4 ? kEpiApply64 : kEpiApply128;
-    hipLaunchKernelGGL((k_conv_zm<HID, kNetIOPad, EPI>), conv_grid_zm(net, zc), dim3(kNetNT), 0,
-                       s, a, zc);
-  }
-
-  // Backward of the loss gradient held in gbuf[0] ([v][kNetIOPad]); `in` is
-  // the network input.  Accumulates into net.grads.
-  static void backward(Net& net, const uint16_t* in, hipStream_t s) {
-    const dim3 G = conv_grid(net);
-    const int D = net.D, H = net.H, W = net.W;
-    const size_t nv = net.nvox;
-    const int vpb = 4096;
-    const dim3 wg((unsigned)((nv + vpb - 1) / vpb), 27);
-    const int gb = grid_of(nv * 64, 2048);
-    uint16_t* g4 = net.gbuf[0];
-    uint16_t* g3 = net.gbuf[1];
-    uint16_t* g2 = net.gbuf[0];
-    uint16_t* g1 = net.gbuf[1];
-    float* Gr = net.grads;
-    // layer 4 (HID -> IO)
-    hipLaunchKernelGGL((k_conv_wgrad<HID, kNetIOPad>), wg, dim3(kNetNT), 0, s, net.act[3], g4,
-                       Gr + net.woff[3], net.cin[3], net.cout[3], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<kNetIOPad>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g4,
-                       Gr + net.boff[3], net.cout[3], nv);
-    hipLaunchKernelGGL((k_conv<kNetIOPad, HID, kEpiBackward, true, kIOStaged>), G, dim3(kNetNT), 0,
-                       s,
-                       conv_args(net, g4, net.wtr + net.wpad_off[3], nullptr, net.act[3], g3,
-                                 nullptr));
-    // layer 3
-    hipLaunchKernelGGL((k_conv_wgrad<HID, HID>), wg, dim3(kNetNT), 0, s, net.act[2], g3,
-                       Gr + net.woff[2], net.cin[2], net.cout[2], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g3, Gr + net.boff[2],
-                       net.cout[2], nv);
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), G, dim3(kNetNT), 0, s,
-                       conv_args(net, g3, net.wtr + net.wpad_off[2], nullptr, net.act[2], g2,
-                                 nullptr));
-    // layer 2
-    hipLaunchKernelGGL((k_conv_wgrad<HID, HID>), wg, dim3(kNetNT), 0, s, net.act[1], g2,
-                       Gr + net.woff[1], net.cin[1], net.cout[1], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g2, Gr + net.boff[1],
-                       net.cout[1], nv);
-    hipLaunchKernelGGL((k_conv<HID, HID, kEpiBackward, true>), G, dim3(kNetNT), 0, s,
-                       conv_args(net, g2, net.wtr + net.wpad_off[1], nullptr, net.act[1], g1,
-                                 nullptr));
-    // layer 1 (IO -> HID): weights and bias only
-    hipLaunchKernelGGL((k_conv_wgrad<kNetIOPad, HID>), wg, dim3(kNetNT), 0, s, in, g1,
-                       Gr + net.woff[0], net.cin[0], net.cout[0], D, H, W, vpb);
-    hipLaunchKernelGGL((k_bias_grad<HID>), dim3(kBiasGrid), dim3(kNetNT), 0, s, g1, Gr + net.boff[0],
-                       net.cout[0], nv);
-  }
-};
-
-template <template <int> class F, typename... A>
-void by_width(int hidden, A&&... args) {
-  switch (hidden) {
-    case 16: F<16>::run(args...); break;
-    case 32: F<32>::run(args...); break;
-    case 48: F<48>::run(args...); break;
-    default: F<64>::run(args...); break;
+// Forward of convolution layer k.  src: 0 = bf16 input grid, 1/2 = computed from
+// a complex64/complex128 vector (first layer of an application); epi of the
+// last layer: kEpiLast, kEpiApply64 or kEpiApply128.
+void conv_forward(const Net& net, int k, ConvArgs a, int src, int epi, hipStream_t s) {
+  const NetLayer& l = net.L[k];
+  if (l.in == 0) {
+    MSC_WIDTH_SWITCH(l.pout, {
+      if (src == 0) launch_zm<kNetIOPad, HW, kEpiHidden, kIOStaged, 0>(a, s);
+      else if (src == 1) launch_zm<kNetIOPad, HW, kEpiHidden, kIOStaged, 1>(a, s);
+      else launch_zm<kNetIOPad, HW, kEpiHidden, kIOStaged, 2>(a, s);
+    })
+  } else if (l.cout == kNetIO) {
+    MSC_WIDTH_SWITCH(l.pin, {
+      if (epi == kEpiLast) launch_zm<HW, kNetIOPad, kEpiLast>(a, s);
+      else if (epi == kEpiApply64) launch_zm<HW, kNetIOPad, kEpiApply64>(a, s);
+      else launch_zm<HW, kNetIOPad, kEpiApply128>(a, s);
+    })
+  } else {
+    MSC_WIDTH_SWITCH(l.pin, launch_zm<HW, HW, kEpiHidden>(a, s))
   }
 }
 
-template <int HID>
-struct Fwd {
-  static void run(Net& n, const uint16_t* in, hipStream_t s, const int* done) {
-    NetOps<HID>::forward(n, in, s, done);
+DirArgs dir_args(const Net& net, int lvl, const uint16_t* in, const uint16_t* w,
+                 const float* bias, const uint16_t* skip, const uint16_t* act, uint16_t* out,
+                 const int* done) {
+  DirArgs r{};
+  r.in = in;
+  r.w = w;
+  r.bias = bias;
+  r.skip = skip;
+  r.act = act;
+  r.out = out;
+  r.done = done;
+  r.D = net.LD[lvl];
+  r.H = net.LH[lvl];
+  r.W = net.LW[lvl];
+  if (lvl > 0) {
+    r.Df = net.LD[lvl - 1];
+    r.Hf = net.LH[lvl - 1];
+    r.Wf = net.LW[lvl - 1];
   }
-};
-template <int HID>
-struct Bwd {
-  static void run(Net& n, const uint16_t* in, hipStream_t s) { NetOps<HID>::backward(n, in, s); }
-};
-template <int HID>
-struct Apply64 {
-  static void run(Net& n, const float2* r, const float2* tp, const unsigned int* tmax, float2* z,
-                  hipStream_t s, const int* done) {
-    NetOps<HID>::template apply<float>(n, r, tp, tmax, z, s, done);
+  return r;
+}
+
+// One wave per (16-voxel tile of the grid a.D x a.H x a.W, output block).
+dim3 dir_grid(const DirArgs& a, int nout) {
+  const size_t tiles = ((size_t)a.D * a.H * a.W + 15) / 16;
+  return dim3((unsigned)((tiles * (nout / 16) + kNetNT / 64 - 1) / (kNetNT / 64)));
+}
+
+// k_conv_dir / k_down_mfma / k_up_mfma with run-time widths.
+template <int EPI, bool MIRROR>
+void launch_conv_dir(int pin, int pout, const DirArgs& a, hipStream_t s) {
+  MSC_WIDTH_SWITCH(pin, {
+    constexpr int PI = HW;
+    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_conv_dir<PI, HW, EPI, MIRROR>), dir_grid(a, HW),
+                                              dim3(kNetNT), 0, s, a))
+  })
+}
+
+template <int EPI>
+void launch_down(int pin, int pout, const DirArgs& a, hipStream_t s) {
+  MSC_WIDTH_SWITCH(pin, {
+    constexpr int PI = HW;
+    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_down_mfma<PI, HW, EPI>), dir_grid(a, HW),
+                                              dim3(kNetNT), 0, s, a))
+  })
+}
+
+template <int EPI>
+void launch_up(int pin, int pout, const DirArgs& a, hipStream_t s) {
+  MSC_WIDTH_SWITCH(pin, {
+    constexpr int PI = HW;
+    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_up_mfma<PI, HW, EPI>), dir_grid(a, HW),
+                                              dim3(kNetNT), 0, s, a))
+  })
+}
+
+// The whole forward pass.  Training (src 0): from the bf16 input `in`, the last
+// layer writes outf (fp32).  Application (src 1/2): the first layer computes
+// its input from r and the T0 parameters tp, the last writes z = r + N / s.
+void forward_pass(Net& net, const uint16_t* in, int src, const void* r, const void* tp,
+                  const unsigned int* tmax, void* z, hipStream_t s, const int* done) {
+  const int last = net.nlayers - 1;
+  for (int k = 0; k < net.nlayers; ++k) {
+    const NetLayer& l = net.L[k];
+    const uint16_t* w = net.wpk + l.wpad_off;
+    const float* b = net.bpk + l.bpad_off;
+    const uint16_t* x = l.in == 0 ? in : net.act[l.in];
+    if (l.kind == kConv3 && net.T[l.out].level > 0) {
+      launch_conv_dir<kDirHidden, false>(
+          l.pin, l.pout,
+          dir_args(net, net.T[l.out].level, x, w, b, nullptr, nullptr, net.act[l.out], done), s);
+    } else if (l.kind == kConv3) {
+      void* out = k == last ? (src == 0 ? (void*)net.outf : z) : (void*)net.act[l.out];
+      ConvArgs a = conv_args(net, net.T[l.out].level, x, w, b, nullptr, out, done);
+      if (l.in == 0 && src != 0) {
+        a.r = r;
+        a.tp = tp;
+        a.tmax = tmax;
+        a.scal = net.scal;
+        a.scale = net.scale_arg;
+      }
+      if (k == last && src != 0) {
+        a.r = r;
+        a.scal = net.scal;
+        a.scale = net.scale_arg;
+      }
+      const int epi = src == 0 ? kEpiLast : (src == 1 ? kEpiApply64 : kEpiApply128);
+      conv_forward(net, k, a, l.in == 0 ? src : 0, epi, s);
+    } else if (l.kind == kDown2) {
+      launch_down<kDirHidden>(
+          l.pin, l.pout,
+          dir_args(net, net.T[l.out].level, x, w, b, nullptr, nullptr, net.act[l.out], done), s);
+    } else {
+      launch_up<kDirUp>(l.pin, l.pout,
+                        dir_args(net, net.T[l.in].level, x, w, b,
+                                 l.skip >= 0 ? net.act[l.skip] : nullptr, nullptr,
+                                 net.act[l.out], done),
+                        s);
+    }
   }
-};
-template <int HID>
-struct Apply128 {
-  static void run(Net& n, const double2* r, const double2* tp, const unsigned int* tmax,
-                  double2* z, hipStream_t s, const int* done) {
-    NetOps<HID>::template apply<double>(n, r, tp, tmax, z, s, done);
+}
+
+template <int NI, int NO, int MODE>
+void launch_wgrad(const uint16_t* in, const uint16_t* g, float* dW, int cin, int cout,
+                  WgradGrid q, hipStream_t s) {
+  const size_t nv = (size_t)q.D * q.H * q.W;
+  const int vpb = 4096;
+  const dim3 wg((unsigned)((nv + vpb - 1) / vpb), MODE == 0 ? 27 : 8);
+  hipLaunchKernelGGL((k_conv_wgrad<NI, NO, MODE>), wg, dim3(kNetNT), 0, s, in, g, dW, cin, cout,
+                     q, vpb);
+}
+
+template <int MODE>
+void wgrad_dispatch(int pin, int pout, const uint16_t* in, const uint16_t* g, float* dW, int cin,
+                    int cout, WgradGrid q, hipStream_t s) {
+  // pin, pout in {16, 32, 48, 64}; (32, *) also covers the 18-channel input
+  MSC_WIDTH_SWITCH(pin, {
+    constexpr int PI = HW;
+    MSC_WIDTH_SWITCH(pout, (launch_wgrad<PI, HW, MODE>(in, g, dW, cin, cout, q, s)))
+  })
+}
+
+// Backward of the loss gradient held in grad[last tensor] ([v][kNetIOPad]);
+// `in` is the network input.  Accumulates into net.grads.
+void backward_pass(Net& net, const uint16_t* in, hipStream_t s) {
+  const int last = net.nlayers - 1;
+  float* Gr = net.grads;
+  for (int k = last; k >= 0; --k) {
+    const NetLayer& l = net.L[k];
+    const int lo = net.T[l.out].level, li = l.in == 0 ? 0 : net.T[l.in].level;
+    const uint16_t* g = net.grad[l.out];
+    const uint16_t* x = l.in == 0 ? in : net.act[l.in];
+    // weight and bias gradients
+    if (l.kind == kConv3) {
+      WgradGrid q{net.LD[lo], net.LH[lo], net.LW[lo], 0, 0, 0};
+      wgrad_dispatch<0>(l.pin, l.pout, x, g, Gr + l.woff, l.cin, l.cout, q, s);
+    } else {
+      const int c = l.kind == kDown2 ? lo : li, f = c - 1;
+      WgradGrid q{net.LD[c], net.LH[c], net.LW[c], net.LD[f], net.LH[f], net.LW[f]};
+      if (l.kind == kDown2)
+        wgrad_dispatch<1>(l.pin, l.pout, x, g, Gr + l.woff, l.cin, l.cout, q, s);
+      else
+        wgrad_dispatch<2>(l.pin, l.pout, x, g, Gr + l.woff, l.cin, l.cout, q, s);
+    }
+    MSC_WIDTH_SWITCH(l.pout, hipLaunchKernelGGL((k_bias_grad<HW>), dim3(kBiasGrid), dim3(kNetNT), 0,
+                                                s, g, Gr + l.boff, l.cout, net.lvox[lo]))
+    if (l.in == 0) continue;
+    // data gradient into the layer's input tensor
+    const uint16_t* wt = net.wtr + l.wpad_off;
+    if (l.kind == kConv3 && li > 0) {
+      const DirArgs a = dir_args(net, li, g, wt, nullptr, nullptr, net.act[l.in], net.grad[l.in],
+                                 nullptr);
+      if (net.T[l.in].act)
+        launch_conv_dir<kDirBackward, true>(l.pout, l.pin, a, s);
+      else
+        launch_conv_dir<kDirBackLin, true>(l.pout, l.pin, a, s);
+    } else if (l.kind == kConv3) {
+      ConvArgs a = conv_args(net, li, g, wt, nullptr, net.act[l.in], net.grad[l.in], nullptr);
+      const dim3 G = conv_grid(a);
+      const bool mask = net.T[l.in].act;
+      if (k == last) {
+        MSC_WIDTH_SWITCH(l.pin, hipLaunchKernelGGL((k_conv<kNetIOPad, HW, kEpiBackward, true,
+                                                            kIOStaged>),
+                                                   G, dim3(kNetNT), 0, s, a))
+      } else if (mask) {
+        MSC_WIDTH_SWITCH(l.pin, hipLaunchKernelGGL((k_conv<HW, HW, kEpiBackward, true>), G,
+                                                   dim3(kNetNT), 0, s, a))
+      } else {
+        MSC_WIDTH_SWITCH(l.pin, hipLaunchKernelGGL((k_conv<HW, HW, kEpiBackwardLin, true>), G,
+                                                   dim3(kNetNT), 0, s, a))
+      }
+    } else if (l.kind == kDown2) {
+      const int su = net.T[l.in].skip_user;
+      const uint16_t* gskip = su >= 0 ? net.grad[net.L[su].out] : nullptr;
+      launch_up<kDirDownBack>(l.pout, l.pin,
+                              dir_args(net, lo, g, wt, nullptr, gskip, net.act[l.in],
+                                       net.grad[l.in], nullptr),
+                              s);
+    } else {
+      launch_down<kDirBackward>(l.pout, l.pin,
+                                dir_args(net, li, g, wt, nullptr, nullptr, net.act[l.in],
+                                         net.grad[l.in], nullptr),
+                                s);
+    }
   }
-};
+}
 
 }  // namespace
 
-int net_forward(Net& net, const uint16_t* in, hipStream_t s) {
-  MSC_REQUIRE(net.packed, "network parameters are not set");
-  by_width<Fwd>(net.hidden, net, in, s, nullptr);
-  MSC_LAUNCH_CHECK();
-  return kOk;
-}
-
 int net_forward_out(Net& net, int D, int H, int W, const uint16_t* in, float* out,
                     hipStream_t s) {
+  MSC_REQUIRE(net.packed, "network parameters are not set");
   int rc = net.ensure_grid(D, H, W);
   if (rc) return rc;
-  rc = net_forward(net, in, s);
-  if (rc) return rc;
+  forward_pass(net, in, 0, nullptr, nullptr, nullptr, nullptr, s, nullptr);
   hipLaunchKernelGGL(k_compact_out, dim3(grid_of(net.nvox * kNetIO)), dim3(kNetNT), 0, s,
                      net.outf, out, net.nvox);
   MSC_LAUNCH_CHECK();
@@ -1159,10 +1563,10 @@ int net_gradients(Net& net, int D, int H, int W, int batch, const uint16_t* in,
   for (int b = 0; b < batch; ++b) {
     const uint16_t* xin = in + (size_t)b * nv * kNetIOPad;
     const float* tgt = target + (size_t)b * nv * kNetIO;
-    by_width<Fwd>(net.hidden, net, xin, s, nullptr);
+    forward_pass(net, xin, 0, nullptr, nullptr, nullptr, nullptr, s, nullptr);
     hipLaunchKernelGGL(k_loss_grad, dim3(grid_of(nv * kNetIOPad, 4096)), dim3(kNetNT), 0, s,
-                       net.outf, tgt, net.gbuf[0], nv, gscale, net.loss);
-    by_width<Bwd>(net.hidden, net, xin, s);
+                       net.outf, tgt, net.grad[net.ntensors - 1], nv, gscale, net.loss);
+    backward_pass(net, xin, s);
   }
   MSC_LAUNCH_CHECK();
   if (loss_host != nullptr) {
@@ -1235,11 +1639,13 @@ int net_apply(Net& net, const Plan& p, const cx<R>* r, cx<R>* z, double scale, h
   if (rc) return rc;
   MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
   net.scale_arg = set_input_scale<R>(p, r, scale, net.scal, s);
+  const void* tp;
   if constexpr (sizeof(R) == 4) {
-    by_width<Apply64>(net.hidden, net, r, p.tp64, p.tmax, z, s, done);
+    tp = p.tp64;
   } else {
-    by_width<Apply128>(net.hidden, net, r, p.tp128, p.tmax, z, s, done);
+    tp = p.tp128;
   }
+  forward_pass(net, nullptr, sizeof(R) == 4 ? 1 : 2, r, tp, p.tmax, z, s, done);
   MSC_LAUNCH_CHECK();
   return kOk;
 }

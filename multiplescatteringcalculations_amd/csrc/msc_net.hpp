@@ -2,10 +2,22 @@
 //
 // The network maps the induced source of a residual to a correction:
 //   z = r + N((s / t_max) T0 r) / s,  s = sqrt(n) / ||r||, t_max = max |T0 params|
-// with N four 3x3x3 "same" convolutions 18 -> h -> h -> h -> 18 over the
-// (N_z, M, M) voxel grid (channels = real and imaginary parts of the 9
-// components, interleaved), leaky-ReLU between layers.  Activations are
-// channels-last BF16, channel counts padded to multiples of 16.
+// over the (N_z, M, M) voxel grid (channels = real and imaginary parts of the
+// 9 components, interleaved).  N is a small layer graph of three kinds of
+// layer: 3x3x3 "same" convolutions, 2x2x2 stride-2 convolutions to a grid
+// half as fine in each axis ("down"), and 2x2x2 stride-2 transposed
+// convolutions back ("up", whose output is added to a skip tensor of the
+// finer grid).  Two architectures:
+//   * local: four 3x3x3 convolutions 18 -> h -> h -> h -> 18 (h 16..64);
+//   * unet: four resolution levels with widths (c0, c1, c2, c3):
+//       x0 = conv(conv(in))                      level 0, c0
+//       x1 = conv(down(x0)), x2 = conv(down(x1)) levels 1, 2
+//       x3 = conv(down(x2))                      level 3
+//       y2 = conv(up(x3) + x2), y1 = conv(up(y2) + x1)
+//       out = conv(conv(up(y1) + x0)) -> 18
+//     with leaky ReLU after every convolution and down layer but the last,
+//     and linear up layers.
+// Activations are channels-last BF16, channel counts padded to multiples of 16.
 #pragma once
 
 #include <cstdint>
@@ -15,19 +27,41 @@
 
 namespace msc {
 
-constexpr int kNetLayers = 4;
 constexpr int kNetIO = 18;           // real channels in and out
 constexpr int kNetIOPad = 32;        // padded to a multiple of 16
 constexpr float kNetSlope = 0.1f;    // leaky-ReLU negative slope
+constexpr int kNetMaxLayers = 16;
+constexpr int kNetMaxTensors = kNetMaxLayers + 1;
+constexpr int kNetLevels = 4;
+
+enum NetArch : int { kArchLocal = 0, kArchUNet = 1 };
+enum LayerKind : int { kConv3 = 0, kDown2 = 1, kUp2 = 2 };
+
+struct NetLayer {
+  int kind = kConv3;
+  int in = 0, out = 0;  // tensor ids (tensor 0 is the network input)
+  int skip = -1;        // kUp2: tensor added to the output (-1: none)
+  int cin = 0, cout = 0, pin = 0, pout = 0, taps = 27;
+  bool act = true;      // leaky ReLU on the output
+  size_t woff = 0, boff = 0;        // offsets in the flat fp32 vector ([cout][cin][taps], [cout])
+  size_t wpad_off = 0, bpad_off = 0;  // offsets in the packed copies
+};
+
+struct NetTensor {
+  int level = 0;       // resolution level (0 = the voxel grid)
+  int ch = 0;          // padded channels
+  bool act = false;    // produced by a leaky-ReLU layer
+  int skip_user = -1;  // layer that adds this tensor as a skip (-1: none)
+};
 
 struct Net {
-  int hidden = 0;
-  int cin[kNetLayers] = {}, cout[kNetLayers] = {};    // real channel counts
-  int pin[kNetLayers] = {}, pout[kNetLayers] = {};    // padded counts
-  size_t woff[kNetLayers] = {}, boff[kNetLayers] = {};  // offsets in the flat fp32 vector
-  size_t nparams = 0;
-  size_t wpad_off[kNetLayers] = {}, bpad_off[kNetLayers] = {};  // packed (padded) offsets
-  size_t nwpad = 0, nbpad = 0;
+  int arch = kArchLocal;
+  int hidden = 0;               // local: h; unet: c0
+  int widths[kNetLevels] = {};  // unet level widths
+  int nlayers = 0, ntensors = 0;
+  NetLayer L[kNetMaxLayers];
+  NetTensor T[kNetMaxTensors];
+  size_t nparams = 0, nwpad = 0, nbpad = 0;
   int device = 0;
   // parameters: fp32 master copy, Adam moments, gradient accumulator
   float* params = nullptr;
@@ -35,20 +69,24 @@ struct Net {
   float* adam_v = nullptr;
   float* grads = nullptr;
   long long adam_t = 0;
-  // BF16 packed weights [27][pout][pin] and transposed [27][pin][pout]; fp32 padded biases
+  // BF16 packed weights [taps][pout][pin] and transposed [taps][pin][pout]; fp32 padded biases
   uint16_t* wpk = nullptr;
   uint16_t* wtr = nullptr;
   float* bpk = nullptr;
   // activation workspace for one grid
   int D = 0, H = 0, W = 0;
+  int LD[kNetLevels] = {}, LH[kNetLevels] = {}, LW[kNetLevels] = {};
+  size_t lvox[kNetLevels] = {};
   size_t nvox = 0;
-  uint16_t* act[kNetLayers] = {};  // act[l] = output of layer l-1 (act[0] unused: the input is
-                                   // the caller's, or computed by an application's first layer)
-  float* outf = nullptr;           // last layer output, fp32 [vox][pout[3]]
-  uint16_t* gbuf[2] = {};          // gradient ping-pong (bf16), sized for the widest layer
-  double* loss = nullptr;          // device loss accumulator (+ partials)
-  double* scal = nullptr;          // device (n, sum |r|^2) of the current apply: s = sqrt(n/sum)
-  float scale_arg = 0.f;           // host: a given s of the current apply, 0 = read scal
+  uint16_t* act[kNetMaxTensors] = {};   // tensor values (act[0] unused: the input is the
+                                        // caller's, or computed by an application's first
+                                        // layer; the last tensor goes to outf)
+  uint16_t* grad[kNetMaxTensors] = {};  // gradient of each tensor (of its pre-activation for
+                                        // leaky-ReLU tensors); grad[last] is the loss gradient
+  float* outf = nullptr;                // last layer output, fp32 [vox][kNetIOPad]
+  double* loss = nullptr;               // device loss accumulator (+ partials)
+  double* scal = nullptr;  // device (n, sum |r|^2) of the current apply: s = sqrt(n/sum)
+  float scale_arg = 0.f;   // host: a given s of the current apply, 0 = read scal
   bool packed = false;
   // changes whenever device buffers are (re)allocated; captured graphs that
   // reference this network are valid only for one generation
@@ -60,6 +98,8 @@ struct Net {
 };
 
 int net_init(Net& net, int hidden);
+// the 4-level network of widths c[0..3] (c0 = 16; c1..c3 in {16, 32, 48})
+int net_init_unet(Net& net, const int* c);
 // pack fp32 master parameters into the BF16 / padded copies
 int net_pack(Net& net, hipStream_t s);
 // forward pass of a [D][H][W][32] bf16 input; out = first 18 channels, fp32 [v][18]

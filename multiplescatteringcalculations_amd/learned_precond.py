@@ -30,25 +30,49 @@ from . import _lib
 
 IO_CHANNELS = 18
 PAD_CHANNELS = 32
+UNET_WIDTHS = (16, 16, 32, 48)
 
 
-def num_params(hidden: int) -> int:
-    c = [IO_CHANNELS, hidden, hidden, hidden, IO_CHANNELS]
-    return sum(27 * c[l] * c[l + 1] + c[l + 1] for l in range(4))
+def layers(arch: str = "local", hidden: int = 16, widths=UNET_WIDTHS) -> list[tuple[str, int, int]]:
+    """The layer list (kind, in channels, out channels) in forward order:
+    kind "conv" (3x3x3), "down" (2x2x2 stride 2) or "up" (2x2x2 stride-2
+    transposed, added to the skip of the finer level)."""
+    io = IO_CHANNELS
+    if arch == "local":
+        return [("conv", io, hidden), ("conv", hidden, hidden), ("conv", hidden, hidden),
+                ("conv", hidden, io)]
+    if arch != "unet":
+        raise ValueError(f"arch must be 'local' or 'unet', got '{arch}'")
+    c0, c1, c2, c3 = widths
+    return [("conv", io, c0), ("conv", c0, c0), ("down", c0, c1), ("conv", c1, c1),
+            ("down", c1, c2), ("conv", c2, c2), ("down", c2, c3), ("conv", c3, c3),
+            ("up", c3, c2), ("conv", c2, c2), ("up", c2, c1), ("conv", c1, c1),
+            ("up", c1, c0), ("conv", c0, c0), ("conv", c0, io)]
 
 
-def init_params(hidden: int, seed: int = 0, last_scale: float = 0.1) -> np.ndarray:
-    """He-normal weights (fan-in 27 * in), zero biases, last layer scaled so that
-    the untrained preconditioner starts close to the identity."""
+def _taps(kind: str) -> int:
+    return 27 if kind == "conv" else 8
+
+
+def num_params(hidden: int = 16, arch: str = "local", widths=UNET_WIDTHS) -> int:
+    return sum(_taps(k) * ci * co + co for k, ci, co in layers(arch, hidden, widths))
+
+
+def init_params(hidden: int = 16, seed: int = 0, last_scale: float = 0.1, arch: str = "local",
+                widths=UNET_WIDTHS) -> np.ndarray:
+    """He-normal weights (fan-in taps x in channels; linear up layers use the
+    Xavier-like fan-in of their inputs), zero biases, last layer scaled so
+    that the untrained preconditioner starts close to the identity."""
     rng = np.random.default_rng(seed)
-    c = [IO_CHANNELS, hidden, hidden, hidden, IO_CHANNELS]
+    lay = layers(arch, hidden, widths)
     parts = []
-    for l in range(4):
-        std = math.sqrt(2.0 / (27 * c[l]))
-        if l == 3:
+    for l, (kind, ci, co) in enumerate(lay):
+        fan = (_taps(kind) if kind != "up" else 1) * ci
+        std = math.sqrt((2.0 if kind != "up" else 1.0) / fan)
+        if l == len(lay) - 1:
             std *= last_scale
-        parts.append(rng.standard_normal(27 * c[l] * c[l + 1]) * std)
-        parts.append(np.zeros(c[l + 1]))
+        parts.append(rng.standard_normal(_taps(kind) * ci * co) * std)
+        parts.append(np.zeros(co))
     return np.concatenate(parts).astype(np.float32)
 
 
@@ -56,21 +80,29 @@ class LearnedPreconditioner:
     """Handle of a device-resident network (msc_net_* of include/msc_precond.h)."""
 
     def __init__(self, hidden: int = 48, device: torch.device | str | None = None,
-                 seed: int | None = 0):
+                 seed: int | None = 0, arch: str = "local", widths=UNET_WIDTHS):
         self.device = torch.device(device) if device is not None else _lib.require_device()
         if self.device.type != "cuda":
             raise ValueError("the learned preconditioner runs on a HIP device only")
+        if arch not in ("local", "unet"):
+            raise ValueError(f"arch must be 'local' or 'unet', got '{arch}'")
         self.lib = _lib.load()
-        self.hidden = int(hidden)
+        self.arch = arch
+        self.widths = tuple(int(c) for c in widths)
+        self.hidden = int(hidden) if arch == "local" else self.widths[0]
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            _lib.check(self.lib.msc_net_create(ctypes.byref(h), self.hidden))
+            if arch == "local":
+                _lib.check(self.lib.msc_net_create(ctypes.byref(h), self.hidden))
+            else:
+                c4 = (ctypes.c_int * 4)(*self.widths)
+                _lib.check(self.lib.msc_net_create_unet(ctypes.byref(h), c4))
         self._h = h
         n = ctypes.c_longlong()
         _lib.check(self.lib.msc_net_num_params(self._h, ctypes.byref(n)))
         self.num_params = int(n.value)
         if seed is not None:
-            self.set_params(init_params(self.hidden, seed))
+            self.set_params(init_params(self.hidden, seed, arch=arch, widths=self.widths))
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -107,12 +139,15 @@ class LearnedPreconditioner:
             torch.cuda.current_stream(self.device).synchronize()
 
     def save(self, path) -> None:
-        np.savez(path, hidden=self.hidden, params=self.params().cpu().numpy())
+        np.savez(path, hidden=self.hidden, arch=self.arch, widths=np.array(self.widths),
+                 params=self.params().cpu().numpy())
 
     @classmethod
     def load(cls, path, device=None) -> "LearnedPreconditioner":
         with np.load(path, allow_pickle=False) as f:
-            net = cls(int(f["hidden"]), device=device, seed=None)
+            arch = str(f["arch"]) if "arch" in f else "local"
+            widths = tuple(int(c) for c in f["widths"]) if "widths" in f else UNET_WIDTHS
+            net = cls(int(f["hidden"]), device=device, seed=None, arch=arch, widths=widths)
             net.set_params(f["params"])
         return net
 

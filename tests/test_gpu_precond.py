@@ -279,3 +279,94 @@ def test_ablation_trained_network_vs_plain(hip_device):
     for prec in ("fp64", "mixed"):
         assert res[("learned", prec)][1]["fell_back"] == 0
         assert res[("learned", prec)][1]["iterations"] < res[("plain", prec)][1]["iterations"]
+
+
+# ── the four-level network (msc_net_create_unet) ─────────────────────────────
+
+
+def _unet(seed=0, widths=(16, 16, 32, 48), jitter=0.02):
+    """He-initialised weights plus a small random perturbation of every
+    parameter (biases included), so that activations stay of order one."""
+    from multiplescatteringcalculations_amd import learned_precond as lp
+
+    net = lp.LearnedPreconditioner(seed=None, arch="unet", widths=widths)
+    g = torch.Generator().manual_seed(seed)
+    p = torch.as_tensor(lp.init_params(seed=seed, arch="unet", widths=widths, last_scale=1.0))
+    p = p + jitter * torch.randn(p.numel(), generator=g)
+    net.set_params(p)
+    return net, p
+
+
+@pytest.mark.parametrize("widths,grid", [((16, 16, 32, 48), (8, 8, 8)),
+                                         ((16, 16, 32, 48), (5, 7, 13)),
+                                         ((16, 16, 32, 48), (4, 24, 72)),
+                                         ((16, 16, 32, 48), (1, 2, 3)),
+                                         ((16, 32, 32, 48), (6, 10, 12)),
+                                         ((16, 16, 16, 16), (3, 9, 66))])
+def test_unet_forward_matches_oracle(hip_device, widths, grid):
+    net, p = _unet(seed=sum(grid), widths=widths)
+    x = _bf16_input(grid, seed=2)
+    out = net.forward(x.to(hip_device))
+    ref, _ = op.unet_forward(p, x[..., :18].float(), bf16=True, widths=widths)
+    assert out.shape == (*grid, 18)
+    assert float(ref.abs().max()) > 0.1
+    assert _nrel(out, ref) < 1e-2, _nrel(out, ref)
+
+
+@pytest.mark.parametrize("grid", [(4, 6, 8), (3, 5, 7)])
+def test_unet_gradients_match_oracle(hip_device, grid):
+    widths = (16, 16, 32, 48)
+    net, p = _unet(seed=9)
+    xs = torch.stack([_bf16_input(grid, seed=s) for s in (3, 4)])
+    ts = torch.randn(2, *grid, 18, generator=torch.Generator().manual_seed(5))
+    gg, loss = net.gradients(xs.to(hip_device), ts.to(hip_device))
+    gr, lr = op.unet_gradients(p, xs[..., :18].float(), ts, bf16=True, widths=widths)
+    assert loss == pytest.approx(lr, rel=1e-3)
+    off = 0
+    for l, (kind, ci, co, _) in enumerate(op.unet_layers(widths)):
+        for n in ((27 if kind == "conv" else 8) * ci * co, co):
+            e = _nrel(gg[off:off + n], gr[off:off + n])
+            assert e < 3e-2, (l, kind, n, e)
+            off += n
+    assert off == net.num_params
+
+
+def test_unet_apply_matches_oracle(hip_device):
+    plan, psi0, mat, omega, ref = _small_plan(M=10, Nz=6)
+    g = plan.geometry
+    net, p = _unet(seed=12)
+    tp = plan.tparams().reshape(g.N_z, g.M, g.M, 4).cpu()
+    rng = np.random.default_rng(8)
+    r = torch.as_tensor(rng.standard_normal(plan.n) + 1j * rng.standard_normal(plan.n))
+    r9 = r.reshape(g.N_z, g.M, g.M, 9)
+    zr = op.unet_apply(p, r9, tp).reshape(-1)
+    for dt in (torch.complex128, torch.complex64):
+        z = net.apply(plan, r.to(hip_device, dt))
+        corr, corr_r = (z.cpu().to(torch.complex128) - r), (zr - r)
+        assert float(torch.linalg.vector_norm(corr_r)) > 1e-2 * float(torch.linalg.vector_norm(r))
+        assert _nrel(corr, corr_r) < 2e-2, (dt, _nrel(corr, corr_r))
+
+
+def test_unet_training_reduces_loss_and_zero_net_is_plain_gmres(hip_device):
+    from multiplescatteringcalculations_amd.learned_precond import LearnedPreconditioner
+
+    net, p = _unet(seed=3)
+    grid = (4, 8, 8)
+    xs = torch.stack([_bf16_input(grid, seed=s) for s in range(4)]).to(hip_device)
+    A = torch.randn(18, 18, generator=torch.Generator().manual_seed(9)).to(hip_device) * 0.3
+    ts = (xs[..., :18].float() @ A).contiguous()
+    _, l0 = net.gradients(xs, ts)
+    losses = [net.train_step(xs, ts, lr=2e-3) for _ in range(80)]
+    assert losses[0] == pytest.approx(l0, rel=1e-6)
+    assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
+    plan, psi0, *_ = _small_plan(seed=1)
+    zero = LearnedPreconditioner(seed=None, arch="unet")
+    x0, s0 = plan.gmres(psi0, "born", tol=1e-11, max_iter=200, restart=20, precision="fp64")
+    x1, s1 = plan.gmres(psi0, "born", tol=1e-11, max_iter=200, restart=20, precision="fp64",
+                        precond=zero)
+    assert s1["converged"] and s1["precond_applies"] == s1["iterations"] > 0
+    assert abs(s1["iterations"] - s0["iterations"]) <= 1
+    assert _nrel(x1, x0) < 1e-10
+    x2, s2 = plan.gmres(psi0, "born", tol=1e-11, max_iter=300, restart=20, precision="mixed",
+                        precond=net)
+    assert s2["converged"] and _nrel(x2, x0) < 1e-9

@@ -107,3 +107,59 @@ def test_zero_network_is_identity():
     tp = torch.randn(2, 3, 3, 4, dtype=torch.complex128)
     z = op.apply(p, h, r, tp)
     assert torch.equal(z, r)
+
+
+# ── the four-level network ───────────────────────────────────────────────────
+
+
+def test_unet_parameter_count():
+    from multiplescatteringcalculations_amd import learned_precond as lp
+
+    w = (16, 16, 32, 48)
+    c0, c1, c2, c3 = w
+    conv = 27 * (18 * c0 + c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3 + c2 * c2 + c1 * c1 + c0 * c0
+                 + c0 * 18)
+    resample = 8 * 2 * (c0 * c1 + c1 * c2 + c2 * c3)
+    bias = (c0 + c0 + c1 + c1 + c2 + c2 + c3 + c3 + c2 + c2 + c1 + c1 + c0 + c0 + 18)
+    assert op.unet_num_params(w) == conv + resample + bias == 197938
+    assert lp.num_params(arch="unet", widths=w) == 197938
+    assert lp.init_params(seed=0, arch="unet", widths=w).shape == (197938,)
+    for w2 in ((16, 32, 32, 48), (16, 16, 16, 16)):
+        assert op.unet_num_params(w2) == lp.num_params(arch="unet", widths=w2)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (5, 7, 6), (3, 4, 9), (1, 2, 1)])
+def test_unet_forward_equals_functional_reference(shape):
+    """The oracle's space-to-depth down and depth-to-space up layers equal
+    conv3d with stride 2 on the zero-extended grid and conv_transpose3d
+    cropped to the fine grid."""
+    p = torch.randn(op.unet_num_params(), generator=torch.Generator().manual_seed(3)) * 0.15
+    x = torch.randn(*shape, 18, generator=torch.Generator().manual_seed(4))
+    out, t = op.unet_forward(p, x, bf16=False)
+    ref = op.unet_reference_forward(p, x)
+    assert float((out - ref).abs().max()) <= 1e-5 * float(ref.abs().max())  # FP32 sums
+    # level sizes: ceil halving
+    assert t[3].shape[2:] == tuple((s + 1) // 2 for s in shape)
+
+
+@pytest.mark.parametrize("shape", [(6, 8, 8), (5, 7, 3)])
+def test_unet_explicit_backprop_equals_autograd(shape):
+    p = torch.randn(op.unet_num_params(), generator=torch.Generator().manual_seed(5)) * 0.12
+    g = torch.Generator().manual_seed(6)
+    xs = torch.randn(2, *shape, 18, generator=g)
+    ts = torch.randn(2, *shape, 18, generator=g)
+    g1, l1 = op.unet_gradients(p, xs, ts, bf16=False)
+    g2, l2 = op.unet_autograd_gradients(p, xs, ts)
+    assert l1 == pytest.approx(l2, rel=1e-6)
+    assert float((g1 - g2).norm() / g2.norm()) < 1e-5
+    # every layer's weights and bias receive a gradient of their own
+    off = 0
+    for kind, ci, co, _ in op.unet_layers():
+        n = (27 if kind == "conv" else 8) * ci * co
+        for m in (n, co):
+            a, b = g1[off:off + m], g2[off:off + m]
+            assert float((a - b).norm()) <= 1e-4 * float(b.norm()) + 1e-9
+            off += m
+    # a planted wrong leaky-ReLU derivative is caught
+    g3, _ = op.unet_gradients(p, xs, ts, bf16=False, slope_bwd=0.2)
+    assert float((g3 - g2).norm() / g2.norm()) > 1e-2
