@@ -9,12 +9,16 @@ GMRES(30) solve of (I - G0 T0) psi = psi0 from the Born guess to
 ||b - A psi|| / ||b|| <= 1e-8 in FP64, with the operator (G0 table, T0) and
 psi0 already resident in HBM.
 
-The headline (``value``) is the learned + mixed solve: flexible GMRES(30) with
-the learned convolutional preconditioner (trained on the GPU, before the timed
-region, on pairs harvested from plain solves of other seeded slabs of the same
-distribution) around complex64 cycles inside FP64 iterative refinement.  The
-plain FP64 GMRES(30) solve and the mixed solve without the network are timed
-the same way and reported beside it.
+The headline (``value``) is the learned + mixed solver: GMRES(30) with
+complex64 cycles inside FP64 iterative refinement, and the learned four-level
+convolutional preconditioner in flexible GMRES when its validation gate is
+open.  The network is trained on the GPU before the timed region, on pairs
+streamed from GMRES cycles on other seeded slabs of the same distribution, and
+validated on a held-out slab: the gate opens only if the solve with the network
+beats the same solve without it.  The plain FP64 GMRES(30) solve, the mixed
+solve and the solve with the network forced on are timed the same way and
+reported beside it; the strong-contrast 96 x 96 x 24 leg
+(``detail.other_paths.learned_strong_contrast``) is where the network pays.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -39,7 +43,7 @@ M, NZ, A = 128, 32, 1.0
 VP, VS, RHO = 3000.0, 1732.0, 2500.0
 PHI, CONTRAST, KA, P_SLOW = 0.3, 0.2, 0.4, 1.5e-4
 TOL, RESTART = 1e-8, 30
-HIDDEN, TRAIN_SEEDS, TRAIN_STEPS, TRAIN_BATCH, TRAIN_LR = 16, 4, 100, 2, 2e-3
+UNET_WIDTHS, TRAIN_PAIRS, TRAIN_BATCH, TRAIN_LR = (16, 16, 32, 48), 4000, 4, 1e-3
 
 
 def setup(device):
@@ -59,10 +63,13 @@ def setup(device):
 
 
 def train_preconditioner(plan, mat0, omega, device):
-    """Harvest training pairs from plain FP64 GMRES runs on seeded slabs 1..K of
-    the benchmark distribution (the timed slab is seed 0) and train the network
-    on the GPU.  The plan's G0 table is shared (same geometry and frequency);
-    the timed slab's T0 is restored afterwards."""
+    """The learned part of the headline solver: the four-level network trained on
+    the GPU on residual -> correction pairs streamed from GMRES(30) cycles on
+    seeded slabs 1, 2, ... of the benchmark distribution (the timed slab is
+    seed 0), then validated on a held-out slab (seed 10000): the gate opens only
+    if flexible GMRES with the network beats the same mixed solve without it.
+    The plan's G0 is shared (same geometry and frequency); the timed slab's T0
+    is restored afterwards."""
     import torch
 
     import multiplescatteringcalculations_amd as msc
@@ -70,29 +77,23 @@ def train_preconditioner(plan, mat0, omega, device):
 
     ref = msc.ReferenceMedium(VP, VS, RHO)
     khat = np.array([np.sqrt(1.0 - (P_SLOW * VP) ** 2), P_SLOW * VP, 0.0])
-    t0 = time.perf_counter()
-    sets = []
-    for seed in range(1, TRAIN_SEEDS + 1):
-        m = msc.random_iid_slab_material(plan.geometry, ref, PHI, CONTRAST, seed=seed)
-        plan.set_material(m, omega)
-        b = plan.incident_field(omega, ref, khat, "P").reshape(-1)
-        sets.append(lp.harvest_pairs(plan, b, restart=RESTART, tol=TOL))
-    pairs = lp.merge_pairs(sets)
+
+    def draw(seed):
+        return msc.random_iid_slab_material(plan.geometry, ref, PHI, CONTRAST, seed=seed)
+
+    net = lp.LearnedPreconditioner(device=device, seed=0, arch="unet", widths=UNET_WIDTHS)
+    info = lp.train_on_slabs(net, plan, draw, omega, ref, khat, target_pairs=TRAIN_PAIRS,
+                             seeds_per_round=16, batch=TRAIN_BATCH, lr=TRAIN_LR, log=log)
+    plan.set_material(draw(10000), omega)
+    val = net.validate(plan, plan.incident_field(omega, ref, khat, "P").reshape(-1), tol=TOL,
+                       restart=RESTART)
     plan.set_material(mat0, omega)
     torch.cuda.synchronize()
-    t_harvest = time.perf_counter() - t0
-    net = lp.LearnedPreconditioner(HIDDEN, device=device, seed=0)
-    t0 = time.perf_counter()
-    losses, before, after = lp.train(net, pairs, steps=TRAIN_STEPS, lr=TRAIN_LR,
-                                     batch=TRAIN_BATCH, holdout=4)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    info = {"pairs": len(pairs), "harvest_s": round(t_harvest, 3), "train_s": round(t_train, 3),
-            "train_steps": TRAIN_STEPS, "batch": TRAIN_BATCH, "hidden": HIDDEN,
-            "params": net.num_params, "loss_first": losses[0], "loss_last": losses[-1],
-            "holdout_loss_before": before, "holdout_loss_after": after}
-    log(f"learned preconditioner: {len(pairs)} pairs in {t_harvest:.2f} s, {TRAIN_STEPS} Adam "
-        f"steps in {t_train:.2f} s, held-out loss {before:.3e} -> {after:.3e}")
+    info = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()}
+    info.update({"network": f"unet {UNET_WIDTHS}", "params": net.num_params,
+                 "validation_held_out_seed_10000": {k: (round(v, 3) if isinstance(v, float) else v)
+                                                    for k, v in val.items()}})
+    log(f"learned preconditioner: {info}")
     return net, info
 
 
@@ -237,48 +238,81 @@ def time_resonance(n_sub: int = 4, count: int = 1024):
     return out
 
 
-def time_strong_contrast(hidden: int = 16):
-    """The learned preconditioner on a strong-contrast slab: 96x96x24 voxels,
-    Bernoulli(0.5) inclusions with contrasts U[-0.4, 0.4], ka 0.75, p 0.15
-    s/km.  The network is trained on pairs harvested from plain solves of
-    seeds 1-4 (300 Adam steps, batch 2) and timed on seed 0 beside plain FP64
-    and mixed GMRES(30), all to the FP64 residual 1e-8."""
+STRONG = {"M": 96, "N_z": 24, "phi": 0.5, "contrast": 0.4, "ka": 0.75, "pairs": 20000,
+          "seeds_per_round": 32, "batch": 4, "lr": 1e-3, "widths": (16, 16, 32, 48)}
+
+
+def time_strong_contrast(pairs: int | None = None):
+    """The learned preconditioner where plain GMRES needs many iterations:
+    96x96x24 voxels, Bernoulli(0.5) inclusions with contrasts U[-0.4, 0.4],
+    ka 0.75, p 0.15 s/km.  The four-level network (about 200k parameters) is
+    trained on the GPU on residual -> correction pairs streamed from GMRES(30)
+    cycles on seeded slabs 1, 2, ... of the same distribution (the last two
+    rounds from flexible cycles with the network), then plain FP64, mixed and
+    learned + mixed GMRES(30) solve the held-out slabs (seeds 0 and 100000) to
+    the FP64 residual 1e-8 (best of 3 timed solves each)."""
     import torch
 
     import multiplescatteringcalculations_amd as msc
     from multiplescatteringcalculations_amd import learned_precond as lp
     from multiplescatteringcalculations_amd.slab_scattering import SlabPlan
 
+    cfg = dict(STRONG)
+    if pairs is not None:
+        cfg["pairs"] = pairs
     ref = msc.ReferenceMedium(VP, VS, RHO)
-    g = msc.SlabGeometry(M=96, N_z=24, a=A)
-    om = 0.75 * VS / A
+    g = msc.SlabGeometry(M=cfg["M"], N_z=cfg["N_z"], a=A)
+    om = cfg["ka"] * VS / A
     khat = np.array([np.sqrt(1.0 - (P_SLOW * VP) ** 2), P_SLOW * VP, 0.0])
     plan = SlabPlan(g)
     plan.build_g0(om, ref)
-    sets = []
-    for seed in range(1, 5):
-        plan.set_material(msc.random_iid_slab_material(g, ref, 0.5, 0.4, seed=seed), om)
-        sets.append(lp.harvest_pairs(plan, plan.incident_field(om, ref, khat, "P").reshape(-1),
-                                     restart=RESTART, tol=TOL))
-    pairs = lp.merge_pairs(sets)
-    net = lp.LearnedPreconditioner(hidden, seed=0)
-    _, before, after = lp.train(net, pairs, steps=300, lr=2e-3, batch=2, holdout=8)
-    plan.set_material(msc.random_iid_slab_material(g, ref, 0.5, 0.4, seed=0), om)
-    b = plan.incident_field(om, ref, khat, "P").reshape(-1)
-    out = {"slab": "96x96x24, phi 0.5, contrasts +-0.4, ka 0.75", "hidden": hidden,
-           "pairs": len(pairs), "holdout_loss_before": before, "holdout_loss_after": after}
-    for label, kw in (("fp64", dict(precision="fp64")), ("mixed", dict(precision="mixed")),
-                      ("learned_mixed", dict(precision="mixed", precond=net))):
-        best = None
-        for _ in range(3):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            _, st = plan.gmres(b, "born", tol=TOL, max_iter=2000, restart=RESTART, **kw)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-        out[label + "_ms"] = round(best * 1e3, 2)
-        out[label + "_iterations"] = int(st["iterations"])
+
+    def draw(seed):
+        return msc.random_iid_slab_material(g, ref, cfg["phi"], cfg["contrast"], seed=seed)
+
+    net = lp.LearnedPreconditioner(seed=0, arch="unet", widths=cfg["widths"])
+    info = lp.train_on_slabs(net, plan, draw, om, ref, khat, target_pairs=cfg["pairs"],
+                             seeds_per_round=cfg["seeds_per_round"], batch=cfg["batch"],
+                             lr=cfg["lr"], log=log)
+    plan.set_material(draw(200000), om)
+    val = net.validate(plan, plan.incident_field(om, ref, khat, "P").reshape(-1), tol=TOL,
+                       restart=RESTART)
+    out = {"slab": "96x96x24, phi 0.5, contrasts +-0.4, ka 0.75, p 0.15 s/km",
+           "network": f"unet {cfg['widths']}", "params": net.num_params,
+           "training": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()},
+           "validation_held_out_seed_200000": {k: (round(v, 3) if isinstance(v, float) else v)
+                                              for k, v in val.items()}}
+    # the network's own cost per application (complex64, as in the solve)
+    r = torch.randn(plan.n, dtype=torch.complex64, device=plan.device)
+    for _ in range(5):
+        net.apply(plan, r)
+    torch.cuda.synchronize()
+    for seed in (0, 100000):
+        plan.set_material(draw(seed), om)
+        b = plan.incident_field(om, ref, khat, "P").reshape(-1)
+        row = {}
+        xs = {}
+        for label, kw in (("fp64", dict(precision="fp64")), ("mixed", dict(precision="mixed")),
+                          ("learned_mixed", dict(precision="mixed", precond=net))):
+            best = None
+            for _ in range(3):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                x, st = plan.gmres(b, "born", tol=TOL, max_iter=3000, restart=RESTART, **kw)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            xs[label] = x
+            row[label + "_ms"] = round(best * 1e3, 2)
+            row[label + "_iterations"] = int(st["iterations"])
+            row[label + "_rel_residual"] = float(st["rel_residual"])
+            if label == "learned_mixed":
+                row["learned_fell_back"] = int(st["fell_back"])
+        row["learned_vs_fp64_max_rel_diff"] = float((xs["learned_mixed"] - xs["fp64"]).abs().max()
+                                                    / xs["fp64"].abs().max())
+        row["learned_speedup_vs_mixed"] = round(row["mixed_ms"] / row["learned_mixed_ms"], 3)
+        row["learned_speedup_vs_fp64"] = round(row["fp64_ms"] / row["learned_mixed_ms"], 3)
+        out[f"seed_{seed}"] = row
     log(f"strong contrast: {out}")
     return out
 
@@ -496,9 +530,12 @@ def main():
     t64, st64, x64 = time_solves(plan, psi0, "fp64", args.steps, args.warmup)
     log("mixed (complex64 cycles in FP64 refinement):")
     tmx, stmx, xmx = time_solves(plan, psi0, "mixed", args.steps, args.warmup)
-    log("learned + mixed (flexible GMRES with the trained network):")
-    tln, stln, xln = time_solves(plan, psi0, "mixed", args.steps, args.warmup, precond=net,
-                                 label="learned+mixed")
+    log("learned + mixed (the validated network in flexible GMRES when its gate is open):")
+    tln, stln, xln = time_solves(plan, psi0, "mixed", args.steps, args.warmup,
+                                 precond=net if net.gate_open else None, label="learned+mixed")
+    log("flexible GMRES with the network forced on (reported beside the headline):")
+    ton, ston, _ = time_solves(plan, psi0, "mixed", args.steps, args.warmup, precond=net,
+                               label="network on")
     diff = float((xmx - x64).abs().max() / x64.abs().max())
     diff_ln = float((xln - x64).abs().max() / x64.abs().max())
     # R_PP of each timed solve against an FP64 solve to 1e-12
@@ -574,10 +611,12 @@ def main():
         "dtype": "c64+c128",
         "data": "synthetic (seeded random slab, Bernoulli(0.3) inclusions, U[-0.2,0.2] contrasts)",
         "config": {"workload": "slab 128x128x32, pad 64x256x256, ka=0.4, p=0.15 s/km, tol 1e-8 "
-                               "FP64; flexible GMRES(30) with the learned conv preconditioner "
-                               f"(hidden {HIDDEN}) around complex64 cycles in FP64 refinement",
+                               "FP64; GMRES(30) with complex64 cycles in FP64 refinement and the "
+                               "learned four-level conv preconditioner in flexible GMRES when its "
+                               "validation gate is open",
                    "M": M, "N_z": NZ, "restart": RESTART, "tol": TOL,
-                   "precision": "mixed", "preconditioner": f"learned conv3d h={HIDDEN}"},
+                   "precision": "mixed", "preconditioner": f"learned unet {UNET_WIDTHS}",
+                   "gate_open": bool(net.gate_open)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": {
@@ -585,6 +624,9 @@ def main():
             "mixed_s_per_solve": round(mean_mx, 6),
             "speedup_vs_fp64": round(mean_64 / mean_ln, 3),
             "mixed_speedup_vs_fp64": round(mean_64 / mean_mx, 3),
+            "network_on_s_per_solve": round(float(np.mean(ton)), 6),
+            "network_on_iterations": ston["iterations"],
+            "network_on_precond_applies": ston["precond_applies"],
             "learned_iterations": stln["iterations"], "learned_fell_back": stln["fell_back"],
             "learned_precond_applies": stln["precond_applies"],
             "learned_vs_fp64_max_rel_diff": diff_ln, "learned": learn_info,

@@ -30,14 +30,11 @@ typedef short v8s __attribute__((ext_vector_type(8)));
 typedef __bf16 v8b __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-#ifndef MSC_NET_VT
-#define MSC_NET_VT 4
-#endif
 #ifndef MSC_NET_ZC
 #define MSC_NET_ZC 4
 #endif
 constexpr int kNetNT = 256;         // threads per workgroup (4 waves)
-constexpr int kVT = MSC_NET_VT;     // 16-voxel tiles per wave of the row-tiled kernels
+constexpr int kVT = 4;              // default 16-voxel tiles per wave of the row-tiled kernels
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
   // round to nearest even
@@ -86,7 +83,6 @@ struct ConvArgs {
 // 16 consecutive voxels fall on distinct banks; every tap then reads its B
 // fragments from LDS.  MIRROR negates the tap offsets (data gradient).
 constexpr int kTH = kNetNT / 64;  // rows per workgroup (one per wave)
-constexpr int kTW = 16 * kVT;     // voxels per row
 
 // Channels staged per voxel: the 32-channel network input and the last
 // layer's gradient carry only 18 non-zero channels, so 20 are staged and the
@@ -95,11 +91,11 @@ constexpr int kTW = 16 * kVT;     // voxels per row
 // consecutive voxels on 16 distinct bank pairs.
 constexpr int kIOStaged = 20;  // staged channels of an 18-channel (padded to 32) input
 
-template <int NS_>
+template <int NS_, int VT = kVT>
 struct Halo {
   static constexpr int NS = NS_;
   static constexpr int LDV = ((NS / 4) % 2 == 1) ? NS : NS + 4;
-  static constexpr int ROWS = kTH + 2, COLS = kTW + 2;
+  static constexpr int ROWS = kTH + 2, COLS = 16 * VT + 2;
   static constexpr int ELEMS = 3 * ROWS * COLS * LDV;
 };
 
@@ -117,25 +113,25 @@ __device__ __forceinline__ void load_wf(v4s (&wf)[NO / 16][NI / 16], const uint1
 
 // The lane's first B-fragment element for tap offset (dz, dx, dy) in the
 // staged halo whose plane d sits in slot (sbase + 1) mod 3.
-template <int NS>
+template <int NS, int VT = kVT>
 __device__ __forceinline__ const uint16_t* tap_base(const uint16_t* sh, int sbase, int dz, int dx,
                                                     int dy) {
-  constexpr int LDV = Halo<NS>::LDV, COLS = Halo<NS>::COLS;
-  constexpr int PLANE = Halo<NS>::ROWS * COLS * LDV;
+  constexpr int LDV = Halo<NS, VT>::LDV, COLS = Halo<NS, VT>::COLS;
+  constexpr int PLANE = Halo<NS, VT>::ROWS * COLS * LDV;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   return sh + ((sbase + 1 + dz) % 3) * PLANE +
          ((1 + wave + dx) * COLS + (1 + dy + (lane & 15))) * LDV + (lane >> 4) * 4;
 }
 
 // The MFMAs of one tap over the wave's kVT voxel tiles.
-template <int NI, int NO, int NS>
-__device__ __forceinline__ void tap_mfma(v4f (&acc)[NO / 16][kVT],
+template <int NI, int NO, int NS, int VT = kVT>
+__device__ __forceinline__ void tap_mfma(v4f (&acc)[NO / 16][VT],
                                          const v4s (&wf)[NO / 16][NI / 16],
                                          const uint16_t* base) {
   constexpr int LDV = Halo<NS>::LDV;
   const int grp = (threadIdx.x & 63) >> 4;
 #pragma unroll
-  for (int t = 0; t < kVT; ++t) {
+  for (int t = 0; t < VT; ++t) {
 #pragma unroll
     for (int ib = 0; ib < NI / 16; ++ib) {
       v4s bf = v4s{0, 0, 0, 0};
@@ -149,8 +145,8 @@ __device__ __forceinline__ void tap_mfma(v4f (&acc)[NO / 16][kVT],
 }
 
 // Epilogue EPI of one output plane d: rows h0 + wave, voxels w0 .. w0 + 63.
-template <int NO, int EPI>
-__device__ __forceinline__ void conv_store(const ConvArgs& a, const v4f (&acc)[NO / 16][kVT],
+template <int NO, int EPI, int VT = kVT>
+__device__ __forceinline__ void conv_store(const ConvArgs& a, const v4f (&acc)[NO / 16][VT],
                                            int d, int h0, int w0) {
   constexpr int NOT = NO / 16;
   const int H = a.H, W = a.W;
@@ -162,7 +158,7 @@ __device__ __forceinline__ void conv_store(const ConvArgs& a, const v4f (&acc)[N
   if constexpr (EPI == kEpiApply64 || EPI == kEpiApply128)
     inv_s = 1.f / (a.scale > 0.f ? a.scale : scale_of(a.scal));
 #pragma unroll
-  for (int t = 0; t < kVT; ++t) {
+  for (int t = 0; t < VT; ++t) {
     const int w = w0 + t * 16 + col;
     if (w >= W) continue;
     const size_t v = ((size_t)d * H + h) * W + w;
@@ -222,15 +218,15 @@ __device__ __forceinline__ void conv_store(const ConvArgs& a, const v4f (&acc)[N
 // from the three staged input planes d-1, d, d+1, which sit in LDS plane
 // slots (sbase + 0, 1, 2) mod 3 of sh.  MFMA over the 27 taps, then the
 // epilogue EPI.  The weight fragments are loaded per tap.
-template <int NI, int NO, int EPI, bool MIRROR, int NS>
+template <int NI, int NO, int EPI, bool MIRROR, int NS, int VT = kVT>
 __device__ __forceinline__ void conv_plane(const ConvArgs& a, const uint16_t* sh, int sbase,
                                            int d, int h0, int w0) {
   constexpr int NOT = NO / 16, NIB = NI / 16;
-  v4f acc[NOT][kVT];
+  v4f acc[NOT][VT];
 #pragma unroll
   for (int o = 0; o < NOT; ++o)
 #pragma unroll
-    for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < VT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 9
   for (int tap = 0; tap < 27; ++tap) {
     int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
@@ -241,9 +237,9 @@ __device__ __forceinline__ void conv_plane(const ConvArgs& a, const uint16_t* sh
     }
     v4s wf[NOT][NIB];
     load_wf<NI, NO>(wf, a.w, tap);
-    tap_mfma<NI, NO, NS>(acc, wf, tap_base<NS>(sh, sbase, dz, dx, dy));
+    tap_mfma<NI, NO, NS, VT>(acc, wf, tap_base<NS, VT>(sh, sbase, dz, dx, dy));
   }
-  conv_store<NO, EPI>(a, acc, d, h0, w0);
+  conv_store<NO, EPI, VT>(a, acc, d, h0, w0);
 }
 
 // Forward layers contract over K = (tap, input channel), 27 NI long, in steps
@@ -272,8 +268,8 @@ __device__ __forceinline__ void load_wf32(v8s (&wf)[NO / 16], const uint16_t* w,
 
 // The MFMAs of step s over the wave's kVT voxel tiles; B fragments from the
 // staged halo (two 8-byte reads; channels at or above NS are zero).
-template <int NI, int NO, int NS>
-__device__ __forceinline__ void step_mfma32(v4f (&acc)[NO / 16][kVT], const v8s (&wf)[NO / 16],
+template <int NI, int NO, int NS, int VT = kVT>
+__device__ __forceinline__ void step_mfma32(v4f (&acc)[NO / 16][VT], const v8s (&wf)[NO / 16],
                                             const uint16_t* sh, int sbase, int s) {
   constexpr int LDV = Halo<NS>::LDV;
   const int lane = threadIdx.x & 63;
@@ -282,10 +278,10 @@ __device__ __forceinline__ void step_mfma32(v4f (&acc)[NO / 16][kVT], const v8s 
   const bool live = tap < 27;
   const int tq = live ? tap : 26;  // an address inside the halo for the zero half
   const uint16_t* base =
-      tap_base<NS>(sh, sbase, tq / 9 - 1, (tq / 3) % 3 - 1, tq % 3 - 1) - (lane >> 4) * 4 + ch;
+      tap_base<NS, VT>(sh, sbase, tq / 9 - 1, (tq / 3) % 3 - 1, tq % 3 - 1) - (lane >> 4) * 4 + ch;
   const bool lo_ok = live && ch < NS, hi_ok = live && ch + 4 < NS;
 #pragma unroll
-  for (int t = 0; t < kVT; ++t) {
+  for (int t = 0; t < VT; ++t) {
     v4s lo = v4s{0, 0, 0, 0}, hi = v4s{0, 0, 0, 0};
     if (lo_ok) lo = *reinterpret_cast<const v4s*>(base + t * 16 * LDV);
     if (hi_ok) hi = *reinterpret_cast<const v4s*>(base + t * 16 * LDV + 4);
@@ -305,7 +301,7 @@ enum WeightSrc : int {
 };
 
 // One output plane of the forward (z-marching) convolution, K = 32 steps.
-template <int NI, int NO, int EPI, int NS, int WSRC>
+template <int NI, int NO, int EPI, int NS, int WSRC, int VT = kVT>
 __device__ __forceinline__ void conv_plane32(const ConvArgs& a,
                                              const v8s (&wr)[WSRC == kWReg ? k32_steps<NI>() : 1]
                                                             [NO / 16],
@@ -313,14 +309,14 @@ __device__ __forceinline__ void conv_plane32(const ConvArgs& a,
                                              int h0, int w0) {
   constexpr int NOT = NO / 16, S = k32_steps<NI>();
   const int lane = threadIdx.x & 63;
-  v4f acc[NOT][kVT];
+  v4f acc[NOT][VT];
 #pragma unroll
   for (int o = 0; o < NOT; ++o)
 #pragma unroll
-    for (int t = 0; t < kVT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < VT; ++t) acc[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
   if constexpr (WSRC == kWReg) {
 #pragma unroll
-    for (int st = 0; st < S; ++st) step_mfma32<NI, NO, NS>(acc, wr[st], sh, sbase, st);
+    for (int st = 0; st < S; ++st) step_mfma32<NI, NO, NS, VT>(acc, wr[st], sh, sbase, st);
   } else {
 #pragma unroll 9
     for (int st = 0; st < S; ++st) {
@@ -331,28 +327,28 @@ __device__ __forceinline__ void conv_plane32(const ConvArgs& a,
       } else {
         load_wf32<NI, NO>(wf, a.w, st);
       }
-      step_mfma32<NI, NO, NS>(acc, wf, sh, sbase, st);
+      step_mfma32<NI, NO, NS, VT>(acc, wf, sh, sbase, st);
     }
   }
-  conv_store<NO, EPI>(a, acc, d, h0, w0);
+  conv_store<NO, EPI, VT>(a, acc, d, h0, w0);
 }
 
 // NS: channels of the input that can be non-zero (staged); NI: channels stored.
-template <int NI, int NO, int EPI, bool MIRROR, int NS = NI>
+template <int NI, int NO, int EPI, bool MIRROR, int NS = NI, int VT = kVT>
 __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
   static_assert(NS % 4 == 0 && NS <= NI, "staged channels must be a multiple of 4");
-  constexpr int LDV = Halo<NS>::LDV;
-  constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
-  __shared__ __align__(16) uint16_t sh[Halo<NS>::ELEMS];
+  constexpr int LDV = Halo<NS, VT>::LDV;
+  constexpr int ROWS = Halo<NS, VT>::ROWS, COLS = Halo<NS, VT>::COLS;
+  __shared__ __align__(16) uint16_t sh[Halo<NS, VT>::ELEMS];
   if (a.done != nullptr && *a.done) return;
   const int D = a.D, H = a.H, W = a.W;
-  const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
+  const int nbw = (W + 16 * VT - 1) / (16 * VT), nbh = (H + kTH - 1) / kTH;
   int b = blockIdx.x;
   const int bw = b % nbw;
   b /= nbw;
   const int bh = b % nbh;
   const int d = b / nbh;
-  const int h0 = bh * kTH, w0 = bw * kTW;
+  const int h0 = bh * kTH, w0 = bw * 16 * VT;
   // stage the halo: 4-channel groups (8 bytes), zero outside the grid.  The
   // loads go out in batches of SB before their LDS stores, so a thread waits
   // for memory once per batch rather than once per element.
@@ -386,7 +382,7 @@ __global__ __launch_bounds__(kNetNT) void k_conv(ConvArgs a) {
       if (off[k] >= 0) *reinterpret_cast<v4s*>(&sh[off[k]]) = val[k];
   }
   __syncthreads();
-  conv_plane<NI, NO, EPI, MIRROR, NS>(a, sh, 0, d, h0, w0);
+  conv_plane<NI, NO, EPI, MIRROR, NS, VT>(a, sh, 0, d, h0, w0);
 }
 
 // The network's input at voxel v: (s / t_max) T0 r as kNetIO bf16 channels,
@@ -433,13 +429,13 @@ __device__ __forceinline__ float input_scale(const double* scal, float scale,
 // application) stages the network input computed from the complex64 /
 // complex128 vector a.r and T0 parameters a.tp, as k_net_gather would
 // write it, so the application needs no separate gather pass.
-template <int NI, int NO, int EPI, int NS = NI, int SRC = 0>
+template <int NI, int NO, int EPI, int NS = NI, int SRC = 0, int VT = kVT>
 __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
   static_assert(NS % 4 == 0 && NS <= NI, "staged channels must be a multiple of 4");
   static_assert(SRC == 0 || (NS == 20 && NI == kNetIOPad), "the fused input stages 20 channels");
   using RS = typename std::conditional<SRC == 2, double, float>::type;
-  constexpr int LDV = Halo<NS>::LDV;
-  constexpr int ROWS = Halo<NS>::ROWS, COLS = Halo<NS>::COLS;
+  constexpr int LDV = Halo<NS, VT>::LDV;
+  constexpr int ROWS = Halo<NS, VT>::ROWS, COLS = Halo<NS, VT>::COLS;
   constexpr int PLANE = ROWS * COLS * LDV;
   constexpr int G4 = NS / 4;
   constexpr int NPL = ROWS * COLS * G4;
@@ -479,14 +475,14 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
     }
   }
   const int D = a.D, H = a.H, W = a.W;
-  const int nbw = (W + kTW - 1) / kTW, nbh = (H + kTH - 1) / kTH;
+  const int nbw = (W + 16 * VT - 1) / (16 * VT), nbh = (H + kTH - 1) / kTH;
   int b = blockIdx.x;
   const int bw = b % nbw;
   b /= nbw;
   const int bh = b % nbh;
   const int d0 = (b / nbh) * zc;
   const int d1 = min(D, d0 + zc);
-  const int h0 = bh * kTH, w0 = bw * kTW;
+  const int h0 = bh * kTH, w0 = bw * 16 * VT;
   // per thread: PT groups of 4 channels (bf16 grid) or PV voxels of 5 groups
   constexpr int NVX = ROWS * COLS, PV = (NVX + kNetNT - 1) / kNetNT;
   v4s val[SRC == 0 ? PT : PV * 5];
@@ -566,9 +562,9 @@ __global__ __launch_bounds__(kNetNT) void k_conv_zm(ConvArgs a, int zc) {
     __syncthreads();
     if (d + 1 < d1) load(d + 2);
     if constexpr (K32)
-      conv_plane32<NI, NO, EPI, NS, WSRC>(a, wr, wl, sh, (d + 2) % 3, d, h0, w0);
+      conv_plane32<NI, NO, EPI, NS, WSRC, VT>(a, wr, wl, sh, (d + 2) % 3, d, h0, w0);
     else
-      conv_plane<NI, NO, EPI, false, NS>(a, sh, (d + 2) % 3, d, h0, w0);
+      conv_plane<NI, NO, EPI, false, NS, VT>(a, sh, (d + 2) % 3, d, h0, w0);
     __syncthreads();
   }
 }
@@ -767,18 +763,21 @@ __device__ __forceinline__ void dir_store(const DirArgs& a, const v4f& acc, size
 }
 
 // 3x3x3 "same" convolution (MIRROR: negated taps, the data gradient with
-// transposed weights); one wave per (16-voxel tile, 16-channel output block).
+// transposed weights).  One workgroup per (16-voxel tile, 16-channel output
+// block); its four waves split the 27 taps (7, 7, 7, 6), issue all their
+// weight and activation loads before the MFMAs (one memory round trip per
+// wave instead of a chain of them), and wave 0 adds the four partial tiles in
+// a fixed order (LDS) before the epilogue.
 template <int NI, int NO, int EPI, bool MIRROR>
 __global__ __launch_bounds__(kNetNT) void k_conv_dir(DirArgs a) {
   if (a.done != nullptr && *a.done) return;
-  constexpr int NOT = NO / 16, NIB = NI / 16;
+  constexpr int NOT = NO / 16, NIB = NI / 16, TPW = 7;
+  __shared__ v4f part[kNetNT / 64][64];
   const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const int wave = threadIdx.x >> 6;
   const size_t nv = (size_t)a.D * a.H * a.W;
-  const size_t ntile = (nv + 15) / 16;
-  const size_t wid = (size_t)blockIdx.x * (kNetNT / 64) + (threadIdx.x >> 6);
-  if (wid >= ntile * NOT) return;
-  const int ob = (int)(wid % NOT);
-  const size_t v = (wid / NOT) * 16 + col;
+  const int ob = (int)(blockIdx.x % NOT);
+  const size_t v = (size_t)(blockIdx.x / NOT) * 16 + col;
   const bool live = v < nv;
   int vw = 0, vh = 0, vd = 0;
   if (live) {
@@ -787,9 +786,10 @@ __global__ __launch_bounds__(kNetNT) void k_conv_dir(DirArgs a) {
     vh = (int)(t % a.H);
     vd = (int)(t / a.H);
   }
-  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 9
-  for (int tap = 0; tap < 27; ++tap) {
+  v4s bf[TPW][NIB], af[TPW][NIB];
+#pragma unroll
+  for (int k = 0; k < TPW; ++k) {
+    const int tap = wave * TPW + k;
     int dz = tap / 9 - 1, dx = (tap / 3) % 3 - 1, dy = tap % 3 - 1;
     if (MIRROR) {
       dz = -dz;
@@ -797,20 +797,36 @@ __global__ __launch_bounds__(kNetNT) void k_conv_dir(DirArgs a) {
       dy = -dy;
     }
     const int nd = vd + dz, nh = vh + dx, nw = vw + dy;
-    const bool ok = live && nd >= 0 && nd < a.D && nh >= 0 && nh < a.H && nw >= 0 && nw < a.W;
+    const bool ok = tap < 27 && live && nd >= 0 && nd < a.D && nh >= 0 && nh < a.H && nw >= 0 &&
+                    nw < a.W;
     const uint16_t* src = a.in + (((size_t)(ok ? nd : 0) * a.H + (ok ? nh : 0)) * a.W +
                                   (ok ? nw : 0)) * NI + grp * 4;
-    const uint16_t* wsrc = a.w + ((size_t)tap * NO + ob * 16 + col) * NI + grp * 4;
+    const uint16_t* wsrc = a.w + ((size_t)(tap < 27 ? tap : 0) * NO + ob * 16 + col) * NI + grp * 4;
 #pragma unroll
     for (int ib = 0; ib < NIB; ++ib) {
-      v4s bf = v4s{0, 0, 0, 0};
-      if (ok) bf = *reinterpret_cast<const v4s*>(src + ib * 16);
-      const v4s af = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf, acc, 0, 0, 0);
+      bf[k][ib] = v4s{0, 0, 0, 0};
+      af[k][ib] = v4s{0, 0, 0, 0};
+      if (ok) bf[k][ib] = *reinterpret_cast<const v4s*>(src + ib * 16);
+      if (tap < 27) af[k][ib] = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
     }
   }
+  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < TPW; ++k)
+#pragma unroll
+    for (int ib = 0; ib < NIB; ++ib)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[k][ib], bf[k][ib], acc, 0, 0, 0);
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave != 0) return;
+  v4f sum = part[0][lane];
+#pragma unroll
+  for (int w = 1; w < kNetNT / 64; ++w) {
+    const v4f q = part[w][lane];
+    sum = v4f{sum[0] + q[0], sum[1] + q[1], sum[2] + q[2], sum[3] + q[3]};
+  }
   // D[4 grp + i][col]: output channel ob*16 + 4 grp + i of voxel v
-  if (live) dir_store<NO, EPI>(a, acc, v, ob * 16 + grp * 4);
+  if (live) dir_store<NO, EPI>(a, sum, v, ob * 16 + grp * 4);
 }
 
 // Fine voxel f(c, s) of coarse voxel c (coords cd, ch, cw), or -1 outside.
@@ -822,17 +838,18 @@ __device__ __forceinline__ long long fine_at(int cd, int ch, int cw, int s, cons
 
 // fine -> coarse: out[c][o] = epi(sum_{s, i} w[s][o][i] in[f(c, s)][i]); EPI
 // kDirHidden (a down layer) or kDirBackward (the data gradient of an up layer).
+// One workgroup per (16 coarse voxels, 16-channel output block); wave w takes
+// the sub-positions s = 2w, 2w + 1 and wave 0 adds the partial tiles in order.
 template <int NI, int NO, int EPI>
 __global__ __launch_bounds__(kNetNT) void k_down_mfma(DirArgs a) {
   if (a.done != nullptr && *a.done) return;
   constexpr int NOT = NO / 16, NIB = NI / 16;
+  __shared__ v4f part[kNetNT / 64][64];
   const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const int wave = threadIdx.x >> 6;
   const size_t nc = (size_t)a.D * a.H * a.W;
-  const size_t ntile = (nc + 15) / 16;
-  const size_t wid = (size_t)blockIdx.x * (kNetNT / 64) + (threadIdx.x >> 6);
-  if (wid >= ntile * NOT) return;
-  const int ob = (int)(wid % NOT);
-  const size_t c = (wid / NOT) * 16 + col;
+  const int ob = (int)(blockIdx.x % NOT);
+  const size_t c = (size_t)(blockIdx.x / NOT) * 16 + col;
   const bool live = c < nc;
   int cw = 0, ch = 0, cd = 0;
   if (live) {
@@ -841,37 +858,51 @@ __global__ __launch_bounds__(kNetNT) void k_down_mfma(DirArgs a) {
     ch = (int)(t % a.H);
     cd = (int)(t / a.H);
   }
-  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+  v4s bf[2][NIB], af[2][NIB];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int k = 0; k < 2; ++k) {
+    const int s = 2 * wave + k;
     const long long f = live ? fine_at(cd, ch, cw, s, a) : -1;
     const uint16_t* src = a.in + (size_t)(f >= 0 ? f : 0) * NI + grp * 4;
     const uint16_t* wsrc = a.w + ((size_t)s * NO + ob * 16 + col) * NI + grp * 4;
 #pragma unroll
     for (int ib = 0; ib < NIB; ++ib) {
-      v4s bf = v4s{0, 0, 0, 0};
-      if (f >= 0) bf = *reinterpret_cast<const v4s*>(src + ib * 16);
-      const v4s af = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf, acc, 0, 0, 0);
+      bf[k][ib] = v4s{0, 0, 0, 0};
+      if (f >= 0) bf[k][ib] = *reinterpret_cast<const v4s*>(src + ib * 16);
+      af[k][ib] = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
     }
   }
-  if (live) dir_store<NO, EPI>(a, acc, c, ob * 16 + grp * 4);
+  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int ib = 0; ib < NIB; ++ib)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[k][ib], bf[k][ib], acc, 0, 0, 0);
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave != 0) return;
+  v4f sum = part[0][lane];
+#pragma unroll
+  for (int w = 1; w < kNetNT / 64; ++w) {
+    const v4f q = part[w][lane];
+    sum = v4f{sum[0] + q[0], sum[1] + q[1], sum[2] + q[2], sum[3] + q[3]};
+  }
+  if (live) dir_store<NO, EPI>(a, sum, c, ob * 16 + grp * 4);
 }
 
 // coarse -> fine: out[f(c, s)][o] = epi(sum_i w[s][o][i] in[c][i]); EPI
 // kDirUp (an up layer) or kDirDownBack (the data gradient of a down layer).
-// The wave's 16 coarse voxels are loaded once and reused for all 8 s.
+// One workgroup per (16 coarse voxels, 16-channel output block); wave w
+// writes the sub-positions s = 2w, 2w + 1 of its 16 coarse voxels.
 template <int NI, int NO, int EPI>
 __global__ __launch_bounds__(kNetNT) void k_up_mfma(DirArgs a) {
   if (a.done != nullptr && *a.done) return;
   constexpr int NOT = NO / 16, NIB = NI / 16;
   const int lane = threadIdx.x & 63, col = lane & 15, grp = lane >> 4;
+  const int wave = threadIdx.x >> 6;
   const size_t nc = (size_t)a.D * a.H * a.W;
-  const size_t ntile = (nc + 15) / 16;
-  const size_t wid = (size_t)blockIdx.x * (kNetNT / 64) + (threadIdx.x >> 6);
-  if (wid >= ntile * NOT) return;
-  const int ob = (int)(wid % NOT);
-  const size_t c = (wid / NOT) * 16 + col;
+  const int ob = (int)(blockIdx.x % NOT);
+  const size_t c = (size_t)(blockIdx.x / NOT) * 16 + col;
   const bool live = c < nc;
   int cw = 0, ch = 0, cd = 0;
   if (live) {
@@ -880,21 +911,23 @@ __global__ __launch_bounds__(kNetNT) void k_up_mfma(DirArgs a) {
     ch = (int)(t % a.H);
     cd = (int)(t / a.H);
   }
-  v4s bf[NIB];
+  v4s bf[NIB], af[2][NIB];
 #pragma unroll
   for (int ib = 0; ib < NIB; ++ib) {
     bf[ib] = v4s{0, 0, 0, 0};
     if (live) bf[ib] = *reinterpret_cast<const v4s*>(a.in + c * NI + ib * 16 + grp * 4);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      af[k][ib] = *reinterpret_cast<const v4s*>(
+          a.w + ((size_t)(2 * wave + k) * NO + ob * 16 + col) * NI + ib * 16 + grp * 4);
   }
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int k = 0; k < 2; ++k) {
+    const int s = 2 * wave + k;
     v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
-    const uint16_t* wsrc = a.w + ((size_t)s * NO + ob * 16 + col) * NI + grp * 4;
 #pragma unroll
-    for (int ib = 0; ib < NIB; ++ib) {
-      const v4s af = *reinterpret_cast<const v4s*>(wsrc + ib * 16);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[ib], acc, 0, 0, 0);
-    }
+    for (int ib = 0; ib < NIB; ++ib)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[k][ib], bf[ib], acc, 0, 0, 0);
     const long long f = live ? fine_at(cd, ch, cw, s, a) : -1;
     if (f >= 0) dir_store<NO, EPI>(a, acc, (size_t)f, ob * 16 + grp * 4);
   }
@@ -1285,13 +1318,24 @@ ConvArgs conv_args(const Net& net, int lvl, const uint16_t* in, const uint16_t* 
   return a;
 }
 
+// Tiles per wave of the row-tiled kernels for a grid row of W voxels: 2 (rows of
+// 32 voxels) when that leaves fewer idle voxels than rows of 64, else 4 (at
+// W = 96 the 32-voxel rows took a hidden 16-channel layer from 21 to 14 us;
+// at W = 128 the 64-voxel rows are faster).
+int row_vt(int W) {
+  const int p4 = (W + 63) / 64 * 64, p2 = (W + 31) / 32 * 32;
+  return p2 < p4 ? 2 : 4;
+}
+
 dim3 conv_grid(const ConvArgs& a) {
-  const int nbw = (a.W + kTW - 1) / kTW, nbh = (a.H + kTH - 1) / kTH;
+  const int tw = 16 * row_vt(a.W);
+  const int nbw = (a.W + tw - 1) / tw, nbh = (a.H + kTH - 1) / kTH;
   return dim3((unsigned)(a.D * nbh * nbw));
 }
 
 dim3 conv_grid_zm(const ConvArgs& a, int zc) {
-  const int nbw = (a.W + kTW - 1) / kTW, nbh = (a.H + kTH - 1) / kTH;
+  const int tw = 16 * row_vt(a.W);
+  const int nbw = (a.W + tw - 1) / tw, nbh = (a.H + kTH - 1) / kTH;
   return dim3((unsigned)(((a.D + zc - 1) / zc) * nbh * nbw));
 }
 
@@ -1318,8 +1362,21 @@ dim3 conv_grid_zm(const ConvArgs& a, int zc) {
 
 template <int NI, int NO, int EPI, int NS = NI, int SRC = 0>
 void launch_zm(const ConvArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_conv_zm<NI, NO, EPI, NS, SRC>), conv_grid_zm(a, kConvZc), dim3(kNetNT), 0,
-                     s, a, kConvZc);
+  if (row_vt(a.W) == 2)
+    hipLaunchKernelGGL((k_conv_zm<NI, NO, EPI, NS, SRC, 2>), conv_grid_zm(a, kConvZc),
+                       dim3(kNetNT), 0, s, a, kConvZc);
+  else
+    hipLaunchKernelGGL((k_conv_zm<NI, NO, EPI, NS, SRC, 4>), conv_grid_zm(a, kConvZc),
+                       dim3(kNetNT), 0, s, a, kConvZc);
+}
+
+// Data gradient (mirrored taps, transposed weights) on the row-tiled kernel.
+template <int NI, int NO, int EPI, int NS>
+void launch_dgrad(const ConvArgs& a, hipStream_t s) {
+  if (row_vt(a.W) == 2)
+    hipLaunchKernelGGL((k_conv<NI, NO, EPI, true, NS, 2>), conv_grid(a), dim3(kNetNT), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_conv<NI, NO, EPI, true, NS, 4>), conv_grid(a), dim3(kNetNT), 0, s, a);
 }
 
 // Forward of convolution layer k.  src: 0 = bf16 input grid, 1/2 = computed from
@@ -1366,27 +1423,31 @@ DirArgs dir_args(const Net& net, int lvl, const uint16_t* in, const uint16_t* w,
   return r;
 }
 
-// One wave per (16-voxel tile of the grid a.D x a.H x a.W, output block).
-dim3 dir_grid(const DirArgs& a, int nout) {
-  const size_t tiles = ((size_t)a.D * a.H * a.W + 15) / 16;
-  return dim3((unsigned)((tiles * (nout / 16) + kNetNT / 64 - 1) / (kNetNT / 64)));
-}
 
 // k_conv_dir / k_down_mfma / k_up_mfma with run-time widths.
 template <int EPI, bool MIRROR>
-void launch_conv_dir(int pin, int pout, const DirArgs& a, hipStream_t s) {
-  MSC_WIDTH_SWITCH(pin, {
-    constexpr int PI = HW;
-    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_conv_dir<PI, HW, EPI, MIRROR>), dir_grid(a, HW),
-                                              dim3(kNetNT), 0, s, a))
-  })
+void launch_conv_dir(int pin, int pout, const DirArgs& a, hipStream_t s);
+
+// One workgroup per (16-voxel tile of the coarse grid, 16-channel output block).
+dim3 tile_grid(const DirArgs& a, int nout) {
+  const size_t tiles = ((size_t)a.D * a.H * a.W + 15) / 16;
+  return dim3((unsigned)(tiles * (nout / 16)));
 }
 
 template <int EPI>
 void launch_down(int pin, int pout, const DirArgs& a, hipStream_t s) {
   MSC_WIDTH_SWITCH(pin, {
     constexpr int PI = HW;
-    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_down_mfma<PI, HW, EPI>), dir_grid(a, HW),
+    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_down_mfma<PI, HW, EPI>), tile_grid(a, HW),
+                                              dim3(kNetNT), 0, s, a))
+  })
+}
+
+template <int EPI, bool MIRROR>
+void launch_conv_dir(int pin, int pout, const DirArgs& a, hipStream_t s) {
+  MSC_WIDTH_SWITCH(pin, {
+    constexpr int PI = HW;
+    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_conv_dir<PI, HW, EPI, MIRROR>), tile_grid(a, HW),
                                               dim3(kNetNT), 0, s, a))
   })
 }
@@ -1395,7 +1456,7 @@ template <int EPI>
 void launch_up(int pin, int pout, const DirArgs& a, hipStream_t s) {
   MSC_WIDTH_SWITCH(pin, {
     constexpr int PI = HW;
-    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_up_mfma<PI, HW, EPI>), dir_grid(a, HW),
+    MSC_WIDTH_SWITCH(pout, hipLaunchKernelGGL((k_up_mfma<PI, HW, EPI>), tile_grid(a, HW),
                                               dim3(kNetNT), 0, s, a))
   })
 }
@@ -1502,19 +1563,13 @@ void backward_pass(Net& net, const uint16_t* in, hipStream_t s) {
         launch_conv_dir<kDirBackLin, true>(l.pout, l.pin, a, s);
     } else if (l.kind == kConv3) {
       ConvArgs a = conv_args(net, li, g, wt, nullptr, net.act[l.in], net.grad[l.in], nullptr);
-      const dim3 G = conv_grid(a);
       const bool mask = net.T[l.in].act;
-      if (k == last) {
-        MSC_WIDTH_SWITCH(l.pin, hipLaunchKernelGGL((k_conv<kNetIOPad, HW, kEpiBackward, true,
-                                                            kIOStaged>),
-                                                   G, dim3(kNetNT), 0, s, a))
-      } else if (mask) {
-        MSC_WIDTH_SWITCH(l.pin, hipLaunchKernelGGL((k_conv<HW, HW, kEpiBackward, true>), G,
-                                                   dim3(kNetNT), 0, s, a))
-      } else {
-        MSC_WIDTH_SWITCH(l.pin, hipLaunchKernelGGL((k_conv<HW, HW, kEpiBackwardLin, true>), G,
-                                                   dim3(kNetNT), 0, s, a))
-      }
+      if (k == last)
+        MSC_WIDTH_SWITCH(l.pin, launch_dgrad<kNetIOPad, HW, kEpiBackward, kIOStaged>(a, s))
+      else if (mask)
+        MSC_WIDTH_SWITCH(l.pin, launch_dgrad<HW, HW, kEpiBackward, HW>(a, s))
+      else
+        MSC_WIDTH_SWITCH(l.pin, launch_dgrad<HW, HW, kEpiBackwardLin, HW>(a, s))
     } else if (l.kind == kDown2) {
       const int su = net.T[l.in].skip_user;
       const uint16_t* gskip = su >= 0 ? net.grad[net.L[su].out] : nullptr;

@@ -101,6 +101,7 @@ class LearnedPreconditioner:
         n = ctypes.c_longlong()
         _lib.check(self.lib.msc_net_num_params(self._h, ctypes.byref(n)))
         self.num_params = int(n.value)
+        self.gate_open = True  # see validate()
         if seed is not None:
             self.set_params(init_params(self.hidden, seed, arch=arch, widths=self.widths))
 
@@ -184,6 +185,34 @@ class LearnedPreconditioner:
                                                    float(lr), ctypes.byref(loss), self._stream()))
         return loss.value
 
+    def validate(self, plan, b: torch.Tensor, *, repeats: int = 3, **solve_kw) -> dict:
+        """Decide whether this network pays for itself on the plan's workload:
+        solve ``plan`` x = b with the solver options ``solve_kw`` (default
+        GMRES(30), mixed precision, tol 1e-8) with and without the network,
+        best of ``repeats`` timed solves each, and open the gate
+        (``self.gate_open``) only when flexible GMRES with the network is faster
+        than the same solve without it.  The slab should be one held out from
+        training.  Returns the measurements."""
+        import time
+
+        kw = dict(tol=1e-8, max_iter=3000, restart=30, precision="mixed")
+        kw.update(solve_kw)
+        out = {}
+        for label, pc in (("without", None), ("with", self)):
+            best, st = None, None
+            for _ in range(repeats + 1):  # the first solve warms caches and graphs
+                torch.cuda.synchronize(plan.device)
+                t0 = time.perf_counter()
+                _, st = plan.gmres(b, "born", precond=pc, **kw)
+                torch.cuda.synchronize(plan.device)
+                dt = time.perf_counter() - t0
+                best = dt if best is None or dt < best else best
+            out[label + "_ms"] = best * 1e3
+            out[label + "_iterations"] = int(st["iterations"])
+        self.gate_open = out["with_ms"] < out["without_ms"]
+        out["gate_open"] = self.gate_open
+        return out
+
     def _batch(self, x, target):
         if x.dim() == 4:
             x = x.unsqueeze(0)
@@ -240,13 +269,18 @@ class PairSet:
 
 
 def harvest_pairs(plan, b: torch.Tensor, *, restart: int = 30, tol: float = 1e-8,
-                  max_pairs: int | None = None) -> PairSet:
-    """Run plain FP64 GMRES(restart) on (I - G0 T0) x = b and turn the last
-    cycle's orthonormal Krylov basis v_0..v_k into pairs
-    (input (s / t_max) T0 (A v), target s (v - A v)), s = sqrt(n)/||A v||, so that the
-    network learns the correction that maps A v back to v."""
+                  max_pairs: int | None = None, precond: "LearnedPreconditioner | None" = None
+                  ) -> PairSet:
+    """Run one FP64 GMRES(restart) cycle on (I - G0 T0) x = b -- plain, or
+    flexible with the right preconditioner ``precond`` -- and turn its
+    orthonormal Krylov basis v_0..v_k into pairs.  With z = v (plain) or
+    z = P(v) (flexible) and w = A z, the pair is (input (s / t_max) T0 w,
+    target s (z - w)), s = sqrt(n)/||w||: the network learns the correction
+    that maps the residual w back to the step z that produces it.  Pairs from
+    flexible cycles follow the distribution of vectors the preconditioner
+    meets inside flexible GMRES."""
     _, st = plan.gmres(b, "born", tol=tol, max_iter=max(restart, 1), restart=restart,
-                       precision="fp64")
+                       precision="fp64", precond=precond)
     k = min(st["iterations"], restart)
     g = plan.geometry
     shape = (g.N_z, g.M, g.M)
@@ -258,17 +292,82 @@ def harvest_pairs(plan, b: torch.Tensor, *, restart: int = 30, tol: float = 1e-8
         with torch.cuda.device(plan.device):
             _lib.check(plan.lib.msc_gmres_basis(plan._h, _lib.MSC_C128, j, _lib.ptr(v),
                                                 plan.stream))
-        w = plan.matvec(v)
+        z = v if precond is None else precond.apply(plan, v)
+        w = plan.matvec(z)
         nw = float(torch.linalg.vector_norm(w))
         if not nw > 0.0:
             continue
         s = math.sqrt(plan.n) / nw
         ins.append(network_input(plan, w, s))
-        corr = (v - w) * s
+        corr = (z - w) * s
         tgts.append(torch.view_as_real(corr).reshape(*shape, IO_CHANNELS).float())
     if not ins:
         raise RuntimeError("no Krylov vectors harvested")
     return PairSet(torch.stack(ins), torch.stack(tgts), [st])
+
+
+def train_on_slabs(net: "LearnedPreconditioner", plan, draw, omega: float, ref, k_hat,
+                   wave_type: str = "P", *, target_pairs: int = 20000, seeds_per_round: int = 32,
+                   first_seed: int = 1, batch: int = 4, lr: float = 2e-3, lr_min: float = 1e-4,
+                   flexible_rounds: int = 2, restart: int = 30, log=None) -> dict:
+    """Train ``net`` on residual -> correction pairs streamed from solves on
+    seeded slabs of one distribution (``draw(seed)`` returns a SlabMaterial of
+    the plan's geometry; the plan's G0 is that of ``omega``), without holding
+    them all: each round harvests one GMRES(restart) cycle per slab of
+    ``seeds_per_round`` slabs, takes one pass of Adam steps over those pairs
+    (mini-batches of ``batch``, cosine learning-rate decay from ``lr`` to
+    ``lr_min`` over the whole run) and drops them.  Rounds continue until
+    ``target_pairs`` pairs have been used; the last ``flexible_rounds`` rounds
+    harvest from flexible GMRES cycles with the network being trained.  The
+    plan's T0 is left at the last slab drawn."""
+    import time
+
+    if target_pairs < 1 or seeds_per_round < 1 or batch < 1:
+        raise ValueError("target_pairs, seeds_per_round and batch must be >= 1")
+    expect_per_round = seeds_per_round * (restart + 1)
+    rounds = max(1, -(-target_pairs // expect_per_round))
+    total_steps = max(1, -(-target_pairs // batch))
+    step = 0
+    used = 0
+    seed = first_seed
+    losses = []
+    t0 = time.perf_counter()
+    t_harvest = 0.0
+    rng = np.random.default_rng(first_seed)
+    for rd in range(rounds):
+        flexible = rd >= rounds - flexible_rounds and rd > 0
+        th = time.perf_counter()
+        sets = []
+        for _ in range(seeds_per_round):
+            plan.set_material(draw(seed), omega)
+            seed += 1
+            b = plan.incident_field(omega, ref, k_hat, wave_type).reshape(-1)
+            sets.append(harvest_pairs(plan, b, restart=restart, tol=1e-14,
+                                      precond=net if flexible else None))
+        pairs = merge_pairs(sets)
+        del sets
+        torch.cuda.synchronize(plan.device)
+        t_harvest += time.perf_counter() - th
+        n = len(pairs)
+        order = torch.as_tensor(rng.permutation(n), device=pairs.inputs.device)
+        for i0 in range(0, n - batch + 1, batch):
+            frac = min(step / total_steps, 1.0)
+            lr_t = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * frac))
+            idx = order[i0:i0 + batch]
+            losses.append(net.train_step(pairs.inputs[idx], pairs.targets[idx], lr_t))
+            step += 1
+        used += n
+        if log is not None:
+            log(f"  round {rd + 1}/{rounds} ({'flexible' if flexible else 'plain'}): {n} pairs "
+                f"({used} so far), loss {np.mean(losses[-max(1, n // batch):]):.4e}")
+        del pairs
+        if used >= target_pairs:
+            break
+    torch.cuda.synchronize(plan.device)
+    return {"pairs": used, "rounds": rd + 1, "steps": step, "slabs": seed - first_seed,
+            "seconds": time.perf_counter() - t0, "harvest_seconds": t_harvest,
+            "loss_first": float(np.mean(losses[:20])) if losses else None,
+            "loss_last": float(np.mean(losses[-20:])) if losses else None}
 
 
 def merge_pairs(sets: list[PairSet]) -> PairSet:

@@ -302,7 +302,8 @@ def _unet(seed=0, widths=(16, 16, 32, 48), jitter=0.02):
                                          ((16, 16, 32, 48), (4, 24, 72)),
                                          ((16, 16, 32, 48), (1, 2, 3)),
                                          ((16, 32, 32, 48), (6, 10, 12)),
-                                         ((16, 16, 16, 16), (3, 9, 66))])
+                                         ((16, 16, 16, 16), (3, 9, 66)),
+                                         ((16, 16, 32, 48), (24, 96, 96))])
 def test_unet_forward_matches_oracle(hip_device, widths, grid):
     net, p = _unet(seed=sum(grid), widths=widths)
     x = _bf16_input(grid, seed=2)
@@ -313,7 +314,7 @@ def test_unet_forward_matches_oracle(hip_device, widths, grid):
     assert _nrel(out, ref) < 1e-2, _nrel(out, ref)
 
 
-@pytest.mark.parametrize("grid", [(4, 6, 8), (3, 5, 7)])
+@pytest.mark.parametrize("grid", [(4, 6, 8), (3, 5, 7), (12, 40, 40)])
 def test_unet_gradients_match_oracle(hip_device, grid):
     widths = (16, 16, 32, 48)
     net, p = _unet(seed=9)
