@@ -83,7 +83,7 @@ typedef struct msc_gmres_stats {
 /* One independent solve of a sweep (msc_sweep). */
 typedef struct msc_instance {
   int M, N_z;               /* slab of M x M x N_z voxels                      */
-  int periodic;             /* 1: laterally periodic (M a power of two)        */
+  int periodic;             /* 1: laterally periodic (any M)                    */
   int volume_averaged;      /* 1: inter-voxel blocks at the 26 neighbours      */
   int n_orders;             /* their frequency orders, 0..3 (volume_averaged)  */
   double a;                 /* cube half-width                                 */
@@ -160,8 +160,10 @@ const char* msc_version(void);
 /*
  * Plan for an M x M x N_z slab of cubes of half-width a (side d = 2a).
  * pz, pxy: padded FFT lengths (powers of two); 0 picks the smallest power of
- * two >= 2N_z-1 (z) and >= 2M-1 (x, y), or pxy = M when periodic.  Periodic
- * mode (circular lateral convolution) needs M to be a power of two.
+ * two >= 2N_z-1 (z) and >= 2M-1 (x, y), or pxy = M when periodic and M is a
+ * power of two.  Periodic mode (circular lateral convolution, any M) runs on
+ * the M-point lateral grid when pxy == M (M a power of two), otherwise on the
+ * padded grid (pxy >= 2M-1) with the periodically folded kernel.
  * Replaces the geometry handling of _build_slab_kernels_gpu /
  * _slab_matvec_gpu (cubic_scattering/slab_scattering_gpu.py:34-155) and
  * SlabGeometry (cubic_scattering/slab_scattering.py:40-99).

@@ -639,15 +639,18 @@ int msc_plan_create(msc_plan** plan, int M, int N_z, double a, int periodic, int
   MSC_REQUIRE(N_z >= 1, "N_z must be >= 1");
   MSC_REQUIRE(a > 0.0 && std::isfinite(a), "a must be > 0");
   if (pz == 0) pz = std::max(2, next_pow2(2 * N_z - 1));
-  if (pxy == 0) pxy = periodic ? M : std::max(2, next_pow2(2 * M - 1));
+  if (pxy == 0)
+    pxy = periodic && is_pow2(M) && M >= 2 ? M : std::max(2, next_pow2(2 * M - 1));
   MSC_REQUIRE(is_pow2(pz) && pz >= 2 * N_z - 1 && pz >= 2 && pz <= 256,
               "pz must be a power of two in [max(2, 2N_z-1), 256]");
-  if (periodic) {
-    if (!(is_pow2(M) && pxy == M && M >= 2 && M <= 512)) {
-      set_error("periodic mode needs M to be a power of two in [2, 512] and pxy == M");
+  if (periodic && pxy == M) {
+    // circular lateral grid: the folded kernel on the M-point grid
+    if (!(is_pow2(M) && M >= 2 && M <= 512)) {
+      set_error("periodic mode on the M-point grid needs M to be a power of two in [2, 512]");
       return kUnsupported;
     }
   } else {
+    // aperiodic, or periodic on a zero-padded grid with the folded kernel (any M)
     MSC_REQUIRE(is_pow2(pxy) && pxy >= 2 * M - 1 && pxy >= 2 && pxy <= 512,
                 "pxy must be a power of two in [max(2, 2M-1), 512]");
   }
@@ -698,8 +701,8 @@ int msc_plan_reshape(msc_plan* plan, int M, int N_z, double a) {
   MSC_REQUIRE(M <= p.capM && N_z <= p.capNz,
               "the slab exceeds the plan's capacity (create the plan at the largest size)");
   MSC_REQUIRE(2 * N_z - 1 <= p.Pz, "N_z does not fit the plan's vertical FFT length");
-  if (p.periodic) {
-    MSC_REQUIRE(M == p.M, "a periodic plan cannot change M");
+  if (p.periodic && p.Pxy == p.M) {
+    MSC_REQUIRE(M == p.M, "a periodic plan on the M-point grid cannot change M");
   } else {
     MSC_REQUIRE(2 * M - 1 <= p.Pxy, "M does not fit the plan's lateral FFT length");
   }

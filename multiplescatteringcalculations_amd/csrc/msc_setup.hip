@@ -23,9 +23,14 @@ namespace msc {
 namespace {
 
 // Spatial kernel on the circular grid (Pz, Pxy, Pxy): offset d stored at
-// d mod P.  Aperiodic: |dz| < Nz, |dx|,|dy| < M.  Periodic: Pxy = M and the
-// lateral offsets dx in {j, j - M} (|dx| < M) are summed (fold without Bloch
-// phase, slab_scattering.py:337-346).  The self term is zero.  Each thread
+// d mod P.  Aperiodic: |dz| < Nz, |dx|,|dy| < M.  Periodic (fold without Bloch
+// phase, slab_scattering.py:337-346): the lateral kernel becomes the circular
+// one, circ[e] = sum of K[d] over |d| < M, d = e mod M, i.e. the offsets
+// {j, j - M} for j >= 1 summed per axis.  With Pxy = M (M a power of two)
+// circ is stored on the M-point grid; with Pxy >= 2M - 1 (any M) the linear
+// kernel K_lin[d] = circ[d mod M], |d| < M, is stored instead, whose linear
+// convolution over the M x M window equals the circular one.  The self term
+// is zero.  Each thread
 // evaluates one point of the octant 0 <= iz <= Pz/2, 0 <= ix, iy <= Pxy/2 into
 // work[iz][ix][iy][45]: the reflections are not stored (the FFT passes read
 // them from the stored half, k_fft_mirror).  Entry (i, j) of the block changes
@@ -63,10 +68,14 @@ __global__ __launch_bounds__(128) void k_g0_spatial(double2* __restrict__ work, 
     if (dz < Nz) {
       int xs[2], ys[2], nx = 0, ny = 0;
       if (periodic) {
-        xs[nx++] = ix;
-        if (ix >= 1) xs[nx++] = ix - M;
-        ys[ny++] = iy;
-        if (iy >= 1) ys[ny++] = iy - M;
+        if (ix < M) {
+          xs[nx++] = ix;
+          if (ix >= 1) xs[nx++] = ix - M;
+        }
+        if (iy < M) {
+          ys[ny++] = iy;
+          if (iy >= 1) ys[ny++] = iy - M;
+        }
       } else {
         if (ix < M) xs[nx++] = ix;
         if (iy < M) ys[ny++] = iy;

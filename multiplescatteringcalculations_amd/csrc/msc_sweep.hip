@@ -35,6 +35,8 @@ int next_pow2(int v) {
   return p;
 }
 
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
 struct CachedPlan {
   int pz = 0, pxy = 0, periodic = 0;
   msc_plan* plan = nullptr;
@@ -62,7 +64,8 @@ void record(msc_instance& I, int rc, const char* what) {
 int run_one(msc_instance& I, std::vector<CachedPlan>& cache, const msc_gmres_opts& o,
             hipStream_t st) {
   const int pz = std::max(2, next_pow2(2 * I.N_z - 1));
-  const int pxy = I.periodic ? I.M : std::max(2, next_pow2(2 * I.M - 1));
+  const bool circ = I.periodic && is_pow2(I.M) && I.M >= 2;  // periodic on the M-point grid
+  const int pxy = circ ? I.M : std::max(2, next_pow2(2 * I.M - 1));
   CachedPlan* cp = nullptr;
   for (CachedPlan& c : cache)
     if (c.pz == pz && c.pxy == pxy && c.periodic == I.periodic) cp = &c;
@@ -72,7 +75,7 @@ int run_one(msc_instance& I, std::vector<CachedPlan>& cache, const msc_gmres_opt
     c.pxy = pxy;
     c.periodic = I.periodic;
     // the largest slab of this grid, so that any instance of the grid fits
-    const int capM = I.periodic ? I.M : pxy / 2;
+    const int capM = circ ? I.M : pxy / 2;
     const int capNz = pz / 2;
     int rc = msc_plan_create(&c.plan, std::max(capM, 1), std::max(capNz, 1), I.a, I.periodic,
                              pz, pxy);

@@ -120,7 +120,6 @@ def compute_ocean_bottom_reflection(config: OceanBottomConfig, *, volume_average
     has nothing to report (the frequencies run as one sweep).  ``precision``
     ("fp64" or "mixed"), ``restart`` and ``workers`` tune the sweep; every
     slab solve reaches ``gmres_tol`` FP64 relative residual either way.
-    Periodic slabs need M to be a power of two.
     """
     del progress
     from .sweep import SweepCase, compute_slab_sweep

@@ -137,7 +137,9 @@ class SlabPlan:
 
     ``pad`` = (Pz, Pxy) padded FFT lengths (powers of two, default the
     smallest >= 2N-1); ``periodic`` selects the circular lateral convolution of
-    the reference's ``periodic=True`` (M must then be a power of two).
+    the reference's ``periodic=True``: on the M-point lateral grid when M is a
+    power of two, otherwise on the zero-padded grid with the periodically
+    folded kernel (the same operator for any M).
     """
 
     def __init__(self, geometry: SlabGeometry, *, periodic: bool = False,

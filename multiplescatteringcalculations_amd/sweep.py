@@ -75,8 +75,6 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
     outs = []
     for i, c in enumerate(cases):
         g, mat = c.geometry, c.material
-        if c.periodic and (g.M & (g.M - 1)) != 0:
-            raise ValueError("periodic cases need M to be a power of two")
         k_hat, k_mag, inc = _incident_vector(c.omega, mat.ref, c.k_hat, c.wave_type)
         shape = (g.N_z, g.M, g.M)
         arrs = [torch.tensor(np.broadcast_to(np.asarray(a, dtype=np.float64), shape),

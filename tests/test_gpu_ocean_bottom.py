@@ -51,7 +51,8 @@ def test_zero_contrast_traces_and_decomposition(hip_device):
 
 
 @pytest.mark.parametrize("p,free_surface,M,Nz", [(0.0, False, 2, 1), (2e-4, True, 4, 2),
-                                                   (4e-4, False, 2, 2)])
+                                                   (4e-4, False, 2, 2), (2e-4, False, 3, 2),
+                                                   (1e-4, True, 6, 1)])
 def test_matches_oracle_composition(hip_device, p, free_surface, M, Nz):
     cfg = _cfg(M=M, Nz=Nz, p=p, free_surface=free_surface)
     r = _run(cfg, gmres_tol=1e-12)

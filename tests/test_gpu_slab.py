@@ -149,8 +149,12 @@ def test_matvec_matches_oracle(hip_device, M, Nz, contrast, pad):
     assert np.abs((x - y64) - (x - y_ref)).max() < tol
 
 
-@pytest.mark.parametrize("M,Nz", [(2, 1), (2, 2), (4, 2), (8, 3)])
+@pytest.mark.parametrize("M,Nz", [(2, 1), (2, 2), (4, 2), (8, 3), (3, 2), (5, 1), (6, 3),
+                                  (10, 2), (12, 3)])
 def test_matvec_periodic_matches_oracle(hip_device, M, Nz):
+    """Circular lateral convolution: on the M-point grid for M a power of two,
+    on the zero-padded grid with the folded kernel otherwise (the reference's
+    fold accepts any M, slab_scattering.py:337-346)."""
     ref, om = REF_C, 692.8
     Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=7)
     plan, *_ = _plan(M, Nz, 1.0, ref, om, Dl, Dm, Dr, periodic=True)
@@ -161,7 +165,11 @@ def test_matvec_periodic_matches_oracle(hip_device, M, Nz):
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     y_ref = osl.slab_matvec(x, T, kh, M, Nz, periodic=True)
     y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    assert plan.Pxy == (M if M & (M - 1) == 0 else 1 << (2 * M - 2).bit_length())
     assert _rel(x - y, x - y_ref) < 1e-11
+    y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
+    tol = 2e-5 * np.abs(x - y_ref).max() + 1e-7 * np.abs(x).max()
+    assert np.abs((x - y64) - (x - y_ref)).max() < tol
 
 
 def test_matvec_identity_when_T_zero(hip_device):

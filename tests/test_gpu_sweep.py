@@ -24,7 +24,7 @@ def _cases(seed=0):
         p = float(rng.choice([0.0, 1e-4, 2e-4, 3e-4]))
         khat = np.array([np.sqrt(1 - (p * ref.alpha) ** 2), p * ref.alpha, 0.0])
         wave = "S" if i == 2 else "P"
-        periodic = M in (8, 16) and i % 2 == 1
+        periodic = i % 2 == 1  # M = 8 (circular grid) and M = 12 (padded grid, folded kernel)
         cases.append(msc.SweepCase(g, mat, ka * ref.beta / g.a, khat, wave, periodic))
     return cases
 
