@@ -39,8 +39,11 @@ def test_host_entry_points_validate_arguments():
     assert lib.msc_plan_create(ctypes.byref(h), 4, 4, -1.0, 0, 0, 0) == _lib.MSC_ERR_INVALID
     # padding below 2N-1 is rejected before any device work
     assert lib.msc_plan_create(ctypes.byref(h), 8, 4, 1.0, 0, 4, 16) == _lib.MSC_ERR_INVALID
-    assert lib.msc_plan_create(ctypes.byref(h), 6, 4, 1.0, 1, 0, 0) == _lib.MSC_ERR_UNSUPPORTED
+    # periodic on the M-point lateral grid needs M a power of two (any M runs on
+    # the padded grid, the default for such M)
+    assert lib.msc_plan_create(ctypes.byref(h), 6, 4, 1.0, 1, 0, 6) == _lib.MSC_ERR_UNSUPPORTED
     assert b"power of two" in lib.msc_last_error()
+    assert lib.msc_plan_create(ctypes.byref(h), 6, 4, 1.0, 1, 0, 8) == _lib.MSC_ERR_INVALID
     out = (ctypes.c_double * 8)()
     assert lib.msc_cube_coefficients(-1.0, 1.0, 3000.0, 1732.0, 2500.0, out) == _lib.MSC_ERR_INVALID
     with pytest.raises(ValueError, match="positive"):
