@@ -272,6 +272,22 @@ int msc_set_tparams(msc_plan* plan, const void* tparams_c128, void* stream);
 int msc_get_tparams(const msc_plan* plan, void* out_c128, void* stream);
 
 /*
+ * A general 9x9 T-matrix per voxel (for instance the resonance-regime
+ * composite T-matrices of msc_resonance_tmatrices): voxel v applies
+ * table[tidx[v]] (row-major, tau = T psi) in the operator, the source sums
+ * and the learned preconditioner's input.  tidx: device int32 array of the
+ * slab's N_z*M*M voxels with values in [0, count) (checked: out of range is
+ * MSC_ERR_INVALID); table: device count x 81 complex128.  Both are copied.
+ * The reference's slab operator takes the cubic Rayleigh T0 only
+ * (slab_scattering.py:169-205, 359-418, resonance_tmatrix.py:650-850 for the
+ * composite T); this extends it to the resonance regime.  Not combinable with
+ * the volume-averaged correction (MSC_ERR_UNSUPPORTED).  msc_set_tmatrix or
+ * msc_set_tparams switches back to the cubic T0.  Synchronises `stream`.
+ */
+int msc_set_tgeneral(msc_plan* plan, const int* tidx, const void* table_c128, int count,
+                     void* stream);
+
+/*
  * Incident plane wave psi0 = inc9 * exp(i k khat . x_c) (complex128 out).
  * inc18 = 9 complex (re, im) on the host: polarisation and Voigt strain.
  * Replaces _build_slab_incident_field (slab_scattering.py:426-478).

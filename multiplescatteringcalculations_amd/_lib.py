@@ -54,6 +54,7 @@ SIGNATURES = {
                                 POINTER(c_double), c_void_p]),
     "msc_set_tparams": (c_int, [c_void_p, c_void_p, c_void_p]),
     "msc_get_tparams": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "msc_set_tgeneral": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "msc_incident_field": (c_int, [c_void_p, c_double, POINTER(c_double), POINTER(c_double),
                                    c_void_p, c_void_p]),
     "msc_matvec": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),

@@ -240,6 +240,14 @@ void free_plan_buffers(Plan& p) {
   if (p.nn64) (void)hipFree(p.nn64);
   if (p.nn_stage) (void)hipFree(p.nn_stage);
   if (p.zmap) (void)hipFree(p.zmap);
+  if (p.tidx) (void)hipFree(p.tidx);
+  if (p.tgen128) (void)hipFree(p.tgen128);
+  if (p.tgen64) (void)hipFree(p.tgen64);
+  if (p.flag) (void)hipFree(p.flag);
+  p.tidx = nullptr;
+  p.tgen128 = nullptr;
+  p.tgen64 = nullptr;
+  p.flag = nullptr;
   p.zmap = nullptr;
   p.tmax = nullptr;
   p.work = nullptr;
@@ -869,6 +877,10 @@ int msc_build_g0(msc_plan* plan, double omega, double alpha, double beta, double
   MSC_REQUIRE(omega > 0 && alpha > 0 && beta > 0 && rho > 0,
               "omega, alpha, beta and rho must be positive");
   MSC_REQUIRE(!volume_averaged || (n_orders >= 0 && n_orders <= 3), "n_orders must be in 0..3");
+  if (volume_averaged && plan->p.general) {
+    set_error("the volume-averaged correction cannot be combined with a general per-voxel T");
+    return kUnsupported;
+  }
   const bool had_nn = plan->p.have_nn;
   int rc = build_table(plan->p, omega, alpha, beta, rho, S(stream));
   if (!rc) rc = set_nn_correction(plan->p, volume_averaged != 0, n_orders, S(stream));
@@ -883,17 +895,39 @@ int msc_build_g0(msc_plan* plan, double omega, double alpha, double beta, double
 int msc_set_tmatrix(msc_plan* plan, const double* dlambda, const double* dmu, const double* drho,
                     double omega, const double* coef8, void* stream) {
   MSC_REQUIRE(plan != nullptr && dlambda && dmu && drho && coef8, "NULL argument");
+  if (plan->p.general) {  // back to the cubic T0: captured steps hold the other pass 1
+    plan->k128.drop_graphs();
+    plan->k64.drop_graphs();
+  }
   return set_tparams_from_contrasts(plan->p, dlambda, dmu, drho, omega, coef8, S(stream));
 }
 
 int msc_set_tparams(msc_plan* plan, const void* tparams, void* stream) {
   MSC_REQUIRE(plan != nullptr && tparams != nullptr, "NULL argument");
+  if (plan->p.general) {
+    plan->k128.drop_graphs();
+    plan->k64.drop_graphs();
+  }
   return set_tparams_direct(plan->p, reinterpret_cast<const double2*>(tparams), S(stream));
+}
+
+int msc_set_tgeneral(msc_plan* plan, const int* tidx, const void* table, int count,
+                     void* stream) {
+  MSC_REQUIRE(plan != nullptr && tidx != nullptr && table != nullptr, "NULL argument");
+  MSC_REQUIRE(count >= 1, "count must be >= 1");
+  if (plan->p.have_nn) {
+    set_error("a general per-voxel T cannot be combined with the volume-averaged correction");
+    return kUnsupported;
+  }
+  // captured Arnoldi steps hold the pass-1 variant and the table pointers
+  plan->k128.drop_graphs();
+  plan->k64.drop_graphs();
+  return set_tgeneral(plan->p, tidx, reinterpret_cast<const double2*>(table), count, S(stream));
 }
 
 int msc_get_tparams(const msc_plan* plan, void* out, void* stream) {
   MSC_REQUIRE(plan != nullptr && out != nullptr, "NULL argument");
-  if (!plan->p.have_tparams) {
+  if (!plan->p.have_tparams || plan->p.general) {
     set_error("T0 parameters not set");
     return kNotReady;
   }

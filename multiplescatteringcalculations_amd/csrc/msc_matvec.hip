@@ -151,9 +151,13 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
 }
 
 // ── pass 1: T0 gather + zero pad + forward FFT along y ─────────────────────
-template <typename R, int P, int B>
+// GEN: voxel v applies the general 9x9 matrix tgen[tidx[v]] (row-major)
+// instead of the 4-parameter cubic T0.
+template <typename R, int P, int B, bool GEN = false>
 __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
                                               const cx<R>* __restrict__ tp,
+                                              const int* __restrict__ tidx,
+                                              const cx<R>* __restrict__ tgen,
                                               cx<R>* __restrict__ buf1, int M, int nrows,
                                               const cx<R>* __restrict__ tw,
                                               const int* __restrict__ done) {
@@ -181,6 +185,26 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
         s[y * LD + b * 9 + c] = v;
       });
   __syncthreads();
+  if constexpr (GEN) {
+    for (int w = threadIdx.x; w < B * M; w += kNT) {
+      const int b = w / M;
+      const int y = w - b * M;
+      const int row = row0 + b;
+      if (row >= nrows) continue;
+      const cx<R>* T = tgen + (size_t)tidx[(size_t)row * M + y] * 81;
+      cx<R>* v = s + y * LD + b * 9;
+      cx<R> in[9];
+#pragma unroll
+      for (int c = 0; c < 9; ++c) in[c] = v[c];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        cx<R> acc = czero<R>();
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc = cadd<R>(acc, cmul<R>(T[i * 9 + j], in[j]));
+        v[i] = acc;
+      }
+    }
+  } else {
   struct T4 {
     cx<R> u, d, o, s;
   };
@@ -209,6 +233,7 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
 #pragma unroll
     for (int c = 6; c < 9; ++c) v[c] = cmul<R>(ts, v[c]);
       });
+  }
   __syncthreads();
   lds_fft<R, P, false, NTR, LD, kNT>(s, tw);
   for (int w = threadIdx.x; w < B * P * 9; w += kNT) {
@@ -485,12 +510,14 @@ __global__ __launch_bounds__(kNT) void k_nn_apply(const cx<R>* __restrict__ x,
 
 template <typename R> struct PlanBufs;
 template <> struct PlanBufs<double> {
+  static const double2* tgen(const Plan& p) { return p.tgen128; }
   static const double2* table(const Plan& p) { return p.table128; }
   static const double2* nn(const Plan& p) { return p.nn128; }
   static const double2* tp(const Plan& p) { return p.tp128; }
   static const double2* tw(const Plan& p) { return p.tw128; }
 };
 template <> struct PlanBufs<float> {
+  static const float2* tgen(const Plan& p) { return p.tgen64; }
   static const float2* table(const Plan& p) { return p.table64; }
   static const float2* nn(const Plan& p) { return p.nn64; }
   static const float2* tp(const Plan& p) { return p.tp64; }
@@ -519,8 +546,14 @@ template <typename R> constexpr int prec_of() { return sizeof(cx<R>) == 8 ? 1 : 
 template <typename R, int P, int B>
 static void fwd_y_b(Plan& p, const cx<R>* x, cx<R>* b1, hipStream_t st, const int* done) {
   const int nrows = p.Nz * p.M;
-  hipLaunchKernelGGL((k_fwd_y<R, P, B>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, x,
-                     PlanBufs<R>::tp(p), b1, p.M, nrows, PlanBufs<R>::tw(p), done);
+  if (p.general)
+    hipLaunchKernelGGL((k_fwd_y<R, P, B, true>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, x,
+                       PlanBufs<R>::tp(p), p.tidx, PlanBufs<R>::tgen(p), b1, p.M, nrows,
+                       PlanBufs<R>::tw(p), done);
+  else
+    hipLaunchKernelGGL((k_fwd_y<R, P, B>), dim3((nrows + B - 1) / B), dim3(kNT), 0, st, x,
+                       PlanBufs<R>::tp(p), nullptr, nullptr, b1, p.M, nrows, PlanBufs<R>::tw(p),
+                       done);
 }
 template <typename R, int P, int B, int MODE>
 static void inv_y_b(Plan& p, const cx<R>* b1, const cx<R>* x, const cx<R>* bvec, cx<R>* out,
@@ -769,6 +802,10 @@ static int apply(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, int mod
   if (rc) return rc;
   rc = xy_inv<R>(p, x, bvec, out, b1, b2, mode, st, done);
   if (rc || !p.have_nn) return rc;
+  if (p.general) {
+    set_error("the volume-averaged correction needs the cubic T0, not a general per-voxel T");
+    return kUnsupported;
+  }
   const size_t nvox = (size_t)p.Nz * p.M * p.M;
   const int grid = (int)std::min<size_t>((nvox + kNT - 1) / kNT, 4096);
   const cx<R>* tp = PlanBufs<R>::tp(p);

@@ -413,6 +413,26 @@ __device__ __forceinline__ void t0_input(const cx<R>* __restrict__ r,
   }
 }
 
+// The same from a general 9x9 T per voxel (tgen[tidx[v]], row-major).
+template <typename R>
+__device__ __forceinline__ void t0_input_gen(const cx<R>* __restrict__ r,
+                                             const int* __restrict__ tidx,
+                                             const cx<R>* __restrict__ tgen, size_t v, float st,
+                                             uint16_t* pk) {
+  const cx<R>* T = tgen + (size_t)tidx[v] * 81;
+  cx<R> x[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) x[c] = r[v * 9 + c];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    cx<R> acc = czero<R>();
+#pragma unroll
+    for (int j = 0; j < 9; ++j) acc = cadd<R>(acc, cmul<R>(T[i * 9 + j], x[j]));
+    pk[2 * i] = f2bf(st * (float)acc.x);
+    pk[2 * i + 1] = f2bf(st * (float)acc.y);
+  }
+}
+
 // s / t_max of the current application (0 when T0 vanishes everywhere).
 __device__ __forceinline__ float input_scale(const double* scal, float scale,
                                              const unsigned int* tmax) {
@@ -1056,9 +1076,11 @@ __global__ __launch_bounds__(kNetNT) void k_pack(const float* __restrict__ W,
 // 2c+1 = Im of component c; channels kNetIO.. are zero.  t_max = max |t| over
 // the slab brings the induced sources to order one whatever the frequency and
 // contrast units.
-template <typename R>
+template <typename R, bool GEN = false>
 __global__ __launch_bounds__(kNetNT) void k_net_gather(const cx<R>* __restrict__ r,
                                                        const cx<R>* __restrict__ tp,
+                                                       const int* __restrict__ tidx,
+                                                       const cx<R>* __restrict__ tgen,
                                                        const double* __restrict__ scal,
                                                        float scale,
                                                        const unsigned int* __restrict__ tmax,
@@ -1069,7 +1091,10 @@ __global__ __launch_bounds__(kNetNT) void k_net_gather(const cx<R>* __restrict__
   for (size_t v = (size_t)blockIdx.x * kNetNT + threadIdx.x; v < nvox;
        v += (size_t)gridDim.x * kNetNT) {
     uint16_t pk[kNetIOPad];
-    t0_input<R>(r, tp, v, s, pk);
+    if constexpr (GEN)
+      t0_input_gen<R>(r, tidx, tgen, v, s, pk);
+    else
+      t0_input<R>(r, tp, v, s, pk);
 #pragma unroll
     for (int c = kNetIO; c < kNetIOPad; ++c) pk[c] = 0;
     uint4* dst = reinterpret_cast<uint4*>(out + v * kNetIOPad);
@@ -1144,7 +1169,9 @@ void Net::release_grid() {
     grad[t] = nullptr;
   }
   if (outf) (void)hipFree(outf);
+  if (inbuf) (void)hipFree(inbuf);
   outf = nullptr;
+  inbuf = nullptr;
   D = H = W = 0;
   nvox = 0;
 }
@@ -1171,6 +1198,7 @@ int Net::ensure_grid(int D_, int H_, int W_) {
     MSC_HIP_CHECK(hipMalloc(&grad[t], bytes));
   }
   MSC_HIP_CHECK(hipMalloc(&outf, std::max<size_t>(lvox[0], 1) * kNetIOPad * sizeof(float)));
+  MSC_HIP_CHECK(hipMalloc(&inbuf, std::max<size_t>(lvox[0], 1) * kNetIOPad * sizeof(uint16_t)));
   D = D_;
   H = H_;
   W = W_;
@@ -1462,8 +1490,9 @@ void launch_up(int pin, int pout, const DirArgs& a, hipStream_t s) {
 }
 
 // The whole forward pass.  Training (src 0): from the bf16 input `in`, the last
-// layer writes outf (fp32).  Application (src 1/2): the first layer computes
-// its input from r and the T0 parameters tp, the last writes z = r + N / s.
+// layer writes outf (fp32).  Application (src 1/2, complex64/complex128): the
+// first layer computes its input from r and the T0 parameters tp, the last
+// writes z = r + N / s; src 3/4: the same from a gathered bf16 input `in`.
 void forward_pass(Net& net, const uint16_t* in, int src, const void* r, const void* tp,
                   const unsigned int* tmax, void* z, hipStream_t s, const int* done) {
   const int last = net.nlayers - 1;
@@ -1491,8 +1520,9 @@ void forward_pass(Net& net, const uint16_t* in, int src, const void* r, const vo
         a.scal = net.scal;
         a.scale = net.scale_arg;
       }
-      const int epi = src == 0 ? kEpiLast : (src == 1 ? kEpiApply64 : kEpiApply128);
-      conv_forward(net, k, a, l.in == 0 ? src : 0, epi, s);
+      const int epi =
+          src == 0 ? kEpiLast : ((src == 1 || src == 3) ? kEpiApply64 : kEpiApply128);
+      conv_forward(net, k, a, l.in == 0 && src <= 2 ? src : 0, epi, s);
     } else if (l.kind == kDown2) {
       launch_down<kDirHidden>(
           l.pin, l.pout,
@@ -1669,19 +1699,33 @@ static float set_input_scale(const Plan& p, const cx<R>* r, double scale, double
   return sarg;
 }
 
+// (s / t_max) T0 r as bf16 [v][32] with the plan's cubic or general T.
+template <typename R>
+static void gather_input(const Plan& p, const cx<R>* r, const double* scal, float sarg,
+                         uint16_t* out, hipStream_t s, const int* done) {
+  const cx<R>* tp;
+  const cx<R>* tg;
+  if constexpr (sizeof(R) == 8) {
+    tp = p.tp128;
+    tg = p.tgen128;
+  } else {
+    tp = p.tp64;
+    tg = p.tgen64;
+  }
+  if (p.general)
+    hipLaunchKernelGGL((k_net_gather<R, true>), dim3(grid_of(p.nvox)), dim3(kNetNT), 0, s, r, tp,
+                       p.tidx, tg, scal, sarg, p.tmax, out, p.nvox, done);
+  else
+    hipLaunchKernelGGL((k_net_gather<R, false>), dim3(grid_of(p.nvox)), dim3(kNetNT), 0, s, r, tp,
+                       nullptr, nullptr, scal, sarg, p.tmax, out, p.nvox, done);
+}
+
 template <typename R>
 int net_input(const Plan& p, const cx<R>* r, double scale, double* scal, uint16_t* out,
               hipStream_t s, const int* done) {
   MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
   const float sarg = set_input_scale<R>(p, r, scale, scal, s);
-  const cx<R>* tp;
-  if constexpr (sizeof(R) == 8) {
-    tp = p.tp128;
-  } else {
-    tp = p.tp64;
-  }
-  hipLaunchKernelGGL((k_net_gather<R>), dim3(grid_of(p.nvox)), dim3(kNetNT), 0, s, r, tp, scal,
-                     sarg, p.tmax, out, p.nvox, done);
+  gather_input<R>(p, r, scal, sarg, out, s, done);
   MSC_LAUNCH_CHECK();
   return kOk;
 }
@@ -1694,6 +1738,13 @@ int net_apply(Net& net, const Plan& p, const cx<R>* r, cx<R>* z, double scale, h
   if (rc) return rc;
   MSC_REQUIRE(p.have_tparams, "T0 is not set on this plan");
   net.scale_arg = set_input_scale<R>(p, r, scale, net.scal, s);
+  if (p.general) {
+    // a general T has no fused first-layer form: gather the input, then run
+    gather_input<R>(p, r, net.scal, net.scale_arg, net.inbuf, s, done);
+    forward_pass(net, net.inbuf, sizeof(R) == 4 ? 3 : 4, r, nullptr, p.tmax, z, s, done);
+    MSC_LAUNCH_CHECK();
+    return kOk;
+  }
   const void* tp;
   if constexpr (sizeof(R) == 4) {
     tp = p.tp64;

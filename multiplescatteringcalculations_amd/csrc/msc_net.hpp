@@ -84,6 +84,7 @@ struct Net {
   uint16_t* grad[kNetMaxTensors] = {};  // gradient of each tensor (of its pre-activation for
                                         // leaky-ReLU tensors); grad[last] is the loss gradient
   float* outf = nullptr;                // last layer output, fp32 [vox][kNetIOPad]
+  uint16_t* inbuf = nullptr;            // gathered input of an application with a general T
   double* loss = nullptr;               // device loss accumulator (+ partials)
   double* scal = nullptr;  // device (n, sum |r|^2) of the current apply: s = sqrt(n/sum)
   float scale_arg = 0.f;   // host: a given s of the current apply, 0 = read scal

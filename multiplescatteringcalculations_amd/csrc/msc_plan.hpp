@@ -53,8 +53,17 @@ struct Plan {
   float2* table64 = nullptr;
   double2* tp128 = nullptr;
   float2* tp64 = nullptr;
-  // max over voxels of |t_u|, |t_d|, |t_o|, |t_s| (device, float bits; set with T0)
+  // max over voxels of |t_u|, |t_d|, |t_o|, |t_s| (device, float bits; set with T0),
+  // or over the entries of the general T-matrices
   unsigned int* tmax = nullptr;
+  // general per-voxel T (msc_set_tgeneral): voxel v applies the 9x9 matrix
+  // tgen[tidx[v]] (row-major, tau = T psi) instead of the 4-parameter cubic T0
+  bool general = false;
+  int ngen = 0, gen_cap = 0;
+  int* tidx = nullptr;         // capvox
+  double2* tgen128 = nullptr;  // gen_cap x 81
+  float2* tgen64 = nullptr;
+  int* flag = nullptr;         // device scratch flag (index validation)
   void* buf1 = nullptr;  // (Nz, M, Pxy, 9) complex128 bytes
   void* buf2 = nullptr;  // (Nz, Pxy, Pxy, 9) complex128 bytes
   void* work = nullptr;   // G0 table build: (Pz, Pxy, Pxy, 45) complex128, kept for rebuilds

@@ -291,9 +291,11 @@ __device__ __forceinline__ void tapply9(const double2* t, const double2* p, doub
 // sums sum_v exp(-i k r_hat . x_v) tau_v for k in {kP, kS} (2 x 9).  One
 // workgroup per layer; the phased sums are per-layer partials.
 __global__ __launch_bounds__(256) void k_source_sums(const double2* __restrict__ psi,
-                                                     const double2* __restrict__ tp, int M, int Nz,
-                                                     double d, double kP, double kS, double rz,
-                                                     double rx, double ry,
+                                                     const double2* __restrict__ tp,
+                                                     const int* __restrict__ tidx,
+                                                     const double2* __restrict__ tgen, int M,
+                                                     int Nz, double d, double kP, double kS,
+                                                     double rz, double rx, double ry,
                                                      double2* __restrict__ layer_sum,
                                                      double2* __restrict__ phased) {
   __shared__ double2 red[27][4];
@@ -304,7 +306,16 @@ __global__ __launch_bounds__(256) void k_source_sums(const double2* __restrict__
   for (int v = threadIdx.x; v < M * M; v += blockDim.x) {
     const size_t g = base + v;
     double2 tau[9];
-    tapply9(tp + g * 4, psi + g * 9, tau);
+    if (tidx != nullptr) {
+      const double2* T = tgen + (size_t)tidx[g] * 81;
+      for (int i = 0; i < 9; ++i) {
+        double2 acc = zc();
+        for (int j = 0; j < 9; ++j) acc = cad(acc, cmd(T[i * 9 + j], psi[g * 9 + j]));
+        tau[i] = acc;
+      }
+    } else {
+      tapply9(tp + g * 4, psi + g * 9, tau);
+    }
     const int i = v / M, j = v - (v / M) * M;
     const double z = (l + 0.5) * d;
     const double x = (i - (M - 1) / 2.0) * d;
@@ -472,7 +483,63 @@ int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, cons
                      G0, A, B, C, omega, V, p.tp128, p.tp64);
   MSC_LAUNCH_CHECK();
   p.have_tparams = true;
+  p.general = false;
   return update_tmax(p, st);
+}
+
+__global__ __launch_bounds__(256) void k_check_index(const int* __restrict__ idx, size_t n,
+                                                     int count, int* __restrict__ bad) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const int v = idx[i];
+    if (v < 0 || v >= count) atomicOr(bad, 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_narrow_n(const double2* __restrict__ in, size_t n,
+                                                  float2* __restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
+int set_tgeneral(Plan& p, const int* tidx, const double2* table, int count, hipStream_t st) {
+  if (p.flag == nullptr) MSC_HIP_CHECK(hipMalloc(&p.flag, sizeof(int)));
+  if (p.tidx == nullptr) {
+    const size_t capvox = (size_t)p.capNz * p.capM * p.capM;
+    MSC_HIP_CHECK(hipMalloc(&p.tidx, capvox * sizeof(int)));
+  }
+  // indices outside the table would read past it: check them first
+  MSC_HIP_CHECK(hipMemsetAsync(p.flag, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_check_index, dim3(std::min(grid1d(p.nvox, 256), 1024)), dim3(256), 0, st,
+                     tidx, p.nvox, count, p.flag);
+  MSC_LAUNCH_CHECK();
+  int bad = 0;
+  MSC_HIP_CHECK(hipMemcpyAsync(&bad, p.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  MSC_HIP_CHECK(hipStreamSynchronize(st));
+  MSC_REQUIRE(bad == 0, "T index out of range [0, count)");
+  if (count > p.gen_cap) {
+    if (p.tgen128) (void)hipFree(p.tgen128);
+    if (p.tgen64) (void)hipFree(p.tgen64);
+    p.tgen128 = nullptr;
+    p.tgen64 = nullptr;
+    p.gen_cap = 0;
+    MSC_HIP_CHECK(hipMalloc(&p.tgen128, (size_t)count * 81 * sizeof(double2)));
+    MSC_HIP_CHECK(hipMalloc(&p.tgen64, (size_t)count * 81 * sizeof(float2)));
+    p.gen_cap = count;
+  }
+  MSC_HIP_CHECK(hipMemcpyAsync(p.tidx, tidx, p.nvox * sizeof(int), hipMemcpyDeviceToDevice, st));
+  MSC_HIP_CHECK(hipMemcpyAsync(p.tgen128, table, (size_t)count * 81 * sizeof(double2),
+                               hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_narrow_n, dim3(std::min(grid1d((size_t)count * 81, 256), 1024)), dim3(256),
+                     0, st, p.tgen128, (size_t)count * 81, p.tgen64);
+  MSC_LAUNCH_CHECK();
+  p.ngen = count;
+  p.general = true;
+  p.have_tparams = true;
+  hipLaunchKernelGGL(k_tmax_clear, dim3(1), dim3(64), 0, st, p.tmax);
+  hipLaunchKernelGGL(k_tmax, dim3(std::min(grid1d((size_t)count * 81, 256), 1024)), dim3(256), 0,
+                     st, p.tgen128, (size_t)count * 81, p.tmax);
+  MSC_LAUNCH_CHECK();
+  return kOk;
 }
 
 int tparams_batch(const double* Dl, const double* Dm, const double* Dr, size_t count,
@@ -494,6 +561,7 @@ int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st) {
                      p.nvox * 4, p.tp64);
   MSC_LAUNCH_CHECK();
   p.have_tparams = true;
+  p.general = false;
   return update_tmax(p, st);
 }
 
@@ -509,8 +577,9 @@ int incident_field(Plan& p, double k_mag, const double* khat3, const double* inc
 
 int source_sums(Plan& p, const double2* psi, double kP, double kS, const double* rhat3,
                 double2* layer_sum, double2* phased, hipStream_t st) {
-  hipLaunchKernelGGL(k_source_sums, dim3(p.Nz), dim3(256), 0, st, psi, p.tp128, p.M, p.Nz,
-                     2.0 * p.a, kP, kS, rhat3[0], rhat3[1], rhat3[2], layer_sum, phased);
+  hipLaunchKernelGGL(k_source_sums, dim3(p.Nz), dim3(256), 0, st, psi, p.tp128,
+                     p.general ? p.tidx : nullptr, p.tgen128, p.M, p.Nz, 2.0 * p.a, kP, kS,
+                     rhat3[0], rhat3[1], rhat3[2], layer_sum, phased);
   MSC_LAUNCH_CHECK();
   return kOk;
 }

@@ -8,6 +8,9 @@ namespace msc {
 int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, const double* Dr,
                                double omega, const double* coef8, hipStream_t st);
 int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st);
+// General per-voxel T: tidx (device, nvox indices in [0, count)), table (device,
+// count x 81 complex128, row-major).  Validates the indices.
+int set_tgeneral(Plan& p, const int* tidx, const double2* table, int count, hipStream_t st);
 // (t_u, t_d, t_o, t_s) of `count` cubes of volume V from contrast arrays
 // (device), with the frequency constants of msc_cube_coefficients.
 int tparams_batch(const double* Dl, const double* Dm, const double* Dr, size_t count,

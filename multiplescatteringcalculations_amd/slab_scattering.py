@@ -219,6 +219,22 @@ class SlabPlan:
             _lib.check(self.lib.msc_set_tparams(self._h, _lib.ptr(t), self.stream))
             torch.cuda.current_stream(self.device).synchronize()
 
+    def set_tgeneral(self, tidx, table) -> None:
+        """A general 9x9 T-matrix per voxel: voxel v applies table[tidx[v]]
+        (tidx: (N_z, M, M) integers, table: (K, 9, 9) complex, tau = T psi)."""
+        g = self.geometry
+        idx = torch.as_tensor(np.asarray(tidx) if not isinstance(tidx, torch.Tensor) else tidx)
+        idx = idx.to(device=self.device, dtype=torch.int32).reshape(-1).contiguous()
+        if idx.numel() != g.n_cubes:
+            raise ValueError(f"tidx must have {g.n_cubes} entries, got {idx.numel()}")
+        tab = torch.as_tensor(table).to(device=self.device, dtype=torch.complex128).contiguous()
+        if tab.dim() != 3 or tab.shape[1:] != (9, 9):
+            raise ValueError(f"table must be (K, 9, 9), got {tuple(tab.shape)}")
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.msc_set_tgeneral(self._h, _lib.ptr(idx), _lib.ptr(tab),
+                                                 int(tab.shape[0]), self.stream))
+        self._material_token = None
+
     def tparams(self) -> torch.Tensor:
         g = self.geometry
         out = torch.empty((g.N_z, g.M, g.M, 4), dtype=torch.complex128, device=self.device)

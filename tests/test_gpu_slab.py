@@ -467,3 +467,93 @@ def test_volume_averaged_toggle_on_one_plan(hip_device):
     for va, x in zip((False, True, False, True), seq):
         assert _rel(x, solve(ss.SlabPlan(g, device=hip_device), va)) < 1e-10
     assert _rel(seq[1], seq[0]) > 1e-6  # the correction changes the field
+
+
+def test_general_tmatrix_operator_matches_oracle(hip_device):
+    """A general 9x9 T per voxel (msc_set_tgeneral): the operator, the fused
+    residual, the complex64 operator and the layer / far-field source sums
+    against the oracle's dense-T formulas, with random full 9x9 matrices
+    shared through an index table (some voxels zero)."""
+    M, Nz = 7, 3
+    om = 692.8
+    plan, *_ = _plan(M, Nz, 1.0, REF_C, om, *_iid(M, Nz, REF_C, 0.5, 0.3, seed=2))
+    rng = np.random.default_rng(31)
+    K = 5
+    scale = np.abs(osl.tparams_from_T(osl.slab_tmatrices(M, Nz, 1.0, *_iid(M, Nz, REF_C, 1.0, 0.3,
+                                                                           seed=3), om,
+                                                         REF_C))).max()
+    table = scale * (rng.standard_normal((K, 9, 9)) + 1j * rng.standard_normal((K, 9, 9)))
+    table[0] = 0.0
+    tidx = rng.integers(0, K, (Nz, M, M))
+    plan.set_tgeneral(tidx, table)
+    T = table[tidx]
+    kh = osl.slab_kernels(M, Nz, 1.0, om, REF_C)
+    n = Nz * M * M * 9
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y_ref = osl.slab_matvec(x, T, kh, M, Nz)
+    y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    assert _rel(x - y, x - y_ref) < 1e-11
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    r = plan.residual(torch.as_tensor(b).to(hip_device), torch.as_tensor(x).to(hip_device))
+    assert _rel(r.cpu().numpy(), b - y_ref) < 1e-11
+    y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
+    tol = 2e-5 * np.abs(x - y_ref).max() + 1e-7 * np.abs(x).max()
+    assert np.abs((x - y64) - (x - y_ref)).max() < tol
+    # source sums: tau = T psi per layer, and the phased sums of the far field
+    kP, kS, rh = om / REF_C.alpha, om / REF_C.beta, np.array([-0.8, -0.36, -0.48])
+    ls, ph = plan.source_sums(torch.as_tensor(x).to(hip_device), kP, kS, rh)
+    tau = np.einsum("lmnab,lmnb->lmna", T, x.reshape(Nz, M, M, 9))
+    assert _rel(ls, tau.sum(axis=(1, 2))) < 1e-12
+    c = osl.centres(M, Nz, 1.0)
+    eP = np.exp(-1j * kP * np.einsum("k,lmnk->lmn", rh, c))
+    assert _rel(ph[:, 0], np.einsum("lmn,lmna->la", eP, tau)) < 1e-12
+    # an index outside the table is rejected before any device work reads it
+    bad = tidx.copy()
+    bad[1, 2, 3] = K
+    with pytest.raises(ValueError, match="out of range"):
+        plan.set_tgeneral(bad, table)
+    # back to the cubic T0 of the material
+    _, g, r_, mat = _plan(M, Nz, 1.0, REF_C, om, *_iid(M, Nz, REF_C, 0.5, 0.3, seed=2))
+    plan.set_material(mat, om)
+    T0 = osl.slab_tmatrices(M, Nz, 1.0, mat.Dlambda, mat.Dmu, mat.Drho, om, REF_C)
+    y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    assert _rel(x - y, x - osl.slab_matvec(x, T0, kh, M, Nz)) < 1e-11
+
+
+def test_general_tmatrix_solve_and_network_input(hip_device):
+    """GMRES with a general T converges to the dense solution; the learned
+    preconditioner's input (s / t_max) T r uses the general T, with t_max the
+    largest entry modulus of the table."""
+    from multiplescatteringcalculations_amd import learned_precond as lp
+
+    M, Nz = 4, 2
+    om = 692.8
+    plan, *_ = _plan(M, Nz, 1.0, REF_C, om, *_iid(M, Nz, REF_C, 0.5, 0.3, seed=5))
+    rng = np.random.default_rng(8)
+    K = 3
+    T0 = osl.slab_tmatrices(M, Nz, 1.0, *_iid(M, Nz, REF_C, 1.0, 0.4, seed=6), om, REF_C)
+    table = np.stack([T0[0, 0, 0] * (1 + 0.3 * rng.standard_normal((9, 9))) for _ in range(K)])
+    tidx = rng.integers(0, K, (Nz, M, M))
+    plan.set_tgeneral(tidx, table)
+    T = table[tidx]
+    A = osl.dense_operator(T, M, Nz, 1.0, om, REF_C)
+    psi0 = osl.incident_field(M, Nz, 1.0, om, REF_C, [0.893, 0.45, 0.0], "P").ravel()
+    x, st = plan.gmres(torch.as_tensor(psi0), "born", tol=1e-13, max_iter=300, restart=30)
+    assert st["converged"]
+    assert _rel(x.cpu().numpy(), np.linalg.solve(A, psi0)) < 1e-10
+    xm, stm = plan.gmres(torch.as_tensor(psi0), "born", tol=1e-8, max_iter=300, restart=30,
+                         precision="mixed")
+    assert stm["converged"] and _rel(xm.cpu().numpy(), x.cpu().numpy()) < 1e-6
+    r = torch.as_tensor(rng.standard_normal(plan.n) + 1j * rng.standard_normal(plan.n))
+    inp = lp.network_input(plan, r.to(hip_device)).float().cpu().numpy()
+    s = np.sqrt(plan.n) / float(torch.linalg.vector_norm(r))
+    tau = np.einsum("lmnab,lmnb->lmna", T, r.numpy().reshape(Nz, M, M, 9)) * s / np.abs(table).max()
+    ref = np.stack([tau.real, tau.imag], axis=-1).reshape(Nz, M, M, 18)
+    assert np.abs(inp[..., :18] - ref).max() <= 1e-2 * np.abs(ref).max()
+    assert np.all(inp[..., 18:] == 0)
+    # the learned preconditioner runs on the general T (flexible GMRES reaches 1e-10)
+    net = lp.LearnedPreconditioner(seed=0, arch="unet")
+    xf, stf = plan.gmres(torch.as_tensor(psi0), "born", tol=1e-10, max_iter=300, restart=30,
+                         precision="mixed", precond=net)
+    assert stf["converged"] and stf["precond_applies"] > 0
+    assert _rel(xf.cpu().numpy(), x.cpu().numpy()) < 1e-8
