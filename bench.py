@@ -317,6 +317,64 @@ def time_strong_contrast(pairs: int | None = None):
     return out
 
 
+RESONANCE_SLAB = {"M": 64, "N_z": 16, "n_sub": 4, "phi": 0.3, "contrast": 0.3,
+                  "ka": (1.0, 2.0), "frequencies": 16, "max_iter": 1000}
+
+
+def time_resonance_slab():
+    """The resonance-regime ocean bottom: a 64x64x16 slab (Bernoulli(0.3)
+    inclusions, contrasts U[-0.3, 0.3]) whose every cube carries its full-wave
+    composite T (4^3 sub-cells), between water (1500 m/s, 1000 kg/m^3) and an
+    elastic half-space (4000 / 2300 m/s, 2600 kg/m^3), normal incidence, 16
+    frequencies ka 1-2.  Per frequency: the composite T-matrices (one batched
+    device solve), the mixed GMRES(30) slab solve (stopped at 1000
+    iterations), the same medium resolved into 256x256x64 sub-cells (Rayleigh
+    T0 at ka/4) as the cross-check, the embedding, and the Kennett R_PP of the
+    CPA medium of the contrast distribution (27-point Gauss-Legendre phases)."""
+    import torch
+
+    import multiplescatteringcalculations_amd as msc
+    from multiplescatteringcalculations_amd import resonance_slab as rs
+
+    cfg = RESONANCE_SLAB
+    ref = msc.ReferenceMedium(VP, VS, RHO)
+    g = msc.SlabGeometry(M=cfg["M"], N_z=cfg["N_z"], a=A)
+    mat = msc.random_iid_slab_material(g, ref, cfg["phi"], cfg["contrast"], seed=0)
+    oms = np.linspace(*cfg["ka"], cfg["frequencies"]) * VS / A
+    phases = rs.uniform_contrast_phases(ref, cfg["phi"], cfg["contrast"])
+    # warm-up: kernels of both grids and the resonance solver
+    rs.resonance_ocean_bottom(g, mat, oms[:1], n_sub=cfg["n_sub"], max_iter=30, subcell=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = rs.resonance_ocean_bottom(g, mat, oms, n_sub=cfg["n_sub"], cpa_phases=phases,
+                                  gmres_tol=TOL, precision="mixed", restart=RESTART,
+                                  max_iter=cfg["max_iter"], subcell=True)
+    total = time.perf_counter() - t0
+    both = [i for i in range(len(oms)) if r.converged[i] and r.subcell_converged[i]]
+    rel = [abs(r.R_slab[i] - r.R_slab_subcell[i]) / abs(r.R_slab_subcell[i]) for i in both]
+    out = {"slab": "64x64x16, n_sub 4, phi 0.3, contrasts +-0.3, ka 1-2 (16), p 0, "
+                   "water 1500/1000 over 4000/2300/2600",
+           "total_s": round(total, 3),
+           "tmatrix_s": round(float(np.sum(r.tmatrix_seconds)), 3),
+           "distinct_tmatrices_per_frequency": int(np.mean(r.distinct_tmatrices)),
+           "composite_solve_s": round(float(np.sum(r.solve_seconds) - np.sum(r.tmatrix_seconds)), 3),
+           "subcell_solve_s": round(float(np.sum(r.subcell_seconds)), 3),
+           "ka": [round(float(k), 4) for k in r.ka],
+           "composite_iterations": r.iterations, "composite_converged": r.converged,
+           "subcell_iterations": r.subcell_iterations, "subcell_converged": r.subcell_converged,
+           "R_total_composite": [[round(v.real, 6), round(v.imag, 6)] for v in r.R_total],
+           "R_total_subcell": [[round(v.real, 6), round(v.imag, 6)] for v in r.R_total_subcell],
+           "R_cpa_kennett": [[round(v.real, 6), round(v.imag, 6)] for v in r.R_cpa],
+           "R_bg_kennett": [[round(v.real, 6), round(v.imag, 6)] for v in r.R_bg],
+           "R_slab_composite_vs_subcell_rel": [round(float(v), 4) for v in rel],
+           "max_abs_R_total_subcell_minus_cpa": round(float(np.max(np.abs(r.R_total_subcell
+                                                                           - r.R_cpa))), 5),
+           "max_abs_R_total_subcell_minus_bg": round(float(np.max(np.abs(r.R_total_subcell
+                                                                          - r.R_bg))), 5)}
+    log(f"resonance slab: {out}")
+    return out
+
+
 def time_sphere(n_sub: int = 32):
     """Voxelised-sphere Foldy-Lax T-matrix (9 right-hand sides, FP64 GMRES to 1e-8)
     with n_sub cells per edge, ka_sphere = 2."""
@@ -590,6 +648,7 @@ def main():
     sweep = None if args.no_sweep else time_sweep()
     extras = None if args.no_extras else {"resonance_batch": time_resonance(),
                                           "sphere_fft": time_sphere(),
+                                          "resonance_slab_ocean_bottom": time_resonance_slab(),
                                           "learned_strong_contrast": time_strong_contrast()}
     cpu = None
     if not args.no_cpu_baseline:

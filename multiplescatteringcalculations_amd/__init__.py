@@ -70,6 +70,15 @@ from .resonance_tmatrix import (
     validate_rayleigh_limit,
     voigt_tmatrix_from_resonance_result,
 )
+from .resonance_slab import (
+    ResonanceOceanBottomResult,
+    compute_resonance_slab_scattering,
+    compute_subcell_slab_scattering,
+    resonance_ocean_bottom,
+    resonance_slab_tmatrices,
+    subcell_slab,
+    uniform_contrast_phases,
+)
 from .slab_scattering_gpu import SlabOperator, compute_slab_scattering_gpu
 from .sphere_scattering_fft import (
     SphereDecompositionResult,
@@ -97,6 +106,9 @@ __all__ = [
     "ResonanceTmatrixResult", "compute_resonance_tmatrices", "compute_resonance_tmatrix",
     "suggest_n_subcells", "validate_rayleigh_limit", "scattering_order_decomposition",
     "voigt_tmatrix_from_resonance_result", "sub_cell_centres", "strain_from_displacement_traction",
+    "ResonanceOceanBottomResult", "compute_resonance_slab_scattering", "resonance_ocean_bottom",
+    "resonance_slab_tmatrices", "uniform_contrast_phases", "subcell_slab",
+    "compute_subcell_slab_scattering",
     "CPAResult", "CubicEffectiveMedium", "Phase", "compute_cpa", "compute_cpa_two_phase",
     "phases_from_two_phase", "voigt_average", "SphereDecompositionResult", "compute_sphere_foldy_lax",
     "compute_sphere_foldy_lax_fft", "compute_sphere_foldy_lax_fft_gpu",
