@@ -230,7 +230,9 @@ int msc_plan_autotune(msc_plan* plan, int precision, void* stream);
 int msc_plan_tiles(const msc_plan* plan, int* out10);
 /* Set the tiles explicitly (same layout as msc_plan_tiles; marks both
  * precisions tuned).  A tile larger than the LDS allows for the plan's FFT
- * lengths falls back to the next smaller one. */
+ * lengths falls back to the next smaller one.  A z tile of Q + 256 (Q = 2, 4,
+ * 8) selects the persistent z pass: resident workgroups walk the tiles and
+ * prefetch the next tile's rows into LDS with direct-to-LDS loads. */
 int msc_plan_set_tiles(msc_plan* plan, const int* tiles10);
 
 /*

@@ -271,6 +271,14 @@ void free_plan_buffers(Plan& p) {
 // two rows adjacent, so every table entry is fetched by at most the two XCDs
 // of its folded (kx, ky) pair.  For Pxy >= 16 the XCDs get equal shares.
 int msc::build_zmap(Plan& p) {
+  // A column (kx, ky) reads the table slice of its folded pair
+  // (max(a, b), min(a, b)), a = min(kx, P - kx), b = min(ky, P - ky): eight
+  // columns (+-kx, +-ky and the swap) share a slice.  A tile is Q consecutive
+  // ky at one kx (chunk c = ky / Q).  Key the tile by A = a / Q and by the
+  // chunk's folded index C = min(c, nck - 1 - c): the swapped columns of tile
+  // (A, C) lie in tiles (C, A), so every tile of the unordered pair {A, C}
+  // goes to XCD (A + C) mod 8, back to back (+-kx, mirrored chunks and the
+  // swapped block together), and a slice is fetched into one L2 once.
   constexpr int kXcd = 8;
   const int P = p.Pxy;
   std::vector<int> all;
@@ -278,11 +286,21 @@ int msc::build_zmap(Plan& p) {
     const int nck = P / Q;
     std::vector<std::vector<int>> per(kXcd);
     if (nck >= 1 && P % Q == 0) {
-      for (int g = 0; g <= P / 2; ++g) {
-        std::vector<int>& L = per[g % kXcd];
-        for (int c = 0; c < nck; ++c) {
-          L.push_back(g * nck + c);
-          if (g != 0 && 2 * g != P) L.push_back((P - g) * nck + c);
+      const int na = (P / 2) / Q + 1;  // groups of folded kx
+      const int nc = (nck + 1) / 2;    // folded chunk indices
+      std::vector<std::vector<int>> kx_of(na), ch_of(nc);
+      for (int kx = 0; kx < P; ++kx) kx_of[std::min(kx, P - kx) / Q].push_back(kx);
+      for (int c = 0; c < nck; ++c) ch_of[std::min(c, nck - 1 - c)].push_back(c);
+      const int nkey = std::max(na, nc);
+      for (int hi = 0; hi < nkey; ++hi) {
+        for (int lo = 0; lo <= hi; ++lo) {
+          std::vector<int>& L = per[(hi + lo) % kXcd];
+          for (int o = 0; o < (lo == hi ? 1 : 2); ++o) {
+            const int A = o == 0 ? hi : lo, C = o == 0 ? lo : hi;
+            if (A >= na || C >= nc) continue;
+            for (int kx : kx_of[A])
+              for (int c : ch_of[C]) L.push_back(kx * nck + c);
+          }
         }
       }
     }
@@ -837,9 +855,11 @@ int msc_plan_set_tiles(msc_plan* plan, const int* tiles10) {
     const int v = tiles10[i];
     const int pass = i % 5;
     const bool ok = (pass == 0 || pass == 4) ? (v == 1 || v == 2)
-                    : (pass == 2) ? (v == 1 || v == 2 || v == 4 || v == 8 || v == 16)
+                    : (pass == 2) ? (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 ||
+                                     v == (2 | kZDma) || v == (4 | kZDma) || v == (8 | kZDma))
                                   : (v == 1 || v == 2 || v == 4);
-    MSC_REQUIRE(ok, "tile out of range (y passes 1-2, x passes 1/2/4, z pass 1/2/4/8/16)");
+    MSC_REQUIRE(ok, "tile out of range (y passes 1-2, x passes 1/2/4, z pass 1/2/4/8/16, "
+                    "or 2/4/8 + 256 for the persistent z pass)");
   }
   for (int pr = 0; pr < 2; ++pr) {
     for (int i = 0; i < 5; ++i) plan->p.tile[pr][i] = tiles10[pr * 5 + i];

@@ -150,4 +150,96 @@ __device__ __forceinline__ void lds_fft(cx<R>* s, const cx<R>* __restrict__ tw) 
   fft_stages<R, N, 1, INV, NTR, LD, NT>(s, tw);
 }
 
+// Sub-transform length NS entering the last Stockham pass of length N (the
+// radix schedule of fft_stages).
+constexpr int last_ns(int N) {
+  int ns = 1;
+  while (true) {
+    const int rem = N / ns;
+    const int rad = rem % 8 == 0 ? 8 : (rem % 4 == 0 ? 4 : 2);
+    if (ns * rad == N) return ns;
+    ns *= rad;
+  }
+}
+
+// Every pass of fft_stages except the last one (ends with a barrier when it
+// runs any pass).
+template <typename R, int N, int NS, bool INV, int NTR, int LD, int NT>
+__device__ __forceinline__ void fft_stages_but_last(cx<R>* s, const cx<R>* __restrict__ tw) {
+  if constexpr (NS < last_ns(N)) {
+    constexpr int RAD = pick_radix<N / NS>::value;
+    stockham_pass<R, N, RAD, NS, INV, NTR, LD, NT>(s, tw);
+    fft_stages_but_last<R, N, NS * RAD, INV, NTR, LD, NT>(s, tw);
+  }
+}
+
+// First forward pass (NS = 1) of a transform whose input rows >= nrows are
+// zero, nrows <= N / 2, read from src[row * NTR + col] (unpadded) and written
+// to dst[row * LDD + col]: only the rows < nrows are read (rows r * N / RAD
+// with r >= RAD / 2 are >= N / 2).  No barrier between read and write
+// (src and dst are distinct); none at the end either.
+template <typename R, int N, int NTR, int LDD, int NT>
+__device__ __forceinline__ void first_pass_pruned(const cx<R>* src, cx<R>* dst, int nrows) {
+  constexpr int RAD = pick_radix<N>::value;
+  constexpr int NB = N / RAD;
+  constexpr int TOT = NB * NTR;
+  constexpr int ITEMS = (TOT + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = (int)threadIdx.x + it * NT;
+    if (ITEMS * NT == TOT || w < TOT) {
+      const int col = w % NTR;
+      const int j = w / NTR;
+      cx<R> v[RAD];
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        const int row = j + r * NB;
+        v[r] = (r < RAD / 2 && row < nrows) ? src[row * NTR + col] : czero<R>();
+      }
+      dft<R, RAD, false>(v);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) dst[(j * RAD + r) * LDD + col] = v[r];
+    }
+  }
+}
+
+// Last pass of an inverse transform held in s (after fft_stages_but_last),
+// handing only the output rows < nrows to store(row, col, value) instead of
+// writing them back to LDS.
+template <typename R, int N, int NTR, int LD, int NT, typename Store>
+__device__ __forceinline__ void last_pass_inv_store(const cx<R>* s, const cx<R>* __restrict__ tw,
+                                                    int nrows, Store store) {
+  constexpr int NS = last_ns(N);
+  constexpr int RAD = N / NS;
+  constexpr int NB = NS;
+  constexpr int TOT = NB * NTR;
+  constexpr int ITEMS = (TOT + NT - 1) / NT;
+  constexpr int TWS = kTwMax / N;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = (int)threadIdx.x + it * NT;
+    if (ITEMS * NT == TOT || w < TOT) {
+      const int col = w % NTR;
+      const int j = w / NTR;
+      cx<R> v[RAD];
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) v[r] = s[(j + r * NB) * LD + col];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) {
+          cx<R> t = tw[r * j * TWS];
+          t.y = -t.y;
+          v[r] = cmul<R>(v[r], t);
+        }
+      }
+      dft<R, RAD, true>(v);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        const int row = j + r * NS;
+        if (row < nrows) store(row, col, v[r]);
+      }
+    }
+  }
+}
+
 }  // namespace msc

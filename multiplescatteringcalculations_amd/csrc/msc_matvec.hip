@@ -345,6 +345,113 @@ __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
   }
 }
 
+// Direct-to-LDS copy of the Nz input rows of one z-pass tile,
+// dst[z * NTR + e] = src[z * plane + e] (z < Nz, e < NTR), in 16-byte chunks
+// (global_load_lds_dwordx4): wave w issues chunks w*64 .. w*64+63, then every
+// 4th group of 64; lane l of a group lands at the group's LDS base + 16 l.
+// Nothing passes through VGPRs; the data is in LDS once the issuing wave's
+// vmcnt drains.  Needs NTR * sizeof(cx<R>) a multiple of 16 and 16-byte
+// aligned sources.
+template <typename R, int NTR>
+__device__ __forceinline__ void dma_rows(cx<R>* dst, const cx<R>* src, size_t plane, int Nz) {
+  constexpr int EPC = 16 / (int)sizeof(cx<R>);  // complex per 16-byte chunk
+  static_assert(NTR % EPC == 0, "a chunk must not straddle two rows");
+  typedef __attribute__((address_space(3))) void* lds_ptr;
+  typedef __attribute__((address_space(1))) void* gbl_ptr;
+  const int nch = Nz * (NTR / EPC);
+  const int lane = (int)threadIdx.x & 63;
+  for (int c0 = ((int)threadIdx.x >> 6) * 64; c0 < nch; c0 += kNT) {
+    const int c = c0 + lane;
+    if (c < nch) {
+      const int i = c * EPC;
+      const int z = i / NTR;
+      const int e = i - z * NTR;
+      __builtin_amdgcn_global_load_lds((gbl_ptr)(src + (size_t)z * plane + e),
+                                       (lds_ptr)(dst + (size_t)c0 * EPC), 16, 0, 0);
+    }
+  }
+}
+
+// Pass 3, persistent form: each workgroup walks the tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ... (gridDim.x a multiple of 8, so every tile keeps
+// the XCD its zmap slot names).  The next tile's Nz input rows stream into
+// a separate LDS buffer by direct-to-LDS loads while the current tile runs
+// its FFTs and block multiply.  The z FFTs are pruned: the forward's first
+// pass reads only the Nz non-zero rows (out of place, from the prefetch
+// buffer), and the inverse's last pass writes back only the rows z < Nz.
+template <typename R, int PZ, int Q>
+__global__ __launch_bounds__(kNT) void k_z_mult_dma(cx<R>* __restrict__ buf2,
+                                                   const cx<R>* __restrict__ table, int Nz,
+                                                   int Pxy, const cx<R>* __restrict__ tw,
+                                                   const int* __restrict__ zmap, int ntiles,
+                                                   const int* __restrict__ done) {
+  constexpr int NTR = 9 * Q;
+  constexpr int LD = lds_ld<NTR>::value;
+  constexpr int NC = PZ / 2 + 1;
+  constexpr int HZ = PZ / 2;
+  __shared__ cx<R> s[PZ * LD];
+  __shared__ cx<R> in[HZ * NTR];
+  if (done != nullptr && *done) return;
+  int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int nck = Pxy / Q;
+  const size_t plane = (size_t)Pxy * Pxy * 9;
+  const int half = Pxy / 2;
+  auto col_of = [&](int tile) {
+    const int kx = tile / nck;
+    return ((size_t)kx * Pxy + (tile - kx * nck) * Q) * 9;
+  };
+  dma_rows<R, NTR>(in, buf2 + col_of(zmap[t]), plane, Nz);
+  for (; t < ntiles; t += gridDim.x) {
+    const int tile = zmap[t];
+    const int kx = tile / nck;
+    const int ky0 = (tile - kx * nck) * Q;
+    const size_t col0 = ((size_t)kx * Pxy + ky0) * 9;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();  // this tile's rows are in `in`; the last tile's reads of s are done
+    first_pass_pruned<R, PZ, NTR, LD, kNT>(in, s, Nz);
+    __syncthreads();  // `in` is free
+    if (t + (int)gridDim.x < ntiles)
+      dma_rows<R, NTR>(in, buf2 + col_of(zmap[t + gridDim.x]), plane, Nz);
+    fft_stages<R, PZ, pick_radix<PZ>::value, false, NTR, LD, kNT>(s, tw);
+    for (int w = threadIdx.x; w < PZ * Q; w += kNT) {
+      const int q = w / PZ;
+      const int kz = w - q * PZ;
+      const int ky = ky0 + q;
+      const int a = kx <= half ? kx : Pxy - kx;
+      const int b = ky <= half ? ky : Pxy - ky;
+      const int c = kz <= HZ ? kz : PZ - kz;
+      const unsigned flip = (kz > HZ ? kFlipZ : 0u) ^ (kx > half ? kFlipX : 0u) ^
+                            (ky > half ? kFlipY : 0u);
+      const bool sw = a < b;
+      const int u = sw ? b : a;
+      const int vv = sw ? a : b;
+      const cx<R>* kp = table + (size_t)((u * (u + 1)) / 2 + vv) * kNE * NC + c;
+      cx<R>* sp = s + kz * LD + q * 9;
+      cx<R> tv[1][9], o[1][9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) tv[0][i] = flip_if<R>(sp[i], flip, i);
+      if (sw) swap_xy<R>(tv[0]);
+      kmul45_stream<R, 1>(kp, NC, tv, o);
+      if (sw) swap_xy<R>(o[0]);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) sp[i] = flip_if<R>(o[0][i], flip, i);
+    }
+    __syncthreads();
+    fft_stages_but_last<R, PZ, 1, true, NTR, LD, kNT>(s, tw);
+    // the last pass reads and writes the same rows of its own column, so it
+    // runs in place without a barrier between its reads and writes
+    last_pass_inv_store<R, PZ, NTR, LD, kNT>(
+        s, tw, Nz, [&](int z, int e, cx<R> v) { s[z * LD + e] = v; });
+    __syncthreads();
+    for (int w = threadIdx.x; w < Nz * NTR; w += kNT) {
+      const int z = w / NTR;
+      const int e = w - z * NTR;
+      buf2[(size_t)z * plane + col0 + e] = s[z * LD + e];
+    }
+  }
+}
+
 // ── pass 4: inverse FFT along x, keep x < M ────────────────────────────────
 template <typename R, int P, int Q>
 __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
@@ -678,13 +785,66 @@ static void z_q(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
                      p.zmap + zmap_offset(p.Pxy, Q), done);
 }
 
+// Persistent direct-to-LDS form: the z tile code is Q | kZDma.  The grid is
+// the number of workgroups resident at once (occupancy from the runtime,
+// times the CU count), a multiple of 8, and at most one per tile.
+template <typename R, int PZ, int Q>
+static void z_q_dma(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
+  static const int per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_z_mult_dma<R, PZ, Q>, kNT, 0) !=
+            hipSuccess || n < 1)
+      n = 1;
+    return n;
+  }();
+  static const int ncu = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n < 1)
+      n = 8;
+    return n;
+  }();
+  const int ntiles = p.Pxy * (p.Pxy / Q);
+  int g = per_cu * ncu;
+  g -= g % 8;
+  g = std::max(8, std::min(g, ntiles));
+  hipLaunchKernelGGL((k_z_mult_dma<R, PZ, Q>), dim3(g), dim3(kNT), 0, st, b2,
+                     PlanBufs<R>::table(p), p.Nz, p.Pxy, PlanBufs<R>::tw(p),
+                     p.zmap + zmap_offset(p.Pxy, Q), ntiles, done);
+}
+
+template <typename R, int PZ, int Q>
+constexpr bool dma_fits() {
+  return (9 * Q * (int)sizeof(cx<R>)) % 16 == 0 &&
+         (PZ * lds_ld<9 * Q>::value + (PZ / 2) * 9 * Q) * (int)sizeof(cx<R>) <= kLdsMax;
+}
+
 template <typename R, int PZ>
 static int launch_z(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
   size_t slot;
   int rc = timing_begin(p, st, &slot, (int)sizeof(cx<R>));
   if (rc) return rc;
-  int q = p.tile[prec_of<R>()][2];
+  const int code = p.tile[prec_of<R>()][2];
+  int q = code & (kZDma - 1);
   while (q > 1 && p.Pxy % q != 0) q >>= 1;
+  if (code & kZDma) {
+    switch (q) {
+      case 8:
+        if constexpr (dma_fits<R, PZ, 8>()) { z_q_dma<R, PZ, 8>(p, b2, st, done); break; }
+        [[fallthrough]];
+      case 4:
+        if constexpr (dma_fits<R, PZ, 4>()) { z_q_dma<R, PZ, 4>(p, b2, st, done); break; }
+        [[fallthrough]];
+      default:
+        if constexpr (dma_fits<R, PZ, 2>()) {
+          if (p.Pxy % 2 == 0) { z_q_dma<R, PZ, 2>(p, b2, st, done); break; }
+        }
+        z_q<R, PZ, 1>(p, b2, st, done);
+    }
+    MSC_LAUNCH_CHECK();
+    return timing_end(p, st, slot);
+  }
   switch (q) {
     case 16:
       if constexpr (lds_fits<R, PZ, 9, 16>()) { z_q<R, PZ, 16>(p, b2, st, done); break; }
@@ -850,13 +1010,13 @@ int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st) {
   MSC_HIP_CHECK(hipEventCreate(&e1));
   static const int cand_y[] = {1, 2};
   static const int cand_x[] = {1, 2, 4};
-  static const int cand_z[] = {2, 4, 8};
+  static const int cand_z[] = {2, 4, 8, 2 | kZDma, 4 | kZDma, 8 | kZDma};
   static const int order[] = {2, 1, 3, 0, 4};
   int rc = apply<R>(p, x, nullptr, out, 2, st, nullptr);  // warm every buffer
   for (int pass : order) {
     if (rc) break;
     const int* cand = pass == 2 ? cand_z : (pass == 0 || pass == 4 ? cand_y : cand_x);
-    const int nc = pass == 2 ? 3 : (pass == 0 || pass == 4 ? 2 : 3);
+    const int nc = pass == 2 ? 6 : (pass == 0 || pass == 4 ? 2 : 3);
     int best = p.tile[pr][pass];
     float best_ms = 1e30f;
     for (int i = 0; i < nc && !rc; ++i) {

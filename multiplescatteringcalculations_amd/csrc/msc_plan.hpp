@@ -18,6 +18,7 @@
 
 namespace msc {
 
+constexpr int kZDma = 256;  // z tile code flag: persistent direct-to-LDS pass 3
 constexpr int kNE = 45;  // independent entries of the 9x9 spectral propagator
 
 // Component sign flips of the lattice reflections (z, x, y) acting on the
@@ -44,7 +45,8 @@ struct Plan {
 
   double omega = 0.0, alpha = 0.0, beta = 0.0, rho = 0.0;
   // tiles per precision (0 complex128, 1 complex64) and pass (y-forward B,
-  // x-forward Q, z Q, x-inverse Q, y-inverse B); see autotune()
+  // x-forward Q, z Q, x-inverse Q, y-inverse B); see autotune().  A z code
+  // Q | kZDma selects the persistent direct-to-LDS form of pass 3.
   int tile[2][5] = {{1, 1, 8, 1, 1}, {1, 2, 8, 2, 1}};
   bool tuned[2] = {false, false};
   bool have_table = false, have_tparams = false;
@@ -77,7 +79,8 @@ struct Plan {
   double* nn_stage = nullptr;  // 26 x 81 real volume-averaged blocks (device copy)
   // z-pass tile order per column tile width Q = 1, 2, 4, 8, 16 (concatenated,
   // zmap_offset): dispatch slot -> tile kx * (Pxy / Q) + chunk, grouping the
-  // rows kx and Pxy - kx (which read the same table entries) on one XCD.
+  // rows kx and Pxy - kx (which read the same table entries) on one XCD, with
+  // the mirrored ky chunks next to each other in time.
   int* zmap = nullptr;
   double2* tw128 = nullptr;
   float2* tw64 = nullptr;

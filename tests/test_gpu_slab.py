@@ -317,6 +317,62 @@ def test_full_size_operator_properties(hip_device):
     assert float(((x - y64) - (x - y128)).norm() / (x - y128).norm()) < 1e-5
 
 
+ZDMA = 256  # z tile code flag of the persistent direct-to-LDS z pass
+
+
+@pytest.mark.parametrize("q", [2, 4, 8])
+@pytest.mark.parametrize("M,Nz,periodic", [(9, 4, False), (7, 3, False), (8, 5, True),
+                                           (16, 1, False)])
+def test_persistent_z_pass_matches_oracle(hip_device, q, M, Nz, periodic):
+    """The persistent z pass (tile code Q | 256: the next tile's rows prefetched
+    into LDS by direct-to-LDS loads, z FFTs pruned to the Nz input and output
+    rows) gives the oracle's operator, with Nz at and below half the padded z
+    length, Pz from 2 to 16, periodic and aperiodic."""
+    ref, om = REF_C, 692.8
+    Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=M + 31 * Nz)
+    plan, *_ = _plan(M, Nz, 1.0, ref, om, Dl, Dm, Dr, periodic=periodic)
+    tiles = (1, 2, q | ZDMA, 2, 1)
+    plan.set_tiles(tiles, tiles)
+    T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, om, ref)
+    kh = osl.slab_kernels(M, Nz, 1.0, om, ref, periodic=periodic)
+    rng = np.random.default_rng(q + M)
+    n = Nz * M * M * 9
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y_ref = osl.slab_matvec(x, T, kh, M, Nz, periodic=periodic)
+    y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    assert _rel(x - y, x - y_ref) < 1e-11
+    y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
+    tol = 2e-5 * np.abs(x - y_ref).max() + 1e-7 * np.abs(x).max()
+    assert np.abs((x - y64) - (x - y_ref)).max() < tol
+
+
+def test_persistent_z_pass_full_size(hip_device):
+    """At 128 x 128 x 32 (16384 tiles of Q = 4, several per workgroup) the
+    persistent z pass equals the one-tile-per-workgroup pass, in complex128 to
+    rounding and in complex64 to its precision, and a mixed solve with it
+    reaches the same FP64 solution."""
+    M, Nz = 128, 32
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=0)
+    plan, g, r, mat = _plan(M, Nz, 1.0, REF_C, 0.4 * REF_C.beta, Dl, Dm, Dr)
+    gen = torch.Generator(device=hip_device).manual_seed(3)
+    x = torch.randn(plan.n, dtype=torch.complex128, device=hip_device, generator=gen)
+    out = {}
+    for code in (4, 4 | ZDMA, 2 | ZDMA, 8 | ZDMA):
+        plan.set_tiles((1, 1, code, 1, 1), (1, 2, code, 2, 1))
+        out[code] = (plan.matvec(x), plan.matvec(x.to(torch.complex64)).to(torch.complex128))
+    base = x - out[4][0]
+    for code in (4 | ZDMA, 2 | ZDMA, 8 | ZDMA):
+        assert float(((x - out[code][0]) - base).abs().max() / base.abs().max()) < 1e-13
+        assert float(((x - out[code][1]) - base).norm() / base.norm()) < 1e-5
+    b = plan.incident_field(0.4 * REF_C.beta, r, [0.893, 0.45, 0.0], "P").reshape(-1)
+    plan.set_tiles((1, 1, 4, 1, 1), (1, 2, 4, 2, 1))
+    x0, st0 = plan.gmres(b, "born", tol=1e-10, max_iter=200, restart=30, precision="mixed")
+    plan.set_tiles((1, 1, 4 | ZDMA, 1, 1), (1, 2, 4 | ZDMA, 2, 1))
+    x1, st1 = plan.gmres(b, "born", tol=1e-10, max_iter=200, restart=30, precision="mixed")
+    assert st0["converged"] and st1["converged"]
+    assert float((x1 - x0).abs().max() / x0.abs().max()) < 1e-8
+
+
 @pytest.mark.parametrize("tiles", [(1, 1, 1, 1, 1), (2, 2, 2, 2, 2), (1, 4, 4, 4, 1), (2, 1, 8, 2, 1),
                                    (1, 2, 16, 2, 1)])
 def test_every_tile_variant_matches_oracle(hip_device, tiles):
@@ -354,7 +410,9 @@ def test_autotune_and_graph_solves_are_reproducible(hip_device):
     plan.autotune("c128")
     plan.autotune("c64")
     t = plan.tiles()
-    assert all(v in (1, 2, 4, 8, 16) for v in t["c128"] + t["c64"])
+    for tiles in (t["c128"], t["c64"]):
+        assert all(v in (1, 2, 4) for v in tiles[:2] + tiles[3:])
+        assert tiles[2] in (1, 2, 4, 8, 16, 2 | ZDMA, 4 | ZDMA, 8 | ZDMA)
     psi0 = plan.incident_field(0.4 * REF_C.beta, r, [0.893, 0.45, 0.0], "P")
     outs = []
     for prec in ("fp64", "mixed", "fp64", "mixed"):
