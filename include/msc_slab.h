@@ -318,6 +318,16 @@ int msc_residual(msc_plan* plan, const void* b, const void* x, void* r, int prec
  */
 int msc_gmres(msc_plan* plan, const void* b, void* x, const msc_gmres_opts* opts,
               msc_gmres_stats* stats, void* stream);
+/* msc_gmres on `count` plans of one padded FFT grid (same P_z, P_xy and
+ * periodic flag; slab sizes may differ) as one batch: every operator pass and
+ * every Arnoldi kernel is a single launch for all systems, and each system
+ * keeps its own Krylov space, tolerance, restarts and statistics (stats[i]).
+ * Replaces the per-frequency solve loop of ocean_bottom.py:290-307 (one
+ * msc_gmres per frequency); cubic T0 only (no msc_set_tgeneral, no
+ * volume-averaged correction), no preconditioner.  A system whose complex64
+ * cycles stall is finished in FP64 cycles, as msc_gmres does. */
+int msc_gmres_batched(msc_plan* const* plans, const void* const* b, void* const* x, int count,
+                      const msc_gmres_opts* opts, msc_gmres_stats* stats, void* stream);
 
 /*
  * Sums of the sources tau = T0 psi (psi complex128): per layer

@@ -60,6 +60,8 @@ SIGNATURES = {
     "msc_matvec": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "msc_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "msc_gmres": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "msc_gmres_batched": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                  c_void_p]),
     "msc_source_sums": (c_int, [c_void_p, c_void_p, c_double, c_double, POINTER(c_double),
                                 c_void_p, c_void_p, c_void_p]),
     "msc_fft_axis": (c_int, [c_void_p, c_void_p, c_int, c_int, c_longlong, c_int, c_int,

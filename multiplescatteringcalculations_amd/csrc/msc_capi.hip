@@ -649,6 +649,336 @@ int solve(msc_plan* pl, const double2* b, double2* x, const msc_gmres_opts& o,
   return kOk;
 }
 
+// Batched GMRES over `ns` plans that share one padded FFT grid, with the
+// cycle structure of solve(): every operator pass and every Arnoldi kernel is
+// one launch for all systems (system = blockIdx.y, its own chunks, workspace
+// and done flag), and a system that has converged, or whose cycle is over,
+// turns its part of each launch into a no-op.  The host decides per system
+// after each cycle from one gathered copy of the residual norms.  The
+// complex64 cycles of MIXED use a unit right-hand side (as solve() does); the
+// FP64 cycles do too here, with tolerance tol ||b|| / ||r||, which is the same
+// stopping rule.  A system whose complex64 cycle stalls leaves the batch and is
+// finished by solve() in FP64 from its current iterate.  No learned
+// preconditioner, general T or near-neighbour correction.
+int solve_batched(msc_plan* const* pls, const double2* const* b, double2* const* x, int ns,
+                  const msc_gmres_opts& o, msc_gmres_stats* stats, hipStream_t st) {
+  MSC_REQUIRE(ns >= 1, "batched solve needs at least one system");
+  MSC_REQUIRE(o.precond == nullptr, "the batched solve takes no preconditioner");
+  const Plan& p0 = pls[0]->p;
+  size_t max_n = 0;
+  int max_nblk = 0, max_M = 0, max_Nz = 0;
+  for (int i = 0; i < ns; ++i) {
+    const Plan& p = pls[i]->p;
+    MSC_REQUIRE(p.Pz == p0.Pz && p.Pxy == p0.Pxy && p.periodic == p0.periodic &&
+                    p.device == p0.device,
+                "batched plans must share one padded grid and device");
+    MSC_REQUIRE(p.have_table && p.have_tparams,
+                "every plan needs build_g0 and set_tmatrix before the batched solve");
+    MSC_REQUIRE(!p.general && !p.have_nn,
+                "the batched solve takes the cubic T0 without the near-neighbour correction");
+    MSC_REQUIRE(b[i] != nullptr && x[i] != nullptr, "b and x must not be NULL");
+    max_n = std::max(max_n, p.n);
+    max_M = std::max(max_M, p.M);
+    max_Nz = std::max(max_Nz, p.Nz);
+  }
+  const int m = std::max(1, std::min(o.restart, std::max(o.max_iter, 1)));
+  const bool mixed = o.mode == MSC_SOLVE_MIXED;
+  int rc = kOk;
+  for (int i = 0; i < ns; ++i) {
+    msc_plan* pl = pls[i];
+    const size_t n = pl->p.n;
+    rc = mixed ? pl->k128.ensure_state(n, m, sizeof(double2)) : pl->k128.ensure(n, m, sizeof(double2));
+    if (rc) return rc;
+    pl->k128.shifted = true;
+    if (mixed) {
+      rc = pl->k64.ensure(n, m, sizeof(float2));
+      if (rc) return rc;
+      pl->k64.shifted = true;
+      if (pl->b64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->b64, pl->p.cap_n * sizeof(float2)));
+      if (pl->r64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r64, pl->p.cap_n * sizeof(float2)));
+    }
+    if (pl->r128 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r128, pl->p.cap_n * sizeof(double2)));
+    max_nblk = std::max(max_nblk, pl->k128.nblk);
+  }
+  // the batched passes use the first plan's tiles (tuned once per grid)
+  Plan& q0 = pls[0]->p;
+  if (autotune_enabled()) {
+    if (!q0.tuned[0] && !lookup_tiles(q0, 0)) {
+      rc = autotune<double>(q0, b[0], pls[0]->r128, st);
+      if (rc) return rc;
+      store_tiles(q0, 0);
+    }
+    if (mixed && !q0.tuned[1] && !lookup_tiles(q0, 1)) {
+      rc = krylov_narrow(b[0], pls[0]->b64, q0.n, st);
+      if (!rc) rc = autotune<float>(q0, pls[0]->b64, pls[0]->r64, st);
+      if (rc) return rc;
+      store_tiles(q0, 1);
+    }
+  }
+
+  // host descriptors: Krylov of FP64 norms / cycles, Krylov of complex64
+  // cycles, and the operator launches they need
+  std::vector<BKry> h128(ns), h64(ns);
+  std::vector<BPass<double>> hres(ns), hstep128(ns);
+  std::vector<BPass<float>> hcheap(ns), hstep64(ns);
+  const size_t bytes_kry = sizeof(BKry) * ns, bytes_p128 = sizeof(BPass<double>) * ns,
+               bytes_p64 = sizeof(BPass<float>) * ns;
+  const size_t off_k64 = bytes_kry, off_res = 2 * bytes_kry, off_s128 = off_res + bytes_p128,
+               off_cheap = off_s128 + bytes_p128, off_s64 = off_cheap + bytes_p64,
+               off_act = off_s64 + bytes_p64, off_fin = off_act + sizeof(int) * ns,
+               off_gat = (off_fin + sizeof(int) * ns + 15) / 16 * 16,
+               total = off_gat + sizeof(double) * 2 * ns;
+  unsigned char* dev = nullptr;
+  MSC_HIP_CHECK(hipMalloc(&dev, total));
+  struct Free {
+    unsigned char* p;
+    ~Free() { if (p) (void)hipFree(p); }
+  } free_dev{dev};
+  BKry* d128 = reinterpret_cast<BKry*>(dev);
+  BKry* d64 = reinterpret_cast<BKry*>(dev + off_k64);
+  BPass<double>* dres = reinterpret_cast<BPass<double>*>(dev + off_res);
+  BPass<double>* ds128 = reinterpret_cast<BPass<double>*>(dev + off_s128);
+  BPass<float>* dcheap = reinterpret_cast<BPass<float>*>(dev + off_cheap);
+  BPass<float>* ds64 = reinterpret_cast<BPass<float>*>(dev + off_s64);
+  int* dact = reinterpret_cast<int*>(dev + off_act);
+  int* dfin = reinterpret_cast<int*>(dev + off_fin);
+  double* dgat = reinterpret_cast<double*>(dev + off_gat);
+  for (int i = 0; i < ns; ++i) {
+    msc_plan* pl = pls[i];
+    const Plan& p = pl->p;
+    h128[i] = bkry_of(pl->k128);
+    h128[i].r = b[i];
+    h128[i].x = x[i];
+    if (mixed) {
+      h64[i] = bkry_of(pl->k64);
+      h64[i].x = x[i];
+      h64[i].src = b[i];
+      h64[i].dst = pl->b64;
+    }
+    auto common = [&](auto& d, const auto* tp, const auto* table) {
+      d.tp = tp;
+      d.buf1 = reinterpret_cast<decltype(d.buf1)>(p.buf1);
+      d.buf2 = reinterpret_cast<decltype(d.buf2)>(p.buf2);
+      d.table = table;
+      d.M = p.M;
+      d.Nz = p.Nz;
+      d.in_step = d.out_step = 0;
+      d.bvec = nullptr;
+      d.done = nullptr;
+    };
+    common(hres[i], p.tp128, p.table128);
+    hres[i].in = x[i];
+    hres[i].out = pl->r128;
+    hres[i].bvec = b[i];
+    hres[i].done = dfin + i;
+    common(hstep128[i], p.tp128, p.table128);
+    if (!mixed) {
+      double2* V = reinterpret_cast<double2*>(pl->k128.V);
+      hstep128[i].in = V;
+      hstep128[i].out = V + pl->k128.ldv;
+      hstep128[i].in_step = hstep128[i].out_step = pl->k128.ldv;
+      hstep128[i].done = &pl->k128.st->done;
+    }
+    common(hcheap[i], p.tp64, p.table64);
+    common(hstep64[i], p.tp64, p.table64);
+    if (mixed) {
+      hcheap[i].in = pl->b64;
+      hcheap[i].out = pl->r64;
+      float2* V = reinterpret_cast<float2*>(pl->k64.V);
+      hstep64[i].in = V;
+      hstep64[i].out = V + pl->k64.ldv;
+      hstep64[i].in_step = hstep64[i].out_step = pl->k64.ldv;
+      hstep64[i].done = &pl->k64.st->done;
+    }
+  }
+  std::vector<int> act(ns, 1), fin(ns, 0), stalled(ns, 0), broke(ns, 0);
+  std::vector<double> gat(2 * ns), bnorm(ns), prev(ns, INFINITY);
+  auto upload = [&]() -> int {
+    MSC_HIP_CHECK(hipMemcpyAsync(d128, h128.data(), bytes_kry, hipMemcpyHostToDevice, st));
+    if (mixed) MSC_HIP_CHECK(hipMemcpyAsync(d64, h64.data(), bytes_kry, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipMemcpyAsync(dres, hres.data(), bytes_p128, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipMemcpyAsync(ds128, hstep128.data(), bytes_p128, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipMemcpyAsync(dcheap, hcheap.data(), bytes_p64, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipMemcpyAsync(ds64, hstep64.data(), bytes_p64, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipMemcpyAsync(dact, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipMemcpyAsync(dfin, fin.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+    return kOk;
+  };
+  auto gather = [&](const BKry* d) -> int {
+    int r2 = bkrylov_gather(d, ns, dgat, st);
+    if (r2) return r2;
+    MSC_HIP_CHECK(hipMemcpyAsync(gat.data(), dgat, sizeof(double) * 2 * ns,
+                                 hipMemcpyDeviceToHost, st));
+    MSC_HIP_CHECK(hipStreamSynchronize(st));
+    return kOk;
+  };
+  rc = upload();
+  if (rc) return rc;
+  // ||b_i||
+  rc = bkrylov_norm<double>(d128, ns, max_nblk, st);
+  if (!rc) rc = gather(d128);
+  if (rc) return rc;
+  std::vector<msc_gmres_stats> out(ns);
+  for (int i = 0; i < ns; ++i) {
+    out[i] = msc_gmres_stats{};
+    bnorm[i] = gat[2 * i];
+    out[i].bnorm = bnorm[i];
+    const size_t bytes = pls[i]->p.n * sizeof(double2);
+    if (o.x0 == MSC_X0_ZERO) MSC_HIP_CHECK(hipMemsetAsync(x[i], 0, bytes, st));
+    else if (o.x0 == MSC_X0_BORN)
+      MSC_HIP_CHECK(hipMemcpyAsync(x[i], b[i], bytes, hipMemcpyDeviceToDevice, st));
+    if (bnorm[i] < 1e-30) {  // zero right-hand side
+      out[i].converged = 1;
+      fin[i] = 1;
+      act[i] = 0;
+    }
+  }
+  const double eta_min = o.inner_tol > 0.0 ? o.inner_tol : 1e-4;
+  const BKry* dcyc = mixed ? d64 : d128;
+  std::vector<BKry>& hcyc = mixed ? h64 : h128;
+  bool cheap = mixed && o.x0 != MSC_X0_GIVEN;
+  for (;;) {
+    const bool from64 = cheap && o.x0 == MSC_X0_BORN;
+    // residual norms of the systems still running, into the cycle Krylov
+    for (int i = 0; i < ns; ++i) hcyc[i].r = from64 ? (const void*)pls[i]->r64
+                                          : (cheap ? (const void*)b[i] : (const void*)pls[i]->r128);
+    rc = upload();
+    if (rc) return rc;
+    if (from64) {
+      rc = bkrylov_narrow(d64, ns, max_n, st);
+      if (!rc) rc = apply_batch<float>(q0, dcheap, ns, max_M, max_Nz, 2, 0, st);
+      if (!rc) rc = bkrylov_norm<float>(d64, ns, max_nblk, st);
+      for (int i = 0; i < ns; ++i) out[i].matvecs_c64 += fin[i] ? 0 : 1;
+    } else if (cheap) {
+      rc = bkrylov_norm<double>(dcyc, ns, max_nblk, st);  // x0 = 0: r = b
+    } else {
+      rc = apply_batch<double>(q0, dres, ns, max_M, max_Nz, 1, 0, st);
+      if (!rc) rc = bkrylov_norm<double>(dcyc, ns, max_nblk, st);
+      for (int i = 0; i < ns; ++i) out[i].matvecs_c128 += fin[i] ? 0 : 1;
+    }
+    if (!rc) rc = gather(dcyc);
+    if (rc) return rc;
+    int running = 0;
+    for (int i = 0; i < ns; ++i) {
+      act[i] = 0;
+      if (fin[i]) continue;
+      const double rnorm = gat[2 * i];
+      const double rel = rnorm / bnorm[i];
+      if (cheap) {
+        if (!(rel > o.tol)) continue;  // confirmed by an FP64 residual next round
+      } else {
+        out[i].rel_residual = rel;
+        if (!(rel > o.tol) || out[i].iterations >= o.max_iter || broke[i]) {
+          fin[i] = 1;
+          continue;
+        }
+        if (mixed && out[i].cycles > 0 && rel > 0.9 * prev[i]) {
+          stalled[i] = 1;  // finished below in FP64
+          fin[i] = 1;
+          continue;
+        }
+      }
+      prev[i] = rel;
+      act[i] = 1;
+      ++running;
+      if (mixed) {
+        const double need = 0.3 * o.tol / rel;
+        hcyc[i].tol_abs = std::min(std::max(eta_min, need), 0.5);
+      } else {
+        hcyc[i].tol_abs = o.tol * bnorm[i] / rnorm;
+      }
+    }
+    const bool was_cheap = cheap;
+    cheap = false;
+    if (running == 0) {
+      if (was_cheap) continue;
+      break;
+    }
+    int steps = 0;
+    for (int i = 0; i < ns; ++i)
+      if (act[i]) steps = std::max(steps, std::min(m, o.max_iter - out[i].iterations));
+    rc = upload();
+    if (rc) return rc;
+    for (int i = 0; i < ns; ++i) {
+      Krylov& kk = mixed ? pls[i]->k64 : pls[i]->k128;
+      *kk.done_flag = act[i] ? 0 : 1;
+    }
+    rc = bkrylov_deactivate(dcyc, ns, dact, st);
+    if (!rc) {
+      if (from64) rc = bkrylov_cycle_init<float, float>(dcyc, ns, max_n, true, st);
+      else if (mixed) rc = bkrylov_cycle_init<double, float>(dcyc, ns, max_n, true, st);
+      else rc = bkrylov_cycle_init<double, double>(dcyc, ns, max_n, true, st);
+    }
+    if (rc) return rc;
+    // the cycle: at most two steps queued ahead of the device
+    hipEvent_t ev[2];
+    MSC_HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    MSC_HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    for (int j = 0; j < steps && rc == kOk; ++j) {
+      if (j >= 2) {
+        if (hipEventSynchronize(ev[j & 1]) != hipSuccess) {
+          set_error("hipEventSynchronize failed in the batched cycle");
+          rc = kHip;
+          break;
+        }
+        bool all = true;
+        for (int i = 0; i < ns && all; ++i) {
+          Krylov& kk = mixed ? pls[i]->k64 : pls[i]->k128;
+          all = *reinterpret_cast<volatile int*>(kk.done_flag) != 0;
+        }
+        if (all) break;
+      }
+      if (mixed) {
+        rc = apply_batch<float>(q0, ds64, ns, max_M, max_Nz, 2, j, st);
+        if (!rc) rc = bkrylov_step<float>(d64, ns, max_n, max_nblk, j, o.orth, st);
+      } else {
+        rc = apply_batch<double>(q0, ds128, ns, max_M, max_Nz, 2, j, st);
+        if (!rc) rc = bkrylov_step<double>(d128, ns, max_n, max_nblk, j, o.orth, st);
+      }
+      if (!rc && hipEventRecord(ev[j & 1], st) != hipSuccess) {
+        set_error("hipEventRecord failed in the batched cycle");
+        rc = kHip;
+      }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
+    rc = mixed ? bkrylov_finish<float>(d64, ns, max_n, st) : bkrylov_finish<double>(d128, ns, max_n, st);
+    if (!rc) rc = gather(dcyc);
+    if (rc) return rc;
+    for (int i = 0; i < ns; ++i) {
+      if (!act[i]) continue;
+      const int taken = (int)gat[2 * i + 1];
+      out[i].iterations += taken;
+      ++out[i].cycles;
+      if (mixed) out[i].matvecs_c64 += taken;
+      else out[i].matvecs_c128 += taken;
+      if (taken == 0) broke[i] = 1;  // breakdown at the first step: stop after the residual
+    }
+  }
+  // systems whose complex64 cycle stalled: FP64 cycles from the current iterate
+  for (int i = 0; i < ns; ++i) {
+    if (stalled[i]) {
+      msc_gmres_opts o2 = o;
+      o2.mode = MSC_SOLVE_FP64;
+      o2.x0 = MSC_X0_GIVEN;
+      o2.max_iter = std::max(1, o.max_iter - out[i].iterations);
+      msc_gmres_stats s2{};
+      rc = solve(pls[i], b[i], x[i], o2, &s2, st);
+      if (rc) return rc;
+      out[i].iterations += s2.iterations;
+      out[i].cycles += s2.cycles;
+      out[i].matvecs_c128 += s2.matvecs_c128;
+      out[i].rel_residual = s2.rel_residual;
+      out[i].fell_back |= 1;
+    }
+    out[i].converged = out[i].rel_residual <= o.tol ? 1 : 0;
+    stats[i] = out[i];
+  }
+  MSC_HIP_CHECK(hipStreamSynchronize(st));
+  return kOk;
+}
+
 }  // namespace
 
 // ── exported C ABI ───────────────────────────────────────────────────────────
@@ -1002,6 +1332,23 @@ int msc_gmres(msc_plan* plan, const void* b, void* x, const msc_gmres_opts* opts
   }
   return solve(plan, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x), *opts,
                stats, S(stream));
+}
+
+int msc_gmres_batched(msc_plan* const* plans, const void* const* b, void* const* x, int count,
+                      const msc_gmres_opts* opts, msc_gmres_stats* stats, void* stream) {
+  MSC_REQUIRE(plans != nullptr && b != nullptr && x != nullptr && stats != nullptr && count >= 1,
+              "NULL argument or empty batch");
+  int rc = check_opts(opts);
+  if (rc) return rc;
+  MSC_REQUIRE(opts->precond == nullptr, "msc_gmres_batched takes no preconditioner");
+  for (int i = 0; i < count; ++i) {
+    MSC_REQUIRE(plans[i] != nullptr, "NULL plan in the batch");
+    MSC_REQUIRE(b[i] != x[i], "b and x must not alias");
+    for (int k = 0; k < i; ++k)
+      MSC_REQUIRE(plans[k] != plans[i], "a plan appears twice in the batch");
+  }
+  return solve_batched(plans, reinterpret_cast<const double2* const*>(b),
+                       reinterpret_cast<double2* const*>(x), count, *opts, stats, S(stream));
 }
 
 int msc_source_sums(msc_plan* plan, const void* psi, double kP, double kS, const double* rhat3,

@@ -64,13 +64,13 @@ __device__ __forceinline__ bool skip_step(const KState* st, int pass) {
 // partial[blk][v] = sum_{t in chunk} conj(V_v[t]) w[t], v < nv;
 // partial[blk][nv] = sum |w|^2 (stored in .x).
 template <typename R>
-__global__ __launch_bounds__(kRT) void k_dots(const cx<R>* __restrict__ V, size_t ldv, int nv,
+__device__ __forceinline__ void dots_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ V, size_t ldv, int nv,
                                               const cx<R>* __restrict__ w, size_t n,
                                               double2* __restrict__ part, const KState* st,
                                               int pass) {
   __shared__ double2 red[kRT / 64];
   if (skip_step(st, pass)) return;
-  const size_t beg = (size_t)blockIdx.x * kChunk;
+  const size_t beg = (size_t)bx_ * kChunk;
   const size_t end = beg + kChunk < n ? beg + kChunk : n;
   const int ldp = nv + 1;
   double2 wn = make_double2(0.0, 0.0);
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kRT) void k_dots(const cx<R>* __restrict__ V, size_
     wn.x += a.x * a.x + a.y * a.y;
   }
   wn = block_sum(wn, red);
-  if (threadIdx.x == 0) part[(size_t)blockIdx.x * ldp + nv] = wn;
+  if (threadIdx.x == 0) part[(size_t)bx_ * ldp + nv] = wn;
   constexpr int VB = 8;
   for (int v0 = 0; v0 < nv; v0 += VB) {
     double2 acc[VB];
@@ -101,11 +101,19 @@ __global__ __launch_bounds__(kRT) void k_dots(const cx<R>* __restrict__ V, size_
     for (int k = 0; k < VB; ++k) {
       if (v0 + k < nv) {
         const double2 s = block_sum(acc[k], red);
-        if (threadIdx.x == 0) part[(size_t)blockIdx.x * ldp + v0 + k] = s;
+        if (threadIdx.x == 0) part[(size_t)bx_ * ldp + v0 + k] = s;
       }
     }
   }
 }
+template <typename R>
+__global__ __launch_bounds__(kRT) void k_dots(const cx<R>* __restrict__ V, size_t ldv, int nv,
+                                              const cx<R>* __restrict__ w, size_t n,
+                                              double2* __restrict__ part, const KState* st,
+                                              int pass) {
+  dots_body<R>(blockIdx.x, gridDim.x, V, ldv, nv, w, n, part, st, pass);
+}
+
 
 __device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
 __device__ __forceinline__ double2 cm(double2 a, double2 b) {
@@ -183,7 +191,7 @@ __device__ void finish_step(KState* st, double2* __restrict__ H, int ldh,
 // Pass 0 also decides, from ||w||^2 - |h|^2 (exact for an orthonormal basis),
 // whether a second Gram-Schmidt pass is needed; if not it closes the step with
 // that norm, and k_update writes the normalised vector directly.
-__global__ __launch_bounds__(kRT) void k_reduce_dots(const double2* __restrict__ part, int nblk,
+__device__ __forceinline__ void reduce_dots_body(unsigned bx_, unsigned gx_, const double2* __restrict__ part, int nblk,
                                                      int nv, KState* st, double2* __restrict__ H,
                                                      int ldh, double2* __restrict__ hcur,
                                                      double2* __restrict__ cs,
@@ -232,12 +240,22 @@ __global__ __launch_bounds__(kRT) void k_reduce_dots(const double2* __restrict__
   st->reorth = 0;
   finish_step(st, H, ldh, cs, g, hnext, m, sqrt(est > 0.0 ? est : 0.0), shifted);
 }
+__global__ __launch_bounds__(kRT) void k_reduce_dots(const double2* __restrict__ part, int nblk,
+                                                     int nv, KState* st, double2* __restrict__ H,
+                                                     int ldh, double2* __restrict__ hcur,
+                                                     double2* __restrict__ cs,
+                                                     double2* __restrict__ g,
+                                                     double* __restrict__ hnext, int m, int pass,
+                                                     int orth_mode, int shifted) {
+  reduce_dots_body(blockIdx.x, gridDim.x, part, nblk, nv, st, H, ldh, hcur, cs, g, hnext, m, pass, orth_mode, shifted);
+}
+
 
 // w -= sum_v hcur[v] V_v.  Without a second pass pending (pass 0, reorth == 0)
 // the result is scaled to unit norm in the same sweep; with one pending the
 // unscaled vector is kept.  Pass 1 writes per-workgroup ||w||^2 partials.
 template <typename R>
-__global__ __launch_bounds__(kRT) void k_update(const cx<R>* __restrict__ V, size_t ldv, int nv,
+__device__ __forceinline__ void update_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ V, size_t ldv, int nv,
                                                 cx<R>* __restrict__ w, size_t n,
                                                 const double2* __restrict__ hcur,
                                                 double* __restrict__ part, const KState* st,
@@ -246,7 +264,7 @@ __global__ __launch_bounds__(kRT) void k_update(const cx<R>* __restrict__ V, siz
   if (pass == 0 ? st->done != 0 : skip_step(st, pass)) return;
   const bool finalise = pass == 0 && st->reorth == 0;
   const double sc = finalise ? st->scale : 1.0;
-  const size_t beg = (size_t)blockIdx.x * kChunk;
+  const size_t beg = (size_t)bx_ * kChunk;
   const size_t end = beg + kChunk < n ? beg + kChunk : n;
   double2 nrm = make_double2(0.0, 0.0);
   for (size_t t = beg + threadIdx.x; t < end; t += kRT) {
@@ -264,9 +282,18 @@ __global__ __launch_bounds__(kRT) void k_update(const cx<R>* __restrict__ V, siz
   }
   if (pass == 1) {
     nrm = block_sum(nrm, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = nrm.x;
+    if (threadIdx.x == 0) part[bx_] = nrm.x;
   }
 }
+template <typename R>
+__global__ __launch_bounds__(kRT) void k_update(const cx<R>* __restrict__ V, size_t ldv, int nv,
+                                                cx<R>* __restrict__ w, size_t n,
+                                                const double2* __restrict__ hcur,
+                                                double* __restrict__ part, const KState* st,
+                                                int pass) {
+  update_body<R>(blockIdx.x, gridDim.x, V, ldv, nv, w, n, hcur, part, st, pass);
+}
+
 
 __device__ double reduce_scalar_partials(const double* part, int nblk, double* red) {
   // one workgroup, fixed order
@@ -283,7 +310,7 @@ __device__ double reduce_scalar_partials(const double* part, int nblk, double* r
 }
 
 // After a second Gram-Schmidt pass: exact norm, then close the step.
-__global__ __launch_bounds__(kRT) void k_norm_givens(const double* __restrict__ part, int nblk,
+__device__ __forceinline__ void norm_givens_body(unsigned bx_, unsigned gx_, const double* __restrict__ part, int nblk,
                                                      KState* st, double2* __restrict__ H, int ldh,
                                                      double2* __restrict__ cs,
                                                      double2* __restrict__ g,
@@ -295,21 +322,36 @@ __global__ __launch_bounds__(kRT) void k_norm_givens(const double* __restrict__ 
   if (threadIdx.x != 0) return;
   finish_step(st, H, ldh, cs, g, hnext, m, sqrt(wn), shifted);
 }
+__global__ __launch_bounds__(kRT) void k_norm_givens(const double* __restrict__ part, int nblk,
+                                                     KState* st, double2* __restrict__ H, int ldh,
+                                                     double2* __restrict__ cs,
+                                                     double2* __restrict__ g,
+                                                     double* __restrict__ hnext, int m,
+                                                     int shifted) {
+  norm_givens_body(blockIdx.x, gridDim.x, part, nblk, st, H, ldh, cs, g, hnext, m, shifted);
+}
+
 
 // V[j+1] = w / ||w|| after a second pass (zero on breakdown).
 template <typename R>
-__global__ __launch_bounds__(kRT) void k_scale_next(cx<R>* __restrict__ w, size_t n,
+__device__ __forceinline__ void scale_next_body(unsigned bx_, unsigned gx_, cx<R>* __restrict__ w, size_t n,
                                                     const KState* st) {
   if (skip_step(st, 1)) return;
   const double inv = st->scale;
-  for (size_t t = (size_t)blockIdx.x * kRT + threadIdx.x; t < n; t += (size_t)gridDim.x * kRT) {
+  for (size_t t = (size_t)bx_ * kRT + threadIdx.x; t < n; t += (size_t)gx_ * kRT) {
     const double2 a = to_c128(w[t]);
     w[t] = from_c128<R>(make_double2(a.x * inv, a.y * inv));
   }
 }
+template <typename R>
+__global__ __launch_bounds__(kRT) void k_scale_next(cx<R>* __restrict__ w, size_t n,
+                                                    const KState* st) {
+  scale_next_body<R>(blockIdx.x, gridDim.x, w, n, st);
+}
+
 
 // Back-substitution R y = g on the leading j x j block (one workgroup).
-__global__ __launch_bounds__(kRT) void k_backsub(const KState* st, const double2* __restrict__ H,
+__device__ __forceinline__ void backsub_body(unsigned bx_, unsigned gx_, const KState* st, const double2* __restrict__ H,
                                                  int ldh, const double2* __restrict__ g,
                                                  double2* __restrict__ y, double2* __restrict__ tmp) {
   const int j = st->j;
@@ -334,16 +376,22 @@ __global__ __launch_bounds__(kRT) void k_backsub(const KState* st, const double2
     __syncthreads();
   }
 }
+__global__ __launch_bounds__(kRT) void k_backsub(const KState* st, const double2* __restrict__ H,
+                                                 int ldh, const double2* __restrict__ g,
+                                                 double2* __restrict__ y, double2* __restrict__ tmp) {
+  backsub_body(blockIdx.x, gridDim.x, st, H, ldh, g, y, tmp);
+}
+
 
 // x += scale * sum_{v < j} y_v V_v  (x complex128 or the basis precision).
 template <typename R, typename RX>
-__global__ __launch_bounds__(kRT) void k_axpy_basis(const cx<R>* __restrict__ V, size_t ldv,
+__device__ __forceinline__ void axpy_basis_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ V, size_t ldv,
                                                     const double2* __restrict__ y,
                                                     const KState* st, cx<RX>* __restrict__ x,
                                                     size_t n, const double* __restrict__ scale) {
   const int j = st->j;
   const double sc = scale != nullptr ? *scale : 1.0;
-  for (size_t t = (size_t)blockIdx.x * kRT + threadIdx.x; t < n; t += (size_t)gridDim.x * kRT) {
+  for (size_t t = (size_t)bx_ * kRT + threadIdx.x; t < n; t += (size_t)gx_ * kRT) {
     double2 acc = make_double2(0.0, 0.0);
     for (int v = 0; v < j; ++v) {
       const double2 b = to_c128(V[(size_t)v * ldv + t]);
@@ -357,13 +405,21 @@ __global__ __launch_bounds__(kRT) void k_axpy_basis(const cx<R>* __restrict__ V,
     x[t] = from_c128<RX>(xv);
   }
 }
+template <typename R, typename RX>
+__global__ __launch_bounds__(kRT) void k_axpy_basis(const cx<R>* __restrict__ V, size_t ldv,
+                                                    const double2* __restrict__ y,
+                                                    const KState* st, cx<RX>* __restrict__ x,
+                                                    size_t n, const double* __restrict__ scale) {
+  axpy_basis_body<R, RX>(blockIdx.x, gridDim.x, V, ldv, y, st, x, n, scale);
+}
+
 
 // partial[blk] = sum |r|^2 over the block's chunk.
 template <typename R>
-__global__ __launch_bounds__(kRT) void k_norm2_part(const cx<R>* __restrict__ r, size_t n,
+__device__ __forceinline__ void norm2_part_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ r, size_t n,
                                                     double* __restrict__ part) {
   __shared__ double2 red[kRT / 64];
-  const size_t beg = (size_t)blockIdx.x * kChunk;
+  const size_t beg = (size_t)bx_ * kChunk;
   const size_t end = beg + kChunk < n ? beg + kChunk : n;
   double2 s = make_double2(0.0, 0.0);
   for (size_t t = beg + threadIdx.x; t < end; t += kRT) {
@@ -371,31 +427,42 @@ __global__ __launch_bounds__(kRT) void k_norm2_part(const cx<R>* __restrict__ r,
     s.x += a.x * a.x + a.y * a.y;
   }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s.x;
+  if (threadIdx.x == 0) part[bx_] = s.x;
+}
+template <typename R>
+__global__ __launch_bounds__(kRT) void k_norm2_part(const cx<R>* __restrict__ r, size_t n,
+                                                    double* __restrict__ part) {
+  norm2_part_body<R>(blockIdx.x, gridDim.x, r, n, part);
 }
 
+
 // *out = sqrt(sum partials)
-__global__ __launch_bounds__(kRT) void k_norm_finish(const double* __restrict__ part, int nblk,
+__device__ __forceinline__ void norm_finish_body(unsigned bx_, unsigned gx_, const double* __restrict__ part, int nblk,
                                                      double* __restrict__ out) {
   __shared__ double red[kRT / 64];
   const double s = reduce_scalar_partials(part, nblk, red);
   if (threadIdx.x == 0) *out = sqrt(s);
 }
+__global__ __launch_bounds__(kRT) void k_norm_finish(const double* __restrict__ part, int nblk,
+                                                     double* __restrict__ out) {
+  norm_finish_body(blockIdx.x, gridDim.x, part, nblk, out);
+}
+
 
 // Start a cycle: V0 = r / ||r|| (r in precision RS, V0 in precision R),
 // g = (||r|| or 1, 0, ...), reset the step state.
 template <typename RS, typename R>
-__global__ __launch_bounds__(kRT) void k_cycle_init(const cx<RS>* __restrict__ r, size_t n,
+__device__ __forceinline__ void cycle_init_body(unsigned bx_, unsigned gx_, const cx<RS>* __restrict__ r, size_t n,
                                                     cx<R>* __restrict__ V0, KState* st,
                                                     double2* __restrict__ g, int m,
                                                     double tol_abs, int unit_rhs) {
   const double rn = st->rnorm;
   const double inv = rn > 0.0 ? 1.0 / rn : 0.0;
-  for (size_t t = (size_t)blockIdx.x * kRT + threadIdx.x; t < n; t += (size_t)gridDim.x * kRT) {
+  for (size_t t = (size_t)bx_ * kRT + threadIdx.x; t < n; t += (size_t)gx_ * kRT) {
     const double2 a = to_c128(r[t]);
     V0[t] = from_c128<R>(make_double2(a.x * inv, a.y * inv));
   }
-  if (blockIdx.x == 0) {
+  if (bx_ == 0) {
     for (int i = threadIdx.x; i <= m; i += kRT) g[i] = make_double2(0.0, 0.0);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -412,11 +479,114 @@ __global__ __launch_bounds__(kRT) void k_cycle_init(const cx<RS>* __restrict__ r
     }
   }
 }
+template <typename RS, typename R>
+__global__ __launch_bounds__(kRT) void k_cycle_init(const cx<RS>* __restrict__ r, size_t n,
+                                                    cx<R>* __restrict__ V0, KState* st,
+                                                    double2* __restrict__ g, int m,
+                                                    double tol_abs, int unit_rhs) {
+  cycle_init_body<RS, R>(blockIdx.x, gridDim.x, r, n, V0, st, g, m, tol_abs, unit_rhs);
+}
 
+
+__device__ __forceinline__ void narrow_body(unsigned bx_, unsigned gx_, const double2* __restrict__ in,
+                                                float2* __restrict__ out, size_t n) {
+  for (size_t t = (size_t)bx_ * kRT + threadIdx.x; t < n; t += (size_t)gx_ * kRT)
+    out[t] = make_float2((float)in[t].x, (float)in[t].y);
+}
 __global__ __launch_bounds__(kRT) void k_narrow(const double2* __restrict__ in,
                                                 float2* __restrict__ out, size_t n) {
-  for (size_t t = (size_t)blockIdx.x * kRT + threadIdx.x; t < n; t += (size_t)gridDim.x * kRT)
-    out[t] = make_float2((float)in[t].x, (float)in[t].y);
+  narrow_body(blockIdx.x, gridDim.x, in, out, n);
+}
+
+
+// ── batched forms: system q = d[blockIdx.y], its own chunks / grid stride ────
+template <typename R>
+__global__ __launch_bounds__(kRT) void kb_dots(const BKry* __restrict__ d, int nv, int pass) {
+  const BKry& q = d[blockIdx.y];
+  if ((int)blockIdx.x >= q.nblk) return;
+  const cx<R>* V = reinterpret_cast<const cx<R>*>(q.V);
+  dots_body<R>(blockIdx.x, q.nblk, V, q.ldv, nv, V + (size_t)nv * q.ldv, q.n, q.part, q.st, pass);
+}
+
+__global__ __launch_bounds__(kRT) void kb_reduce_dots(const BKry* __restrict__ d, int nv, int pass,
+                                                      int orth_mode) {
+  const BKry& q = d[blockIdx.y];
+  reduce_dots_body(0, 1, q.part, q.nblk, nv, q.st, q.H, q.ldh, q.hcur, q.cs, q.g, q.hnext, q.m,
+                   pass, orth_mode, q.shifted);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kRT) void kb_update(const BKry* __restrict__ d, int nv, int pass) {
+  const BKry& q = d[blockIdx.y];
+  if ((int)blockIdx.x >= q.nblk) return;
+  cx<R>* V = reinterpret_cast<cx<R>*>(q.V);
+  update_body<R>(blockIdx.x, q.nblk, V, q.ldv, nv, V + (size_t)nv * q.ldv, q.n, q.hcur,
+                 reinterpret_cast<double*>(q.part), q.st, pass);
+}
+
+__global__ __launch_bounds__(kRT) void kb_norm_givens(const BKry* __restrict__ d) {
+  const BKry& q = d[blockIdx.y];
+  norm_givens_body(0, 1, reinterpret_cast<const double*>(q.part), q.nblk, q.st, q.H, q.ldh, q.cs,
+                   q.g, q.hnext, q.m, q.shifted);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kRT) void kb_scale_next(const BKry* __restrict__ d, int nv) {
+  const BKry& q = d[blockIdx.y];
+  scale_next_body<R>(blockIdx.x, gridDim.x, reinterpret_cast<cx<R>*>(q.V) + (size_t)nv * q.ldv,
+                     q.n, q.st);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kRT) void kb_norm2_part(const BKry* __restrict__ d) {
+  const BKry& q = d[blockIdx.y];
+  if ((int)blockIdx.x >= q.nblk) return;
+  norm2_part_body<R>(blockIdx.x, q.nblk, reinterpret_cast<const cx<R>*>(q.r), q.n,
+                     reinterpret_cast<double*>(q.part));
+}
+
+// ||r|| into the system's KState::rnorm
+__global__ __launch_bounds__(kRT) void kb_norm_finish(const BKry* __restrict__ d) {
+  const BKry& q = d[blockIdx.y];
+  norm_finish_body(0, 1, reinterpret_cast<const double*>(q.part), q.nblk, &q.st->rnorm);
+}
+
+template <typename RS, typename R>
+__global__ __launch_bounds__(kRT) void kb_cycle_init(const BKry* __restrict__ d, int unit_rhs) {
+  const BKry& q = d[blockIdx.y];
+  cycle_init_body<RS, R>(blockIdx.x, gridDim.x, reinterpret_cast<const cx<RS>*>(q.r), q.n,
+                         reinterpret_cast<cx<R>*>(q.V), q.st, q.g, q.m, q.tol_abs, unit_rhs);
+}
+
+__global__ __launch_bounds__(kRT) void kb_backsub(const BKry* __restrict__ d) {
+  const BKry& q = d[blockIdx.y];
+  backsub_body(0, 1, q.st, q.H, q.ldh, q.g, q.y, q.tmp);
+}
+
+// x += ||r|| sum_v y_v V_v (the complex64 cycle of iterative refinement, or
+// the FP64 cycle with a unit right-hand side)
+template <typename R>
+__global__ __launch_bounds__(kRT) void kb_axpy_basis(const BKry* __restrict__ d) {
+  const BKry& q = d[blockIdx.y];
+  axpy_basis_body<R, double>(blockIdx.x, gridDim.x, reinterpret_cast<const cx<R>*>(q.V), q.ldv,
+                             q.y, q.st, q.x, q.n, &q.st->rnorm);
+}
+
+__global__ __launch_bounds__(kRT) void kb_narrow(const BKry* __restrict__ d) {
+  const BKry& q = d[blockIdx.y];
+  narrow_body(blockIdx.x, gridDim.x, q.src, q.dst, q.n);
+}
+
+__global__ void kb_gather(const BKry* __restrict__ d, int nsys, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsys) return;
+  out[2 * i] = d[i].st->rnorm;
+  out[2 * i + 1] = (double)d[i].st->j;
+}
+
+__global__ void kb_deactivate(const BKry* __restrict__ d, int nsys, const int* __restrict__ active) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsys && active[i] == 0) d[i].st->rnorm = 0.0;
 }
 
 // ── host side ───────────────────────────────────────────────────────────────
@@ -583,6 +753,99 @@ int krylov_read_state(Krylov& k, KState* out, hipStream_t s) {
   *out = *reinterpret_cast<KState*>(k.host);
   return kOk;
 }
+
+BKry bkry_of(const Krylov& k) {
+  BKry q{};
+  q.V = k.V;
+  q.ldv = k.ldv;
+  q.n = k.n;
+  q.nblk = k.nblk;
+  q.m = k.m;
+  q.ldh = k.ldh;
+  q.shifted = k.shifted ? 1 : 0;
+  q.st = k.st;
+  q.H = k.H;
+  q.cs = k.cs;
+  q.g = k.g;
+  q.hcur = k.hcur;
+  q.y = k.y;
+  q.tmp = k.tmp;
+  q.part = k.part;
+  q.hnext = k.hnext;
+  return q;
+}
+
+static dim3 bgrid(size_t gx, int nsys) { return dim3((unsigned)std::max<size_t>(gx, 1), nsys); }
+
+template <typename R>
+int bkrylov_step(const BKry* d, int nsys, size_t max_n, int max_nblk, int j, int orth_mode,
+                 hipStream_t s) {
+  const int nv = j + 1;
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL((kb_dots<R>), bgrid(max_nblk, nsys), dim3(kRT), 0, s, d, nv, pass);
+    hipLaunchKernelGGL(kb_reduce_dots, bgrid(1, nsys), dim3(kRT), 0, s, d, nv, pass, orth_mode);
+    hipLaunchKernelGGL((kb_update<R>), bgrid(max_nblk, nsys), dim3(kRT), 0, s, d, nv, pass);
+    if (orth_mode == kOrthCGS1) break;
+  }
+  if (orth_mode != kOrthCGS1) {
+    hipLaunchKernelGGL(kb_norm_givens, bgrid(1, nsys), dim3(kRT), 0, s, d);
+    hipLaunchKernelGGL((kb_scale_next<R>), bgrid(grid_for(max_n), nsys), dim3(kRT), 0, s, d, nv);
+  }
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename R>
+int bkrylov_norm(const BKry* d, int nsys, int max_nblk, hipStream_t s) {
+  hipLaunchKernelGGL((kb_norm2_part<R>), bgrid(max_nblk, nsys), dim3(kRT), 0, s, d);
+  hipLaunchKernelGGL(kb_norm_finish, bgrid(1, nsys), dim3(kRT), 0, s, d);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename RS, typename R>
+int bkrylov_cycle_init(const BKry* d, int nsys, size_t max_n, bool unit_rhs, hipStream_t s) {
+  hipLaunchKernelGGL((kb_cycle_init<RS, R>), bgrid(grid_for(max_n), nsys), dim3(kRT), 0, s, d,
+                     unit_rhs ? 1 : 0);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename R>
+int bkrylov_finish(const BKry* d, int nsys, size_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL(kb_backsub, bgrid(1, nsys), dim3(kRT), 0, s, d);
+  hipLaunchKernelGGL((kb_axpy_basis<R>), bgrid(grid_for(max_n), nsys), dim3(kRT), 0, s, d);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+int bkrylov_narrow(const BKry* d, int nsys, size_t max_n, hipStream_t s) {
+  hipLaunchKernelGGL(kb_narrow, bgrid(grid_for(max_n), nsys), dim3(kRT), 0, s, d);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+int bkrylov_gather(const BKry* d, int nsys, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(kb_gather, dim3((nsys + 63) / 64), dim3(64), 0, s, d, nsys, out);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+int bkrylov_deactivate(const BKry* d, int nsys, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(kb_deactivate, dim3((nsys + 63) / 64), dim3(64), 0, s, d, nsys, active);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template int bkrylov_step<double>(const BKry*, int, size_t, int, int, int, hipStream_t);
+template int bkrylov_step<float>(const BKry*, int, size_t, int, int, int, hipStream_t);
+template int bkrylov_norm<double>(const BKry*, int, int, hipStream_t);
+template int bkrylov_norm<float>(const BKry*, int, int, hipStream_t);
+template int bkrylov_cycle_init<double, float>(const BKry*, int, size_t, bool, hipStream_t);
+template int bkrylov_cycle_init<float, float>(const BKry*, int, size_t, bool, hipStream_t);
+template int bkrylov_cycle_init<double, double>(const BKry*, int, size_t, bool, hipStream_t);
+template int bkrylov_finish<float>(const BKry*, int, size_t, hipStream_t);
+template int bkrylov_finish<double>(const BKry*, int, size_t, hipStream_t);
 
 template int krylov_step<double>(Krylov&, int, int, hipStream_t);
 template int krylov_step<float>(Krylov&, int, int, hipStream_t);

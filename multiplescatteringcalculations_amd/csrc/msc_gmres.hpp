@@ -60,6 +60,45 @@ struct Krylov {
   int ensure_z();
 };
 
+// One system of a batched launch (device copy, filled by the host from its
+// Krylov): the batched kernels take an array of these, system = blockIdx.y,
+// and every system runs the single-system kernel body on its own workspace.
+// `r` is the vector a norm or a cycle start reads (precision of the call),
+// `x` the FP64 solution a cycle's update adds to, `src` / `dst` the
+// complex128 -> complex64 narrowing pair; `tol_abs` the system's cycle
+// tolerance.
+struct BKry {
+  void* V;
+  size_t ldv, n;
+  int nblk, m, ldh, shifted;
+  KState* st;
+  double2 *H, *cs, *g, *hcur, *y, *tmp, *part;
+  double* hnext;
+  const void* r;
+  double2* x;
+  const double2* src;
+  float2* dst;
+  double tol_abs;
+};
+BKry bkry_of(const Krylov& k);
+// Batched counterparts of the calls below over `nsys` systems (max_n / max_nblk:
+// the largest vector length / chunk count among them).
+template <typename R>
+int bkrylov_step(const BKry* d, int nsys, size_t max_n, int max_nblk, int j, int orth_mode,
+                 hipStream_t s);
+template <typename R>
+int bkrylov_norm(const BKry* d, int nsys, int max_nblk, hipStream_t s);
+template <typename RS, typename R>
+int bkrylov_cycle_init(const BKry* d, int nsys, size_t max_n, bool unit_rhs, hipStream_t s);
+template <typename R>
+int bkrylov_finish(const BKry* d, int nsys, size_t max_n, hipStream_t s);
+int bkrylov_narrow(const BKry* d, int nsys, size_t max_n, hipStream_t s);
+// out[2 i] = ||r|| of system i's cycle (KState::rnorm), out[2 i + 1] = its
+// completed steps (KState::j)
+int bkrylov_gather(const BKry* d, int nsys, double* out, hipStream_t s);
+// systems with active[i] == 0 start their next cycle done (rnorm = 0)
+int bkrylov_deactivate(const BKry* d, int nsys, const int* active, hipStream_t s);
+
 template <typename R>
 int krylov_step(Krylov& k, int j, int orth_mode, hipStream_t s);
 template <typename R>

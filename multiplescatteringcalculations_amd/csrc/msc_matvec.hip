@@ -153,8 +153,8 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
 // ── pass 1: T0 gather + zero pad + forward FFT along y ─────────────────────
 // GEN: voxel v applies the general 9x9 matrix tgen[tidx[v]] (row-major)
 // instead of the 4-parameter cubic T0.
-template <typename R, int P, int B, bool GEN = false>
-__global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
+template <typename R, int P, int B, bool GEN>
+__device__ __forceinline__ void fwd_y_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ x,
                                               const cx<R>* __restrict__ tp,
                                               const int* __restrict__ tidx,
                                               const cx<R>* __restrict__ tgen,
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
   constexpr int LD = lds_ld<NTR>::value;
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
-  const int row0 = blockIdx.x * B;
+  const int row0 = bx_ * B;
   staged<cx<R>>(
       B * P * 9,
       [&](int w) {
@@ -245,10 +245,21 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
     if (row < nrows) buf1[(size_t)row * P * 9 + rem] = s[y * LD + b * 9 + c];
   }
 }
+template <typename R, int P, int B, bool GEN = false>
+__global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
+                                              const cx<R>* __restrict__ tp,
+                                              const int* __restrict__ tidx,
+                                              const cx<R>* __restrict__ tgen,
+                                              cx<R>* __restrict__ buf1, int M, int nrows,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  fwd_y_body<R, P, B, GEN>(blockIdx.x, gridDim.x, x, tp, tidx, tgen, buf1, M, nrows, tw, done);
+}
+
 
 // ── pass 2: forward FFT along x of the M non-zero rows ─────────────────────
 template <typename R, int P, int Q>
-__global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
+__device__ __forceinline__ void fwd_x_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ buf1,
                                               cx<R>* __restrict__ buf2, int M,
                                               const cx<R>* __restrict__ tw,
                                               const int* __restrict__ done) {
@@ -257,7 +268,7 @@ __global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
   constexpr int nck = P / Q;
-  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int lb = xcd_block((int)bx_, (int)gx_);
   const int z = lb / nck;
   const int ky0 = (lb - z * nck) * Q;
   staged<cx<R>>(
@@ -281,10 +292,18 @@ __global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
     buf2[(((size_t)z * P + kx) * P + ky0) * 9 + e] = s[kx * LD + e];
   }
 }
+template <typename R, int P, int Q>
+__global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
+                                              cx<R>* __restrict__ buf2, int M,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  fwd_x_body<R, P, Q>(blockIdx.x, gridDim.x, buf1, buf2, M, tw, done);
+}
+
 
 // ── pass 3: z FFT, spectral 9x9 multiply, inverse z FFT (in place) ─────────
 template <typename R, int PZ, int Q>
-__global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
+__device__ __forceinline__ void z_mult_body(unsigned bx_, unsigned gx_, cx<R>* __restrict__ buf2,
                                                const cx<R>* __restrict__ table, int Nz,
                                                int Pxy, const cx<R>* __restrict__ tw,
                                                const int* __restrict__ zmap,
@@ -295,7 +314,7 @@ __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
   __shared__ cx<R> s[PZ * LD];
   if (done != nullptr && *done) return;
   const int nck = Pxy / Q;
-  const int tile = zmap[blockIdx.x];
+  const int tile = zmap[bx_];
   const int kx = tile / nck;
   const int ky0 = (tile - kx * nck) * Q;
   const size_t plane = (size_t)Pxy * Pxy * 9;
@@ -344,6 +363,15 @@ __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
     buf2[(size_t)z * plane + col0 + e] = s[z * LD + e];
   }
 }
+template <typename R, int PZ, int Q>
+__global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
+                                               const cx<R>* __restrict__ table, int Nz,
+                                               int Pxy, const cx<R>* __restrict__ tw,
+                                               const int* __restrict__ zmap,
+                                               const int* __restrict__ done) {
+  z_mult_body<R, PZ, Q>(blockIdx.x, gridDim.x, buf2, table, Nz, Pxy, tw, zmap, done);
+}
+
 
 // Direct-to-LDS copy of the Nz input rows of one z-pass tile,
 // dst[z * NTR + e] = src[z * plane + e] (z < Nz, e < NTR), in 16-byte chunks
@@ -454,7 +482,7 @@ __global__ __launch_bounds__(kNT) void k_z_mult_dma(cx<R>* __restrict__ buf2,
 
 // ── pass 4: inverse FFT along x, keep x < M ────────────────────────────────
 template <typename R, int P, int Q>
-__global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
+__device__ __forceinline__ void inv_x_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ buf2,
                                               cx<R>* __restrict__ buf1, int M,
                                               const cx<R>* __restrict__ tw,
                                               const int* __restrict__ done) {
@@ -463,7 +491,7 @@ __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
   constexpr int nck = P / Q;
-  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int lb = xcd_block((int)bx_, (int)gx_);
   const int z = lb / nck;
   const int ky0 = (lb - z * nck) * Q;
   staged<cx<R>>(
@@ -484,12 +512,20 @@ __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
     buf1[(((size_t)z * M + xr) * P + ky0) * 9 + e] = s[xr * LD + e];
   }
 }
+template <typename R, int P, int Q>
+__global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
+                                              cx<R>* __restrict__ buf1, int M,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  inv_x_body<R, P, Q>(blockIdx.x, gridDim.x, buf2, buf1, M, tw, done);
+}
+
 
 // ── pass 5: inverse FFT along y, keep y < M and form the output ────────────
 // MODE 0: out = x - acc (the operator), 1: out = b - x + acc (residual),
 // 2: out = acc (the scattered part G0 T0 x).
 template <typename R, int P, int B, int MODE>
-__global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
+__device__ __forceinline__ void inv_y_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ buf1,
                                               const cx<R>* __restrict__ x,
                                               const cx<R>* __restrict__ bvec,
                                               cx<R>* __restrict__ out, int M, int nrows,
@@ -499,7 +535,7 @@ __global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
   constexpr int LD = lds_ld<NTR>::value;
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
-  const int row0 = blockIdx.x * B;
+  const int row0 = bx_ * B;
   staged<cx<R>>(
       B * P * 9,
       [&](int w) {
@@ -543,6 +579,16 @@ __global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
         else out[g] = csub<R>(xv, acc);
       });
 }
+template <typename R, int P, int B, int MODE>
+__global__ __launch_bounds__(kNT) void k_inv_y(const cx<R>* __restrict__ buf1,
+                                              const cx<R>* __restrict__ x,
+                                              const cx<R>* __restrict__ bvec,
+                                              cx<R>* __restrict__ out, int M, int nrows,
+                                              const cx<R>* __restrict__ tw,
+                                              const int* __restrict__ done) {
+  inv_y_body<R, P, B, MODE>(blockIdx.x, gridDim.x, buf1, x, bvec, out, M, nrows, tw, done);
+}
+
 
 // ── volume-averaged nearest-neighbour correction (after pass 5) ──────────────
 // acc_v = sum over the 26 offsets R of nn[R] tau(v - R), tau = T0 x, with
@@ -980,6 +1026,192 @@ static int apply(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, int mod
   MSC_LAUNCH_CHECK();
   return kOk;
 }
+
+// ── batched operator: instance q = d[blockIdx.y] ──────────────────────────
+template <typename R, int P, int B>
+__global__ __launch_bounds__(kNT) void kb_fwd_y(const BPass<R>* __restrict__ d, int j,
+                                               const cx<R>* __restrict__ tw) {
+  const BPass<R>& q = d[blockIdx.y];
+  const int nrows = q.Nz * q.M;
+  if ((int)blockIdx.x * B >= nrows) return;
+  fwd_y_body<R, P, B, false>(blockIdx.x, 0, q.in + (size_t)j * q.in_step, q.tp, nullptr,
+                             nullptr, q.buf1, q.M, nrows, tw, q.done);
+}
+
+template <typename R, int P, int Q>
+__global__ __launch_bounds__(kNT) void kb_fwd_x(const BPass<R>* __restrict__ d,
+                                               const cx<R>* __restrict__ tw) {
+  const BPass<R>& q = d[blockIdx.y];
+  const int nblk = q.Nz * (P / Q);
+  if ((int)blockIdx.x >= nblk) return;
+  fwd_x_body<R, P, Q>(blockIdx.x, nblk, q.buf1, q.buf2, q.M, tw, q.done);
+}
+
+template <typename R, int PZ, int Q>
+__global__ __launch_bounds__(kNT) void kb_z_mult(const BPass<R>* __restrict__ d, int Pxy,
+                                                const cx<R>* __restrict__ tw,
+                                                const int* __restrict__ zmap) {
+  const BPass<R>& q = d[blockIdx.y];
+  z_mult_body<R, PZ, Q>(blockIdx.x, gridDim.x, q.buf2, q.table, q.Nz, Pxy, tw, zmap, q.done);
+}
+
+template <typename R, int P, int Q>
+__global__ __launch_bounds__(kNT) void kb_inv_x(const BPass<R>* __restrict__ d,
+                                               const cx<R>* __restrict__ tw) {
+  const BPass<R>& q = d[blockIdx.y];
+  const int nblk = q.Nz * (P / Q);
+  if ((int)blockIdx.x >= nblk) return;
+  inv_x_body<R, P, Q>(blockIdx.x, nblk, q.buf2, q.buf1, q.M, tw, q.done);
+}
+
+template <typename R, int P, int B, int MODE>
+__global__ __launch_bounds__(kNT) void kb_inv_y(const BPass<R>* __restrict__ d, int j,
+                                               const cx<R>* __restrict__ tw) {
+  const BPass<R>& q = d[blockIdx.y];
+  const int nrows = q.Nz * q.M;
+  if ((int)blockIdx.x * B >= nrows) return;
+  inv_y_body<R, P, B, MODE>(blockIdx.x, 0, q.buf1, q.in + (size_t)j * q.in_step, q.bvec,
+                            q.out + (size_t)j * q.out_step, q.M, nrows, tw, q.done);
+}
+
+// blocks of a batched x pass: a multiple of 8, so that slot b of every
+// instance runs on XCD b % 8 as in the single-instance launch
+static unsigned round8(int v) { return (unsigned)((v + 7) / 8 * 8); }
+
+template <typename R, int P, int Q>
+static bool batch_x_q(bool fwd, const BPass<R>* d, int nsys, int max_Nz, const cx<R>* tw,
+                      hipStream_t st) {
+  if constexpr (P >= Q && lds_fits<R, P, 9, Q>()) {
+    const dim3 g(round8(max_Nz * (P / Q)), nsys);
+    if (fwd) hipLaunchKernelGGL((kb_fwd_x<R, P, Q>), g, dim3(kNT), 0, st, d, tw);
+    else hipLaunchKernelGGL((kb_inv_x<R, P, Q>), g, dim3(kNT), 0, st, d, tw);
+    return true;
+  }
+  return false;
+}
+
+template <typename R, int P>
+static int batch_xy(const Plan& ref, const BPass<R>* d, int nsys, int max_M, int max_Nz, int mode,
+                    int j, bool forward, hipStream_t st) {
+  const int pr = prec_of<R>();
+  const cx<R>* tw = PlanBufs<R>::tw(ref);
+  const int rows = max_Nz * max_M;
+  bool two = false;
+  if constexpr (lds_fits<R, P, 9, 2>()) two = ref.tile[pr][forward ? 0 : 4] == 2;
+  const dim3 gy(two ? (rows + 1) / 2 : rows, nsys);
+  const int q = ref.tile[pr][forward ? 1 : 3];
+  if (forward) {
+    if constexpr (lds_fits<R, P, 9, 2>()) {
+      if (two) hipLaunchKernelGGL((kb_fwd_y<R, P, 2>), gy, dim3(kNT), 0, st, d, j, tw);
+    }
+    if (!two) hipLaunchKernelGGL((kb_fwd_y<R, P, 1>), gy, dim3(kNT), 0, st, d, j, tw);
+  }
+  bool done = false;
+  if (q >= 4) done = batch_x_q<R, P, 4>(forward, d, nsys, max_Nz, tw, st);
+  if (!done && q >= 2) done = batch_x_q<R, P, 2>(forward, d, nsys, max_Nz, tw, st);
+  if (!done) batch_x_q<R, P, 1>(forward, d, nsys, max_Nz, tw, st);
+  if (!forward) {
+    if constexpr (lds_fits<R, P, 9, 2>()) {
+      if (two) {
+        if (mode == 1) hipLaunchKernelGGL((kb_inv_y<R, P, 2, 1>), gy, dim3(kNT), 0, st, d, j, tw);
+        else hipLaunchKernelGGL((kb_inv_y<R, P, 2, 2>), gy, dim3(kNT), 0, st, d, j, tw);
+      }
+    }
+    if (!two) {
+      if (mode == 1) hipLaunchKernelGGL((kb_inv_y<R, P, 1, 1>), gy, dim3(kNT), 0, st, d, j, tw);
+      else hipLaunchKernelGGL((kb_inv_y<R, P, 1, 2>), gy, dim3(kNT), 0, st, d, j, tw);
+    }
+  }
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename R, int PZ>
+static int batch_z(const Plan& ref, const BPass<R>* d, int nsys, hipStream_t st) {
+  const cx<R>* tw = PlanBufs<R>::tw(ref);
+  int q = ref.tile[prec_of<R>()][2] & (kZDma - 1);
+  while (q > 1 && ref.Pxy % q != 0) q >>= 1;
+  const int P = ref.Pxy;
+  switch (q) {
+    case 16:
+    case 8:
+      if constexpr (lds_fits<R, PZ, 9, 8>()) {
+        hipLaunchKernelGGL((kb_z_mult<R, PZ, 8>), dim3(P * (P / 8), nsys), dim3(kNT), 0, st, d, P,
+                           tw, ref.zmap + zmap_offset(P, 8));
+        break;
+      }
+      [[fallthrough]];
+    case 4:
+      if constexpr (lds_fits<R, PZ, 9, 4>()) {
+        if (P % 4 == 0) {
+          hipLaunchKernelGGL((kb_z_mult<R, PZ, 4>), dim3(P * (P / 4), nsys), dim3(kNT), 0, st, d,
+                             P, tw, ref.zmap + zmap_offset(P, 4));
+          break;
+        }
+      }
+      [[fallthrough]];
+    case 2:
+      if constexpr (lds_fits<R, PZ, 9, 2>()) {
+        if (P % 2 == 0) {
+          hipLaunchKernelGGL((kb_z_mult<R, PZ, 2>), dim3(P * (P / 2), nsys), dim3(kNT), 0, st, d,
+                             P, tw, ref.zmap + zmap_offset(P, 2));
+          break;
+        }
+      }
+      [[fallthrough]];
+    default:
+      hipLaunchKernelGGL((kb_z_mult<R, PZ, 1>), dim3(P * P, nsys), dim3(kNT), 0, st, d, P, tw,
+                         ref.zmap);
+  }
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
+template <typename R>
+int apply_batch(const Plan& ref, const BPass<R>* d, int nsys, int max_M, int max_Nz, int mode,
+                int j, hipStream_t st) {
+  if (nsys < 1) return kOk;
+  if (mode != 1 && mode != 2) {
+    set_error("apply_batch: mode must be 1 (residual) or 2 (scattered)");
+    return kInvalid;
+  }
+  const Plan& p = ref;
+  int rc = kUnsupported;
+  auto xy = [&](bool fwd) -> int {
+    switch (p.Pxy) {
+      case 2: return batch_xy<R, 2>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 4: return batch_xy<R, 4>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 8: return batch_xy<R, 8>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 16: return batch_xy<R, 16>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 32: return batch_xy<R, 32>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 64: return batch_xy<R, 64>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 128: return batch_xy<R, 128>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 256: return batch_xy<R, 256>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 512: return batch_xy<R, 512>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      default: set_error("unsupported lateral FFT length"); return kUnsupported;
+    }
+  };
+  rc = xy(true);
+  if (rc) return rc;
+  switch (p.Pz) {
+    case 2: rc = batch_z<R, 2>(p, d, nsys, st); break;
+    case 4: rc = batch_z<R, 4>(p, d, nsys, st); break;
+    case 8: rc = batch_z<R, 8>(p, d, nsys, st); break;
+    case 16: rc = batch_z<R, 16>(p, d, nsys, st); break;
+    case 32: rc = batch_z<R, 32>(p, d, nsys, st); break;
+    case 64: rc = batch_z<R, 64>(p, d, nsys, st); break;
+    case 128: rc = batch_z<R, 128>(p, d, nsys, st); break;
+    case 256: rc = batch_z<R, 256>(p, d, nsys, st); break;
+    default: set_error("unsupported vertical FFT length"); return kUnsupported;
+  }
+  if (rc) return rc;
+  return xy(false);
+}
+
+template int apply_batch<double>(const Plan&, const BPass<double>*, int, int, int, int, int,
+                                 hipStream_t);
+template int apply_batch<float>(const Plan&, const BPass<float>*, int, int, int, int, int,
+                                hipStream_t);
 
 template <typename R>
 int matvec(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done) {

@@ -131,6 +131,32 @@ int build_table(Plan& p, double omega, double alpha, double beta, double rho, hi
 // volume_averaged: set the nearest-neighbour correction for the plan's table
 // (after build_table, same frequency and medium); otherwise clear it.
 int set_nn_correction(Plan& p, bool volume_averaged, int n_orders, hipStream_t st);
+// One instance of a batched operator launch (device copy): its T0, FFT
+// buffers and G0 table, the input in + j * in_step and output out + j *
+// out_step (j: the Arnoldi step of the launch, 0 for a residual), the
+// residual's right-hand side and the done flag of its Krylov cycle.
+template <typename R>
+struct BPass {
+  const cx<R>* tp;
+  cx<R>* buf1;
+  cx<R>* buf2;
+  const cx<R>* table;
+  const cx<R>* in;
+  cx<R>* out;
+  const cx<R>* bvec;
+  size_t in_step, out_step;
+  const int* done;
+  int M, Nz;
+};
+// The operator on nsys instances of `ref`'s padded grid (Pz, Pxy, periodic)
+// in one launch per pass, with `ref`'s tiles (the persistent z form maps to
+// its one-tile-per-workgroup width): mode 1 out = bvec - in + acc (the
+// residual), 2 out = acc (G0 T0 in).  Cubic T0 only, no near-neighbour
+// correction.  max_M / max_Nz: the largest instance.
+template <typename R>
+int apply_batch(const Plan& ref, const BPass<R>* d, int nsys, int max_M, int max_Nz, int mode,
+                int j, hipStream_t st);
+
 // Offset of the tile order for width Q (a power of two <= 16) in Plan::zmap.
 inline size_t zmap_offset(int Pxy, int Q) {
   size_t off = 0;

@@ -382,6 +382,47 @@ class SlabPlan:
 # ═══════════════════════════════════════════════════════════════
 
 
+def gmres_batched(plans: list["SlabPlan"], bs, x0: str = "born", *, tol: float = 1e-6,
+                  max_iter: int = 500, restart: int = 30, precision: str = "fp64",
+                  inner_tol: float = 1e-4, orth: str = "adaptive"):
+    """``SlabPlan.gmres`` on several plans of one padded grid as one batch
+    (msc_gmres_batched): each operator pass and Arnoldi kernel is a single
+    launch for all of them.  ``bs`` holds one right-hand side per plan;
+    returns ([x_i], [stats_i])."""
+    if len(plans) != len(bs) or not plans:
+        raise ValueError("one right-hand side per plan, at least one plan")
+    dev = plans[0].device
+    if any(torch.device(p.device) != torch.device(dev) for p in plans):
+        raise ValueError("the plans of a batch must be on one device")
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', got '{precision}'")
+    orth_code = {"adaptive": 0, "cgs": 1, "cgs2": 2}.get(orth)
+    if orth_code is None:
+        raise ValueError(f"orth must be 'adaptive', 'cgs' or 'cgs2', got '{orth}'")
+    b_t = [_vec_c128(b, p.n, dev) for p, b in zip(plans, bs)]
+    if isinstance(x0, str):
+        mode_x0 = {"zero": 0, "born": 1}.get(x0)
+        if mode_x0 is None:
+            raise ValueError(f"Invalid initial_guess '{x0}'. Must be 'born' or 'zero'.")
+        xs = [torch.empty_like(b) for b in b_t]
+    else:
+        mode_x0 = 2
+        xs = [_vec_c128(v, p.n, dev).clone() for p, v in zip(plans, x0)]
+    k = len(plans)
+    opts = _lib.GmresOpts(max(1, int(restart)), int(max_iter), float(tol), mode_x0,
+                          1 if precision == "mixed" else 0, float(inner_tol), orth_code, None)
+    sts = (_lib.GmresStats * k)()
+    hp = (ctypes.c_void_p * k)(*[p._h.value for p in plans])
+    hb = (ctypes.c_void_p * k)(*[b.data_ptr() for b in b_t])
+    hx = (ctypes.c_void_p * k)(*[x.data_ptr() for x in xs])
+    lib = plans[0].lib
+    with torch.cuda.device(dev):
+        _lib.check(lib.msc_gmres_batched(hp, hb, hx, k, ctypes.byref(opts), sts,
+                                         plans[0].stream))
+    stats = [{f: getattr(s, f) for f, _ in _lib.GmresStats._fields_} for s in sts]
+    return xs, stats
+
+
 def tparams_to_tmatrices(tp: NDArray) -> NDArray:
     """Expand (..., 4) T0 parameters to (..., 9, 9) block-diagonal matrices."""
     tp = np.asarray(tp)
