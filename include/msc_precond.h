@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-/* Create a network of hidden width 16, 32, 48 or 64 (parameters zero:
+/* Create a network of hidden width 16, 32 or 48 (parameters zero:
  * P(r) = r until parameters are set) on the current device. */
 int msc_net_create(msc_net** net, int hidden);
 

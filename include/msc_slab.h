@@ -132,6 +132,18 @@ int msc_resonance_tmatrices(double omega, double a, double alpha, double beta, d
                             double* pivot_ratio, int* neumann_iters, void* stream);
 
 /*
+ * The internal Foldy-Lax matrix A = I - P (I x T_loc) (9N x 9N, N = n_sub^3,
+ * complex128, column-major: A[c * 9N + r]) of one cube with contrasts
+ * (dlambda, dmu, drho) (host values), written to the device buffer A_out:
+ * the matrix whose 2-norm condition number _solve_coupled
+ * (cubic_scattering/resonance_tmatrix.py:770-795) reports and warns on.
+ * Synchronises `stream` before returning.
+ */
+int msc_resonance_system(double omega, double a, double alpha, double beta, double rho,
+                         int n_sub, double dlambda, double dmu, double drho, void* A_out,
+                         void* stream);
+
+/*
  * msc_resonance_tmatrices without any device allocation.  No counterpart in
  * the reference: this entry point is for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -214,6 +226,26 @@ int msc_plan_use_graphs(msc_plan* plan, int enable);
  * failing instance's status.  opts->precond must be NULL.
  */
 int msc_sweep(msc_instance* instances, int count, const msc_gmres_opts* opts, int workers);
+/*
+ * msc_sweep with the instances of one padded FFT grid solved as batches
+ * (msc_gmres_batched): every operator pass and Arnoldi kernel of a GMRES step
+ * is one launch for a batch of slabs of that grid (as many as fill the device,
+ * at most `max_batch`), whatever their sizes, frequencies and incidences.
+ * Replaces the same per-frequency loop of compute_ocean_bottom_reflection
+ * (cubic_scattering/ocean_bottom.py:290-307).  Per grid a pool of plans at
+ * the grid's largest slab is re-shaped per instance; pools stay in a bounded
+ * process-wide cache (40% of the device memory) for later sweeps until
+ * msc_sweep_release.  Solves and source sums run on `stream`; the next batch's
+ * set-up (G0, T0, incident field) runs meanwhile on a helper thread and
+ * stream of the library's own.  Instances with the volume-averaged correction
+ * are solved one at a time.  Blocks until all instances are done; statuses
+ * and return value as msc_sweep.  opts->precond must be NULL.
+ */
+int msc_sweep_batched(msc_instance* instances, int count, const msc_gmres_opts* opts,
+                      int max_batch, void* stream);
+/* Free the plan pools msc_sweep_batched keeps between calls.  No reference
+ * counterpart. */
+int msc_sweep_release(void);
 /* out[0..5] = M, N_z, Pz, Pxy, NUV (table columns), NC (table depth). */
 int msc_plan_info(const msc_plan* plan, long long* out6);
 

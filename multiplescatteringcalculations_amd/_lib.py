@@ -34,6 +34,10 @@ SIGNATURES = {
     "msc_plan_use_graphs": (c_int, [c_void_p, c_int]),
     "msc_plan_copy_g0": (c_int, [c_void_p, c_void_p, c_void_p]),
     "msc_sweep": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "msc_sweep_batched": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "msc_sweep_release": (c_int, []),
+    "msc_resonance_system": (c_int, [c_double, c_double, c_double, c_double, c_double, c_int,
+                                     c_double, c_double, c_double, c_void_p, c_void_p]),
     "msc_resonance_tmatrices": (c_int, [c_double, c_double, c_double, c_double, c_double, c_int,
                                         c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                         c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p,

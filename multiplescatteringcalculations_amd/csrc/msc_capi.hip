@@ -234,12 +234,15 @@ void free_plan_buffers(Plan& p) {
   if (p.tp128) (void)hipFree(p.tp128);
   if (p.tp64) (void)hipFree(p.tp64);
   if (p.tmax) (void)hipFree(p.tmax);
-  if (p.work) (void)hipFree(p.work);
-  if (p.work2) (void)hipFree(p.work2);
+  if (p.work && p.owns_work) (void)hipFree(p.work);
+  if (p.work2 && p.owns_work) (void)hipFree(p.work2);
   if (p.nn128) (void)hipFree(p.nn128);
   if (p.nn64) (void)hipFree(p.nn64);
   if (p.nn_stage) (void)hipFree(p.nn_stage);
   if (p.zmap) (void)hipFree(p.zmap);
+  if (p.sums_part) (void)hipFree(p.sums_part);
+  p.sums_part = nullptr;
+  p.sums_cap = 0;
   if (p.tidx) (void)hipFree(p.tidx);
   if (p.tgen128) (void)hipFree(p.tgen128);
   if (p.tgen64) (void)hipFree(p.tgen64);
@@ -729,11 +732,25 @@ int solve_batched(msc_plan* const* pls, const double2* const* b, double2* const*
                off_gat = (off_fin + sizeof(int) * ns + 15) / 16 * 16,
                total = off_gat + sizeof(double) * 2 * ns;
   unsigned char* dev = nullptr;
+  unsigned char* hst = nullptr;  // pinned mirror of dev: one copy per upload
+  hipEvent_t ev_up = nullptr;    // the last upload's copy (hst is reused after it)
   MSC_HIP_CHECK(hipMalloc(&dev, total));
   struct Free {
     unsigned char* p;
-    ~Free() { if (p) (void)hipFree(p); }
-  } free_dev{dev};
+    unsigned char*& h;
+    hipEvent_t& e;
+    ~Free() {
+      if (p) (void)hipFree(p);
+      if (e) {
+        (void)hipEventSynchronize(e);
+        (void)hipEventDestroy(e);
+      }
+      if (h) (void)hipHostFree(h);
+    }
+  } free_dev{dev, hst, ev_up};
+  MSC_HIP_CHECK(hipHostMalloc(&hst, total, hipHostMallocDefault));
+  std::memset(hst, 0, total);
+  MSC_HIP_CHECK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
   BKry* d128 = reinterpret_cast<BKry*>(dev);
   BKry* d64 = reinterpret_cast<BKry*>(dev + off_k64);
   BPass<double>* dres = reinterpret_cast<BPass<double>*>(dev + off_res);
@@ -792,22 +809,29 @@ int solve_batched(msc_plan* const* pls, const double2* const* b, double2* const*
     }
   }
   std::vector<int> act(ns, 1), fin(ns, 0), stalled(ns, 0), broke(ns, 0);
-  std::vector<double> gat(2 * ns), bnorm(ns), prev(ns, INFINITY);
+  std::vector<double> bnorm(ns), prev(ns, INFINITY);
+  // the host descriptors go to the device as one copy from the pinned mirror
+  // (per-array copies from pageable memory cost tens of microseconds each)
   auto upload = [&]() -> int {
-    MSC_HIP_CHECK(hipMemcpyAsync(d128, h128.data(), bytes_kry, hipMemcpyHostToDevice, st));
-    if (mixed) MSC_HIP_CHECK(hipMemcpyAsync(d64, h64.data(), bytes_kry, hipMemcpyHostToDevice, st));
-    MSC_HIP_CHECK(hipMemcpyAsync(dres, hres.data(), bytes_p128, hipMemcpyHostToDevice, st));
-    MSC_HIP_CHECK(hipMemcpyAsync(ds128, hstep128.data(), bytes_p128, hipMemcpyHostToDevice, st));
-    MSC_HIP_CHECK(hipMemcpyAsync(dcheap, hcheap.data(), bytes_p64, hipMemcpyHostToDevice, st));
-    MSC_HIP_CHECK(hipMemcpyAsync(ds64, hstep64.data(), bytes_p64, hipMemcpyHostToDevice, st));
-    MSC_HIP_CHECK(hipMemcpyAsync(dact, act.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
-    MSC_HIP_CHECK(hipMemcpyAsync(dfin, fin.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipEventSynchronize(ev_up));
+    std::memcpy(hst, h128.data(), bytes_kry);
+    if (mixed) std::memcpy(hst + off_k64, h64.data(), bytes_kry);
+    std::memcpy(hst + off_res, hres.data(), bytes_p128);
+    std::memcpy(hst + off_s128, hstep128.data(), bytes_p128);
+    std::memcpy(hst + off_cheap, hcheap.data(), bytes_p64);
+    std::memcpy(hst + off_s64, hstep64.data(), bytes_p64);
+    std::memcpy(hst + off_act, act.data(), sizeof(int) * ns);
+    std::memcpy(hst + off_fin, fin.data(), sizeof(int) * ns);
+    MSC_HIP_CHECK(hipMemcpyAsync(dev, hst, off_gat, hipMemcpyHostToDevice, st));
+    MSC_HIP_CHECK(hipEventRecord(ev_up, st));
     return kOk;
   };
+  // the gathered norms land in the pinned mirror's last 2 ns doubles
+  const double* gat = reinterpret_cast<const double*>(hst + off_gat);
   auto gather = [&](const BKry* d) -> int {
     int r2 = bkrylov_gather(d, ns, dgat, st);
     if (r2) return r2;
-    MSC_HIP_CHECK(hipMemcpyAsync(gat.data(), dgat, sizeof(double) * 2 * ns,
+    MSC_HIP_CHECK(hipMemcpyAsync(hst + off_gat, dgat, sizeof(double) * 2 * ns,
                                  hipMemcpyDeviceToHost, st));
     MSC_HIP_CHECK(hipStreamSynchronize(st));
     return kOk;
@@ -980,6 +1004,41 @@ int solve_batched(msc_plan* const* pls, const double2* const* b, double2* const*
 }
 
 }  // namespace
+
+int msc::plan_reserve(msc_plan* pl, int m, bool mixed) {
+  MSC_REQUIRE(pl != nullptr && m >= 1, "plan_reserve: NULL plan or restart < 1");
+  const size_t cap = pl->p.cap_n;
+  int rc = mixed ? pl->k128.ensure_state(cap, m, sizeof(double2))
+                 : pl->k128.ensure(cap, m, sizeof(double2));
+  if (rc) return rc;
+  if (mixed) {
+    rc = pl->k64.ensure(cap, m, sizeof(float2));
+    if (rc) return rc;
+    if (pl->b64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->b64, cap * sizeof(float2)));
+    if (pl->r64 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r64, cap * sizeof(float2)));
+  }
+  if (pl->r128 == nullptr) MSC_HIP_CHECK(hipMalloc(&pl->r128, cap * sizeof(double2)));
+  return kOk;
+}
+
+int msc::plan_share_build_work(msc_plan* dst, msc_plan* src) {
+  MSC_REQUIRE(dst != nullptr && src != nullptr, "plan_share_build_work: NULL plan");
+  Plan& d = dst->p;
+  Plan& s = src->p;
+  MSC_REQUIRE(d.Pz == s.Pz && d.Pxy == s.Pxy && d.device == s.device,
+              "plans sharing a G0-build workspace need one padded grid and device");
+  if (dst == src) return kOk;
+  int rc = ensure_build_work(s);
+  if (rc) return rc;
+  if (d.owns_work) {
+    if (d.work) (void)hipFree(d.work);
+    if (d.work2) (void)hipFree(d.work2);
+  }
+  d.work = s.work;
+  d.work2 = s.work2;
+  d.owns_work = false;
+  return kOk;
+}
 
 // ── exported C ABI ───────────────────────────────────────────────────────────
 

@@ -1270,8 +1270,7 @@ int alloc_params(Net& net) {
 }  // namespace
 
 int net_init(Net& net, int hidden) {
-  MSC_REQUIRE(hidden == 16 || hidden == 32 || hidden == 48 || hidden == 64,
-              "hidden width must be 16, 32, 48 or 64");
+  MSC_REQUIRE(hidden == 16 || hidden == 32 || hidden == 48, "hidden width must be 16, 32 or 48");
   net.arch = kArchLocal;
   net.hidden = hidden;
   net.nlayers = 0;

@@ -68,8 +68,12 @@ struct Plan {
   int* flag = nullptr;         // device scratch flag (index validation)
   void* buf1 = nullptr;  // (Nz, M, Pxy, 9) complex128 bytes
   void* buf2 = nullptr;  // (Nz, Pxy, Pxy, 9) complex128 bytes
-  void* work = nullptr;   // G0 table build: (Pz, Pxy, Pxy, 45) complex128, kept for rebuilds
-  void* work2 = nullptr;  // its pruned half: (Pz, Pxy, Pxy/2 + 1, 45) complex128
+  // G0 table build: two octant workspaces (NC, Pxy/2 + 1, Pxy/2 + 1, 45)
+  // complex128, kept for rebuilds; plans of one grid whose builds run on one
+  // stream may share them (owns_work false on all but one)
+  void* work = nullptr;
+  void* work2 = nullptr;
+  bool owns_work = true;
   // volume-averaged nearest-neighbour correction: for the 26 offsets R in
   // {-1,0,1}^3 \ 0 (lexicographic (dz, dx, dy)), the 9x9 difference between the
   // volume-averaged block and the point propagator the table holds, row-major
@@ -82,6 +86,9 @@ struct Plan {
   // rows kx and Pxy - kx (which read the same table entries) on one XCD, with
   // the mirrored ky chunks next to each other in time.
   int* zmap = nullptr;
+  // per-workgroup partial source sums (source_sums), allocated on first use
+  double2* sums_part = nullptr;
+  size_t sums_cap = 0;
   double2* tw128 = nullptr;
   float2* tw64 = nullptr;
 
@@ -165,4 +172,16 @@ inline size_t zmap_offset(int Pxy, int Q) {
 }
 int build_zmap(Plan& p);
 
+}  // namespace msc
+
+struct msc_plan;
+
+namespace msc {
+// Allocate a plan's GMRES workspaces for its capacity (restart m; the
+// complex64 basis and vectors too when `mixed`), so that solves of any slab
+// the plan can be re-shaped to reuse them.
+int plan_reserve(msc_plan* pl, int m, bool mixed);
+// Let `dst` use the G0-build workspaces of `src` (same padded grid; their
+// builds must run on one stream, one after another).
+int plan_share_build_work(msc_plan* dst, msc_plan* src);
 }  // namespace msc

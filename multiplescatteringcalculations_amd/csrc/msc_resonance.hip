@@ -700,6 +700,64 @@ extern "C" int msc_resonance_tmatrices(double omega, double a, double alpha, dou
   return kOk;
 }
 
+extern "C" int msc_resonance_system(double omega, double a, double alpha, double beta,
+                                    double rho, int n_sub, double dlambda, double dmu,
+                                    double drho, void* A_out, void* stream) {
+  set_error("");
+  MSC_REQUIRE(omega > 0.0 && a > 0.0 && alpha > 0.0 && beta > 0.0 && rho > 0.0,
+              "omega, a, alpha, beta, rho must be positive");
+  MSC_REQUIRE(n_sub >= 1 && n_sub <= 8, "n_sub must be in 1..8");
+  MSC_REQUIRE(A_out != nullptr, "A_out must not be NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = n_sub, N = n * n * n, n9 = 9 * N, nc = n9 + 9, no = 2 * n - 1;
+  const double as = a / n, kP = omega / alpha, kS = omega / beta;
+  double coef[8];
+  int rc = msc_cube_coefficients(omega, as, alpha, beta, rho, coef);
+  if (rc) return rc;
+  const size_t stride = (size_t)n9 * nc;
+  double2 *dP = nullptr, *dtp = nullptr, *dA = nullptr, *dpsi = nullptr;
+  double* dc = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)dP, (void*)dtp, (void*)dA, (void*)dpsi, (void*)dc})
+      if (p) (void)hipFree(p);
+  };
+#define SYS_CHECK(expr)                                                   \
+  do {                                                                    \
+    hipError_t _e = (expr);                                               \
+    if (_e != hipSuccess) {                                               \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));       \
+      cleanup();                                                          \
+      return kHip;                                                        \
+    }                                                                     \
+  } while (0)
+  SYS_CHECK(hipMalloc(&dP, (size_t)no * no * no * 81 * sizeof(double2)));
+  SYS_CHECK(hipMalloc(&dtp, 4 * sizeof(double2)));
+  SYS_CHECK(hipMalloc(&dA, stride * sizeof(double2)));
+  SYS_CHECK(hipMalloc(&dpsi, (size_t)n9 * 9 * sizeof(double2)));
+  SYS_CHECK(hipMalloc(&dc, 3 * sizeof(double)));
+  const double hc[3] = {dlambda, dmu, drho};
+  SYS_CHECK(hipMemcpyAsync(dc, hc, sizeof(hc), hipMemcpyHostToDevice, st));
+  // the right-hand-side columns are not part of A: zeros
+  SYS_CHECK(hipMemsetAsync(dpsi, 0, (size_t)n9 * 9 * sizeof(double2), st));
+  hipLaunchKernelGGL(k_res_offsets, dim3((no * no * no + 63) / 64), dim3(64), 0, st, dP, n,
+                     2.0 * as, kP, kS, 1.0 / (4.0 * M_PI * rho * omega * omega));
+  SYS_CHECK(hipGetLastError());
+  rc = tparams_batch(dc, dc + 1, dc + 2, 1, omega, coef, 8.0 * as * as * as, dtp, st);
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  const int gx = (int)std::min<size_t>((stride + kRT - 1) / kRT, 2048);
+  hipLaunchKernelGGL(k_res_assemble, dim3(gx, 1), dim3(kRT), 0, st, dA, stride, dP, dtp, dpsi, n);
+  SYS_CHECK(hipGetLastError());
+  SYS_CHECK(hipMemcpyAsync(A_out, dA, (size_t)n9 * n9 * sizeof(double2), hipMemcpyDeviceToDevice,
+                           st));
+  SYS_CHECK(hipStreamSynchronize(st));
+#undef SYS_CHECK
+  cleanup();
+  return kOk;
+}
+
 // ─────────────────────────────────────────────────────────────────────────────
 // No-allocation variant (msc_resonance_tmatrices_noalloc).
 //

@@ -287,23 +287,26 @@ __device__ __forceinline__ void tapply9(const double2* t, const double2* p, doub
   for (int c = 6; c < 9; ++c) tau[c] = cmd(t[3], p[c]);
 }
 
-// Sums of tau = T0 psi: per layer (Nz x 9) and, when kP/kS given, the phased
-// sums sum_v exp(-i k r_hat . x_v) tau_v for k in {kP, kS} (2 x 9).  One
-// workgroup per layer; the phased sums are per-layer partials.
+// Sums of tau = T0 psi: per layer (Nz x 9) and the phased sums
+// sum_v exp(-i k r_hat . x_v) tau_v for k in {kP, kS} (2 x 9).  Workgroup
+// (c, l) sums chunk c of layer l's voxels into part[(l * C + c) * 27 + k];
+// k_source_reduce adds the C chunks of each layer in a fixed order, so the
+// sums are reproducible bit for bit.
 __global__ __launch_bounds__(256) void k_source_sums(const double2* __restrict__ psi,
                                                      const double2* __restrict__ tp,
                                                      const int* __restrict__ tidx,
                                                      const double2* __restrict__ tgen, int M,
-                                                     int Nz, double d, double kP, double kS,
-                                                     double rz, double rx, double ry,
-                                                     double2* __restrict__ layer_sum,
-                                                     double2* __restrict__ phased) {
+                                                     double d, double kP, double kS, double rz,
+                                                     double rx, double ry,
+                                                     double2* __restrict__ part) {
   __shared__ double2 red[27][4];
-  const int l = blockIdx.x;
+  const int c = blockIdx.x, C = gridDim.x, l = blockIdx.y;
+  const int per = (M * M + C - 1) / C;
+  const int v0 = c * per, v1 = min(M * M, v0 + per);
   double2 s[27];
   for (int k = 0; k < 27; ++k) s[k] = zc();
   const size_t base = (size_t)l * M * M;
-  for (int v = threadIdx.x; v < M * M; v += blockDim.x) {
+  for (int v = v0 + (int)threadIdx.x; v < v1; v += blockDim.x) {
     const size_t g = base + v;
     double2 tau[9];
     if (tidx != nullptr) {
@@ -344,12 +347,19 @@ __global__ __launch_bounds__(256) void k_source_sums(const double2* __restrict__
   if (threadIdx.x < 27) {
     double2 t = zc();
     for (int w = 0; w < (int)(blockDim.x / 64); ++w) t = cad(t, red[threadIdx.x][w]);
-    if (threadIdx.x < 9) {
-      layer_sum[(size_t)l * 9 + threadIdx.x] = t;
-    } else {
-      phased[(size_t)l * 18 + threadIdx.x - 9] = t;
-    }
+    part[((size_t)l * C + c) * 27 + threadIdx.x] = t;
   }
+}
+
+__global__ __launch_bounds__(64) void k_source_reduce(const double2* __restrict__ part, int C,
+                                                      int Nz, double2* __restrict__ layer_sum,
+                                                      double2* __restrict__ phased) {
+  const int l = blockIdx.x, k = threadIdx.x;
+  if (l >= Nz || k >= 27) return;
+  double2 t = zc();
+  for (int c = 0; c < C; ++c) t = cad(t, part[((size_t)l * C + c) * 27 + k]);
+  if (k < 9) layer_sum[(size_t)l * 9 + k] = t;
+  else phased[(size_t)l * 18 + k - 9] = t;
 }
 
 int grid1d(size_t n, int bs) { return (int)std::min<size_t>((n + bs - 1) / bs, 65536); }
@@ -381,13 +391,19 @@ __global__ __launch_bounds__(32) void k_nn_delta(const double* __restrict__ va, 
 
 // ── host entry points ────────────────────────────────────────────────────────
 
+int ensure_build_work(Plan& p) {
+  const int H = p.Pxy / 2 + 1;
+  const size_t bytes = (size_t)p.NC * H * H * kNE * sizeof(double2);
+  if (p.work == nullptr) MSC_HIP_CHECK(hipMalloc(&p.work, bytes));
+  if (p.work2 == nullptr) MSC_HIP_CHECK(hipMalloc(&p.work2, bytes));
+  return kOk;
+}
+
 int build_table(Plan& p, double omega, double alpha, double beta, double rho, hipStream_t st) {
   const size_t P = (size_t)p.Pz * p.Pxy * p.Pxy;
   const int P2 = p.Pxy, H = p.Pxy / 2 + 1, NC = p.NC;  // NC = Pz/2 + 1
-  if (p.work == nullptr)
-    MSC_HIP_CHECK(hipMalloc(&p.work, (size_t)NC * H * H * kNE * sizeof(double2)));
-  if (p.work2 == nullptr)
-    MSC_HIP_CHECK(hipMalloc(&p.work2, (size_t)NC * H * H * kNE * sizeof(double2)));
+  const int rc0 = ensure_build_work(p);
+  if (rc0) return rc0;
   double2* work = reinterpret_cast<double2*>(p.work);
   double2* work2 = reinterpret_cast<double2*>(p.work2);
   const double kP = omega / alpha, kS = omega / beta;
@@ -434,18 +450,24 @@ __global__ __launch_bounds__(256) void k_tmax_clear(unsigned int* tmax) {
   if (threadIdx.x == 0) *tmax = 0u;
 }
 
+// One atomic per workgroup (the waves' maxima meet in LDS first): a few
+// hundred atomics on one address instead of one per wave.
 __global__ __launch_bounds__(256) void k_tmax(const double2* __restrict__ tp, size_t n,
                                               unsigned int* __restrict__ tmax) {
+  __shared__ float wm[4];
   float m = 0.f;
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
     m = fmaxf(m, (float)sqrt(tp[i].x * tp[i].x + tp[i].y * tp[i].y));
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(tmax, __float_as_uint(m));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(tmax, __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]))));
 }
 
 static int update_tmax(Plan& p, hipStream_t st) {
   hipLaunchKernelGGL(k_tmax_clear, dim3(1), dim3(64), 0, st, p.tmax);
-  hipLaunchKernelGGL(k_tmax, dim3(std::min(grid1d(p.nvox * 4, 256), 1024)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_tmax, dim3(std::min(grid1d(p.nvox * 4, 256), 512)), dim3(256), 0, st,
                      p.tp128, p.nvox * 4, p.tmax);
   MSC_LAUNCH_CHECK();
   return kOk;
@@ -536,7 +558,7 @@ int set_tgeneral(Plan& p, const int* tidx, const double2* table, int count, hipS
   p.general = true;
   p.have_tparams = true;
   hipLaunchKernelGGL(k_tmax_clear, dim3(1), dim3(64), 0, st, p.tmax);
-  hipLaunchKernelGGL(k_tmax, dim3(std::min(grid1d((size_t)count * 81, 256), 1024)), dim3(256), 0,
+  hipLaunchKernelGGL(k_tmax, dim3(std::min(grid1d((size_t)count * 81, 256), 512)), dim3(256), 0,
                      st, p.tgen128, (size_t)count * 81, p.tmax);
   MSC_LAUNCH_CHECK();
   return kOk;
@@ -577,9 +599,22 @@ int incident_field(Plan& p, double k_mag, const double* khat3, const double* inc
 
 int source_sums(Plan& p, const double2* psi, double kP, double kS, const double* rhat3,
                 double2* layer_sum, double2* phased, hipStream_t st) {
-  hipLaunchKernelGGL(k_source_sums, dim3(p.Nz), dim3(256), 0, st, psi, p.tp128,
-                     p.general ? p.tidx : nullptr, p.tgen128, p.M, p.Nz, 2.0 * p.a, kP, kS,
-                     rhat3[0], rhat3[1], rhat3[2], layer_sum, phased);
+  // about 1024 workgroups in all, at least 256 voxels each
+  const int vox = p.M * p.M;
+  const int C = std::max(1, std::min((1024 + p.Nz - 1) / p.Nz, (vox + 255) / 256));
+  const size_t need = (size_t)p.Nz * C * 27;
+  if (need > p.sums_cap) {
+    if (p.sums_part) (void)hipFree(p.sums_part);
+    p.sums_part = nullptr;
+    p.sums_cap = 0;
+    MSC_HIP_CHECK(hipMalloc(&p.sums_part, need * sizeof(double2)));
+    p.sums_cap = need;
+  }
+  hipLaunchKernelGGL(k_source_sums, dim3(C, p.Nz), dim3(256), 0, st, psi, p.tp128,
+                     p.general ? p.tidx : nullptr, p.tgen128, p.M, 2.0 * p.a, kP, kS, rhat3[0],
+                     rhat3[1], rhat3[2], p.sums_part);
+  hipLaunchKernelGGL(k_source_reduce, dim3(p.Nz), dim3(64), 0, st, p.sums_part, C, p.Nz,
+                     layer_sum, phased);
   MSC_LAUNCH_CHECK();
   return kOk;
 }

@@ -5,6 +5,9 @@
 
 namespace msc {
 
+// The G0 build's two octant workspaces (NC x H x H x 45 complex128 each,
+// H = Pxy/2 + 1), allocated once and kept for rebuilds.
+int ensure_build_work(Plan& p);
 int set_tparams_from_contrasts(Plan& p, const double* Dl, const double* Dm, const double* Dr,
                                double omega, const double* coef8, hipStream_t st);
 int set_tparams_direct(Plan& p, const double2* tp, hipStream_t st);

@@ -10,10 +10,14 @@ library (``msc_resonance_tmatrices``): the sub-cell propagator table, the
 T-matrix are built on the device, for one cube or for a batch of contrasts
 (``compute_resonance_tmatrices``) in one call.
 
-``condition_number`` is the pivot ratio max|u_ii| / min|u_ii| of the
-elimination (an inexpensive lower estimate of the 2-norm condition number the
-reference computes with an SVD); the reference's 1e10 warning threshold
-applies to it.
+``condition_number`` of ``compute_resonance_tmatrix`` is the reference's:
+the 2-norm condition number of the device-assembled 9N x 9N matrix
+(``msc_resonance_system``), from its singular values on the host
+(np.linalg.cond, as resonance_tmatrix.py:788 does), with the same 1e10
+warning.  The batch ``compute_resonance_tmatrices`` returns the pivot ratios
+max|u_ii| / min|u_ii| of the device eliminations instead (an SVD per cube
+would cost more than the batch); a pivot ratio bounds the condition number
+from below.
 """
 
 from __future__ import annotations
@@ -129,6 +133,26 @@ def compute_resonance_tmatrices(omega: float, a: float, ref: ReferenceMedium, Dl
     return (tc if keep_on_device else _lib.to_host(tc)), ratio
 
 
+def _condition_number(omega: float, a: float, ref: ReferenceMedium, contrast: MaterialContrast,
+                      n_sub: int) -> float:
+    """2-norm condition number of the cube's internal Foldy-Lax matrix
+    I - P (I x T_loc) (resonance_tmatrix.py:783-788): the matrix is assembled
+    on the device and its singular values are taken on the host."""
+    device = torch.device(_lib.require_device())
+    lib = _lib.load()
+    n9 = 9 * n_sub ** 3
+    A = torch.empty((n9, n9), dtype=torch.complex128, device=device)
+    with torch.cuda.device(device):
+        torch.cuda.current_stream(device).synchronize()
+        rc = lib.msc_resonance_system(float(omega), float(a), float(ref.alpha), float(ref.beta),
+                                      float(ref.rho), int(n_sub), float(contrast.Dlambda),
+                                      float(contrast.Dmu), float(contrast.Drho), A.data_ptr(),
+                                      None)
+    _lib.check(rc)
+    # column-major on the device; the condition number of A^T is that of A
+    return float(np.linalg.cond(_lib.to_host(A)))
+
+
 def compute_resonance_tmatrix(omega: float, a: float, ref: ReferenceMedium,
                               contrast: MaterialContrast, n_sub: int | None = None,
                               k_hat: NDArray | None = None, wave_type: str = "S",
@@ -150,8 +174,8 @@ def compute_resonance_tmatrix(omega: float, a: float, ref: ReferenceMedium,
                                 [contrast.Drho], n_sub, k_hat, wave_type, neumann_order,
                                 neumann_tol, True, None)
     T = tc[0].cpu().numpy()
-    cond = float(ratio[0])
-    if neumann_order == 0 and cond > 1.0e10:
+    cond = _condition_number(omega, a, ref, contrast, n_sub)
+    if cond > 1.0e10:
         warnings.warn(f"Coupled Foldy-Lax condition number = {cond:.2e}. Near-resonance or "
                       "strong-coupling instability.", UserWarning, stacklevel=2)
     n_conv = None

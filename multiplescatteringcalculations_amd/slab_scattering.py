@@ -525,9 +525,11 @@ def _give_plan(key, plan) -> None:
 
 def release_plans() -> None:
     """Drop the plan kept between solves (its device memory is freed once no
-    result refers to it)."""
+    result refers to it) and the plan pools the batched sweep keeps."""
     with _PLAN_LOCK:
         _PLAN_CACHE.clear()
+    if _lib._lib is not None:
+        _lib._lib.msc_sweep_release()
 
 
 # free the cached plan while the HIP runtime is still up

@@ -4,12 +4,16 @@ The reference solves one slab per call and loops over frequencies on the host
 (compute_ocean_bottom_reflection, cubic_scattering/ocean_bottom.py:290-307:
 compute_slab_scattering + compute_slab_tmatrices + slab_rpp_periodic per
 frequency).  ``compute_slab_sweep`` hands a whole list of cases to the
-library's native sweep driver (``msc_sweep``): worker threads with their own
-HIP streams and one re-shapable plan per FFT grid run the solves concurrently
-on the device, G0 tables are rebuilt only when a case needs a new one, and
-the layer / far-field source sums come back with each solution, so
-``slab_rpp_periodic`` and ``slab_reflected_field`` of the results need no
-further device work.
+library's native sweep driver.  The default (``mode="batched"``,
+``msc_sweep_batched``) groups the cases by padded FFT grid and solves each
+group as one batch: every operator pass and Arnoldi kernel of a GMRES step is
+a single launch for all slabs of the group, whatever their sizes.  The
+alternative (``mode="workers"``, ``msc_sweep``) runs the cases one by one on
+worker threads with their own HIP streams, concurrently on the device.  Either
+way G0 tables are built on the device per case, plans are re-shaped rather
+than re-created, and the layer / far-field source sums come back with each
+solution, so ``slab_rpp_periodic`` and ``slab_reflected_field`` of the results
+need no further device work.
 """
 
 from __future__ import annotations
@@ -43,12 +47,16 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
                        restart: int = 30, precision: str = "mixed", inner_tol: float = 1e-4,
                        orth: str = "adaptive", workers: int = 4,
                        device: torch.device | None = None,
-                       keep_on_device: bool = False) -> list[SlabResult]:
+                       keep_on_device: bool = False, mode: str = "batched",
+                       max_batch: int = 64) -> list[SlabResult]:
     """Solve every case (GMRES(restart) from the Born guess to ``gmres_tol`` FP64
     relative residual) and return one ``SlabResult`` per case, in order.
 
-    ``workers`` host threads run cases concurrently on the device.  With
-    ``keep_on_device`` the fields stay torch tensors on the device.
+    ``mode="batched"``: the cases of one padded FFT grid are solved as batches
+    of at most ``max_batch`` (one launch per operator pass and Arnoldi kernel
+    for the whole batch).  ``mode="workers"``: ``workers`` host threads run
+    cases one at a time, concurrently on the device.  With ``keep_on_device``
+    the fields stay torch tensors on the device.
     Non-convergence warns per case (as the reference's solver does); invalid
     cases raise ``ValueError`` before anything runs.
     """
@@ -66,24 +74,45 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
     orth_code = {"adaptive": 0, "cgs": 1, "cgs2": 2}.get(orth)
     if orth_code is None:
         raise ValueError(f"orth must be 'adaptive', 'cgs' or 'cgs2', got '{orth}'")
+    if mode not in ("batched", "workers"):
+        raise ValueError(f"mode must be 'batched' or 'workers', got '{mode}'")
     if workers < 1:
         raise ValueError("workers must be >= 1")
+    if max_batch < 1:
+        raise ValueError("max_batch must be >= 1")
     lib = _lib.load()
     n = len(cases)
     inst = (_lib.Instance * max(n, 1))()
-    keep = []  # device tensors referenced by the instances
+    # every case's contrasts in one pinned host buffer and one upload; every
+    # case's layer / far-field sums in one device buffer and one download
+    nv = [c.geometry.N_z * c.geometry.M * c.geometry.M for c in cases]
+    voff = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
+    nzs = [c.geometry.N_z for c in cases]
+    soff = np.concatenate([[0], np.cumsum(nzs)]).astype(np.int64)
+    host = torch.empty(3 * int(voff[-1]), dtype=torch.float64, pin_memory=True)
+    hv = host.numpy()
+    for i, c in enumerate(cases):
+        g, mat = c.geometry, c.material
+        for k, a in enumerate((mat.Dlambda, mat.Dmu, mat.Drho)):
+            seg = hv[3 * voff[i] + k * nv[i]:3 * voff[i] + (k + 1) * nv[i]]
+            seg[:] = np.broadcast_to(np.asarray(a, dtype=np.float64),
+                                     (g.N_z, g.M, g.M)).reshape(-1)
+    contrasts = host.to(device, non_blocking=True)
+    sums = torch.empty(27 * int(soff[-1]), dtype=torch.complex128, device=device)
+    keep = [host, contrasts]  # buffers referenced by the instances
     outs = []
     for i, c in enumerate(cases):
         g, mat = c.geometry, c.material
         k_hat, k_mag, inc = _incident_vector(c.omega, mat.ref, c.k_hat, c.wave_type)
         shape = (g.N_z, g.M, g.M)
-        arrs = [torch.tensor(np.broadcast_to(np.asarray(a, dtype=np.float64), shape),
-                             device=device) for a in (mat.Dlambda, mat.Dmu, mat.Drho)]
+        arrs = [contrasts[3 * voff[i] + k * nv[i]:3 * voff[i] + (k + 1) * nv[i]]
+                for k in range(3)]
         psi = torch.empty((*shape, 9), dtype=torch.complex128, device=device)
         psi0 = torch.empty_like(psi)
-        ls = torch.empty((g.N_z, 9), dtype=torch.complex128, device=device)
-        ph = torch.empty((g.N_z, 2, 9), dtype=torch.complex128, device=device)
-        keep.extend(arrs)
+        # contiguous (N_z, 9) layer sums, then (N_z, 2, 9) phased sums
+        o = 27 * int(soff[i])
+        ls = sums[o:o + 9 * g.N_z].view(g.N_z, 9)
+        ph = sums[o + 9 * g.N_z:o + 27 * g.N_z].view(g.N_z, 2, 9)
         outs.append((psi, psi0, ls, ph, k_hat))
         I = inst[i]
         I.M, I.N_z, I.periodic, I.a = g.M, g.N_z, int(bool(c.periodic)), float(g.a)
@@ -102,13 +131,20 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
     m = max(1, int(restart))
     opts = _lib.GmresOpts(m, int(max_iter), float(gmres_tol), 1,
                           1 if precision == "mixed" else 0, float(inner_tol), orth_code, None)
-    torch.cuda.synchronize(device)  # the uploads are on torch's stream; the workers use their own
+    torch.cuda.synchronize(device)  # the uploads are on torch's stream; the sweep uses its own
     with torch.cuda.device(device):
-        rc = lib.msc_sweep(ctypes.cast(inst, ctypes.c_void_p), n, ctypes.byref(opts), int(workers))
+        if mode == "batched":
+            rc = lib.msc_sweep_batched(ctypes.cast(inst, ctypes.c_void_p), n, ctypes.byref(opts),
+                                       int(max_batch),
+                                       ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream))
+        else:
+            rc = lib.msc_sweep(ctypes.cast(inst, ctypes.c_void_p), n, ctypes.byref(opts),
+                               int(workers))
     if rc in (_lib.MSC_ERR_INVALID, _lib.MSC_ERR_UNSUPPORTED):
         bad = [i for i in range(n) if inst[i].status == rc]
         msg = inst[bad[0]].error.decode(errors="replace") if bad else "invalid sweep arguments"
         raise ValueError(f"sweep: {msg}")
+    sums_h = _lib.to_host(sums)
     results = []
     for i, c in enumerate(cases):
         I = inst[i]
@@ -130,7 +166,9 @@ def compute_slab_sweep(cases: list[SweepCase], *, gmres_tol: float = 1e-8, max_i
                          gmres_residual=float(st["rel_residual"]), periodic=bool(c.periodic),
                          stats=st)
         ref = c.material.ref
-        res._sums = (ls.cpu().numpy(), ph.cpu().numpy(),
+        o, nz = 27 * int(soff[i]), c.geometry.N_z
+        res._sums = (sums_h[o:o + 9 * nz].reshape(nz, 9).copy(),
+                     sums_h[o + 9 * nz:o + 27 * nz].reshape(nz, 2, 9).copy(),
                      (c.omega / ref.alpha, c.omega / ref.beta), -k_hat)
         results.append(res)
     return results

@@ -224,11 +224,13 @@ def test_config2_solves_agree(hip_device):
         assert abs(_rpp_of(plan, x, p) - rpp_ref) <= 1e-6 * abs(rpp_ref), kw
 
 
-def test_config3_sweep_instances(hip_device):
+@pytest.mark.parametrize("mode", ["batched", "workers"])
+def test_config3_sweep_instances(hip_device, mode):
     """Eight instances of the 256-case sweep workload (lateral 32-128, depth
     4-32, ka across 0.1-1.0, all four slownesses) plus one 32 x 32 x 6 slab,
-    in one mixed-precision sweep: each reaches its FP64 residual 1e-8 and
-    equals its own FP64 solve to 1e-6; the small one equals the oracle."""
+    in one mixed-precision sweep (batches of one padded grid, or worker
+    threads): each reaches its FP64 residual 1e-8 and equals its own FP64
+    solve to 1e-6; the small one equals the oracle."""
     import bench
 
     msc = _msc()
@@ -242,7 +244,7 @@ def test_config3_sweep_instances(hip_device):
     small = msc.SweepCase(gs, msc.random_iid_slab_material(gs, ref, 0.3, 0.2, seed=77),
                           0.55 * VS, _khat(2e-4), "P")
     cases.append(small)
-    res = msc.compute_slab_sweep(cases, gmres_tol=1e-8, precision="mixed", workers=4)
+    res = msc.compute_slab_sweep(cases, gmres_tol=1e-8, precision="mixed", workers=4, mode=mode)
     for c, r in zip(cases, res):
         assert r.stats["converged"] and r.gmres_residual <= 1e-8, c.geometry
         one = msc.compute_slab_scattering_gpu(c.geometry, c.material, c.omega, c.k_hat, "P",
