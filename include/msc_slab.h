@@ -264,7 +264,10 @@ int msc_plan_tiles(const msc_plan* plan, int* out10);
  * precisions tuned).  A tile larger than the LDS allows for the plan's FFT
  * lengths falls back to the next smaller one.  A z tile of Q + 256 (Q = 2, 4,
  * 8) selects the persistent z pass: resident workgroups walk the tiles and
- * prefetch the next tile's rows into LDS with direct-to-LDS loads. */
+ * prefetch the next tile's rows into LDS with direct-to-LDS loads.  A z tile
+ * of Q + 512 selects the pruned z pass: the forward z FFT's first pass reads
+ * the N_z non-zero rows straight from HBM and the inverse's last pass writes
+ * the N_z kept rows straight back. */
 int msc_plan_set_tiles(msc_plan* plan, const int* tiles10);
 
 /*

@@ -83,6 +83,47 @@ __device__ __forceinline__ void dft(cx<R>* v) {
   }
 }
 
+// dft() of inputs whose upper half v[RAD/2 ..] is zero (not read): the zero
+// terms are left out instead of added (x + 0 is not x for x = -0, so the
+// compiler keeps such additions).
+template <typename R, bool INV>
+__device__ __forceinline__ void dft4_half(cx<R>& x0, cx<R>& x1, cx<R>& x2, cx<R>& x3) {
+  const cx<R> a = x0, m = mul_mi<R, INV>(x1);
+  x2 = csub<R>(a, x1);
+  x0 = cadd<R>(a, x1);
+  x1 = cadd<R>(a, m);
+  x3 = csub<R>(a, m);
+}
+
+template <typename R, int RAD, bool INV>
+__device__ __forceinline__ void dft_half(cx<R>* v) {
+  if constexpr (RAD == 2) {
+    v[1] = v[0];
+  } else if constexpr (RAD == 4) {
+    dft4_half<R, INV>(v[0], v[1], v[2], v[3]);
+  } else {
+    static_assert(RAD == 8, "radix must be 2, 4 or 8");
+    cx<R> e0 = v[0], e1 = v[2], e2, e3;
+    cx<R> o0 = v[1], o1 = v[3], o2, o3;
+    dft4_half<R, INV>(e0, e1, e2, e3);
+    dft4_half<R, INV>(o0, o1, o2, o3);
+    const R h = R(0.70710678118654752440084436210485);
+    cx<R> w1 = INV ? mk<R>(h * (o1.x - o1.y), h * (o1.x + o1.y))
+                   : mk<R>(h * (o1.x + o1.y), h * (o1.y - o1.x));
+    cx<R> w2 = mul_mi<R, INV>(o2);
+    cx<R> w3 = INV ? mk<R>(-h * (o3.x + o3.y), h * (o3.x - o3.y))
+                   : mk<R>(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+    v[0] = cadd<R>(e0, o0);
+    v[4] = csub<R>(e0, o0);
+    v[1] = cadd<R>(e1, w1);
+    v[5] = csub<R>(e1, w1);
+    v[2] = cadd<R>(e2, w2);
+    v[6] = csub<R>(e2, w2);
+    v[3] = cadd<R>(e3, w3);
+    v[7] = csub<R>(e3, w3);
+  }
+}
+
 // One Stockham pass: sub-transform length NS grows to NS*RAD.
 template <typename R, int N, int RAD, int NS, bool INV, int NTR, int LD, int NT>
 __device__ __forceinline__ void stockham_pass(cx<R>* s, const cx<R>* __restrict__ tw) {
@@ -196,16 +237,56 @@ __device__ __forceinline__ void first_pass_pruned(const cx<R>* src, cx<R>* dst, 
         const int row = j + r * NB;
         v[r] = (r < RAD / 2 && row < nrows) ? src[row * NTR + col] : czero<R>();
       }
-      dft<R, RAD, false>(v);
+      dft_half<R, RAD, false>(v);
 #pragma unroll
       for (int r = 0; r < RAD; ++r) dst[(j * RAD + r) * LDD + col] = v[r];
     }
   }
 }
 
+// The same first pass reading the rows < nrows straight from global memory,
+// src[row * stride + col] (issued together before any is used), and writing
+// the pass's output to s[row * LDD + col].  No barrier.
+template <typename R, int N, int NTR, int LDD, int NT>
+__device__ __forceinline__ void first_pass_pruned_global(const cx<R>* __restrict__ src,
+                                                         size_t stride, cx<R>* dst, int nrows) {
+  constexpr int RAD = pick_radix<N>::value;
+  constexpr int NB = N / RAD;
+  constexpr int TOT = NB * NTR;
+  constexpr int ITEMS = (TOT + NT - 1) / NT;
+  constexpr int HR = RAD > 1 ? RAD / 2 : 1;
+  cx<R> v[ITEMS][RAD];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = (int)threadIdx.x + it * NT;
+    const bool in = ITEMS * NT == TOT || w < TOT;
+    const int col = w % NTR;
+    const int j = w / NTR;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) {
+      const int row = j + r * NB;
+      const bool ok = in && r < HR && row < nrows;
+      const cx<R> x = src[ok ? (size_t)row * stride + col : 0];
+      v[it][r] = ok ? x : czero<R>();
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = (int)threadIdx.x + it * NT;
+    if (ITEMS * NT == TOT || w < TOT) {
+      const int col = w % NTR;
+      const int j = w / NTR;
+      if constexpr (RAD > 1) dft_half<R, RAD, false>(v[it]);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) dst[(j * RAD + r) * LDD + col] = v[it][r];
+    }
+  }
+}
+
 // Last pass of an inverse transform held in s (after fft_stages_but_last),
 // handing only the output rows < nrows to store(row, col, value) instead of
-// writing them back to LDS.
+// writing them back to LDS.  nrows <= N / 2: the outputs j + r NS with
+// r >= RAD / 2 (rows >= N / 2) are never formed.
 template <typename R, int N, int NTR, int LD, int NT, typename Store>
 __device__ __forceinline__ void last_pass_inv_store(const cx<R>* s, const cx<R>* __restrict__ tw,
                                                     int nrows, Store store) {
@@ -234,7 +315,7 @@ __device__ __forceinline__ void last_pass_inv_store(const cx<R>* s, const cx<R>*
       }
       dft<R, RAD, true>(v);
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) {
+      for (int r = 0; r < (RAD > 1 ? RAD / 2 : 1); ++r) {
         const int row = j + r * NS;
         if (row < nrows) store(row, col, v[r]);
       }

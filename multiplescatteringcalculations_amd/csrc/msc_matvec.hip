@@ -302,34 +302,13 @@ __global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
 
 
 // ── pass 3: z FFT, spectral 9x9 multiply, inverse z FFT (in place) ─────────
-template <typename R, int PZ, int Q>
-__device__ __forceinline__ void z_mult_body(unsigned bx_, unsigned gx_, cx<R>* __restrict__ buf2,
-                                               const cx<R>* __restrict__ table, int Nz,
-                                               int Pxy, const cx<R>* __restrict__ tw,
-                                               const int* __restrict__ zmap,
-                                               const int* __restrict__ done) {
-  constexpr int NTR = 9 * Q;
-  constexpr int LD = lds_ld<NTR>::value;
+// Block multiply of one z-pass tile held in LDS (Q columns ky0.. of row kx,
+// PZ wavenumbers kz each), one (column, kz) per item: the table entries are
+// streamed (each loaded and applied at once) rather than held in registers.
+template <typename R, int PZ, int Q, int LD>
+__device__ __forceinline__ void zmul_items(cx<R>* s, const cx<R>* __restrict__ table, int kx,
+                                           int ky0, int Pxy) {
   constexpr int NC = PZ / 2 + 1;
-  __shared__ cx<R> s[PZ * LD];
-  if (done != nullptr && *done) return;
-  const int nck = Pxy / Q;
-  const int tile = zmap[bx_];
-  const int kx = tile / nck;
-  const int ky0 = (tile - kx * nck) * Q;
-  const size_t plane = (size_t)Pxy * Pxy * 9;
-  const size_t col0 = ((size_t)kx * Pxy + ky0) * 9;
-  for (int w = threadIdx.x; w < PZ * NTR; w += kNT) {
-    const int z = w / NTR;
-    const int e = w - z * NTR;
-    const bool ok = z < Nz;
-    const cx<R> v = buf2[ok ? (size_t)z * plane + col0 + e : col0];
-    s[z * LD + e] = ok ? v : czero<R>();
-  }
-  __syncthreads();
-  lds_fft<R, PZ, false, NTR, LD, kNT>(s, tw);
-  // Block multiply, one (column, kz) per item: the table entries are
-  // streamed (each loaded and applied at once) rather than held in registers.
   const int half = Pxy / 2;
   constexpr int HZ = PZ / 2;
   for (int w = threadIdx.x; w < PZ * Q; w += kNT) {
@@ -355,6 +334,33 @@ __device__ __forceinline__ void z_mult_body(unsigned bx_, unsigned gx_, cx<R>* _
 #pragma unroll
     for (int i = 0; i < 9; ++i) sp[i] = flip_if<R>(o[0][i], flip, i);
   }
+}
+template <typename R, int PZ, int Q>
+__device__ __forceinline__ void z_mult_body(unsigned bx_, unsigned gx_, cx<R>* __restrict__ buf2,
+                                               const cx<R>* __restrict__ table, int Nz,
+                                               int Pxy, const cx<R>* __restrict__ tw,
+                                               const int* __restrict__ zmap,
+                                               const int* __restrict__ done) {
+  constexpr int NTR = 9 * Q;
+  constexpr int LD = lds_ld<NTR>::value;
+  __shared__ cx<R> s[PZ * LD];
+  if (done != nullptr && *done) return;
+  const int nck = Pxy / Q;
+  const int tile = zmap[bx_];
+  const int kx = tile / nck;
+  const int ky0 = (tile - kx * nck) * Q;
+  const size_t plane = (size_t)Pxy * Pxy * 9;
+  const size_t col0 = ((size_t)kx * Pxy + ky0) * 9;
+  for (int w = threadIdx.x; w < PZ * NTR; w += kNT) {
+    const int z = w / NTR;
+    const int e = w - z * NTR;
+    const bool ok = z < Nz;
+    const cx<R> v = buf2[ok ? (size_t)z * plane + col0 + e : col0];
+    s[z * LD + e] = ok ? v : czero<R>();
+  }
+  __syncthreads();
+  lds_fft<R, PZ, false, NTR, LD, kNT>(s, tw);
+  zmul_items<R, PZ, Q, LD>(s, table, kx, ky0, Pxy);
   __syncthreads();
   lds_fft<R, PZ, true, NTR, LD, kNT>(s, tw);
   for (int w = threadIdx.x; w < Nz * NTR; w += kNT) {
@@ -363,6 +369,47 @@ __device__ __forceinline__ void z_mult_body(unsigned bx_, unsigned gx_, cx<R>* _
     buf2[(size_t)z * plane + col0 + e] = s[z * LD + e];
   }
 }
+
+// Pass 3 with pruned z FFTs and no staging round trips through LDS: the
+// forward transform's first pass reads its non-zero input rows (z < Nz)
+// straight from HBM into registers, and the inverse transform's last pass
+// writes the kept rows (z < Nz) straight back; the zero rows are neither
+// read nor transformed, the rows z >= Nz of the result never formed.  Same
+// tiles, tile order and block multiply as z_mult_body.
+template <typename R, int PZ, int Q>
+__device__ __forceinline__ void z_mult_pruned_body(unsigned bx_, cx<R>* __restrict__ buf2,
+                                                   const cx<R>* __restrict__ table, int Nz,
+                                                   int Pxy, const cx<R>* __restrict__ tw,
+                                                   const int* __restrict__ zmap,
+                                                   const int* __restrict__ done) {
+  constexpr int NTR = 9 * Q;
+  constexpr int LD = lds_ld<NTR>::value;
+  __shared__ cx<R> s[PZ * LD];
+  if (done != nullptr && *done) return;
+  const int nck = Pxy / Q;
+  const int tile = zmap[bx_];
+  const int kx = tile / nck;
+  const int ky0 = (tile - kx * nck) * Q;
+  const size_t plane = (size_t)Pxy * Pxy * 9;
+  cx<R>* col = buf2 + ((size_t)kx * Pxy + ky0) * 9;
+  first_pass_pruned_global<R, PZ, NTR, LD, kNT>(col, plane, s, Nz);
+  __syncthreads();
+  fft_stages<R, PZ, pick_radix<PZ>::value, false, NTR, LD, kNT>(s, tw);
+  zmul_items<R, PZ, Q, LD>(s, table, kx, ky0, Pxy);
+  __syncthreads();
+  fft_stages_but_last<R, PZ, 1, true, NTR, LD, kNT>(s, tw);
+  last_pass_inv_store<R, PZ, NTR, LD, kNT>(
+      s, tw, Nz, [&](int row, int c, cx<R> v) { col[(size_t)row * plane + c] = v; });
+}
+template <typename R, int PZ, int Q>
+__global__ __launch_bounds__(kNT) void k_z_mult_pr(cx<R>* __restrict__ buf2,
+                                                  const cx<R>* __restrict__ table, int Nz,
+                                                  int Pxy, const cx<R>* __restrict__ tw,
+                                                  const int* __restrict__ zmap,
+                                                  const int* __restrict__ done) {
+  z_mult_pruned_body<R, PZ, Q>(blockIdx.x, buf2, table, Nz, Pxy, tw, zmap, done);
+}
+
 template <typename R, int PZ, int Q>
 __global__ __launch_bounds__(kNT) void k_z_mult(cx<R>* __restrict__ buf2,
                                                const cx<R>* __restrict__ table, int Nz,
@@ -830,6 +877,12 @@ static void z_q(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
                      PlanBufs<R>::table(p), p.Nz, p.Pxy, PlanBufs<R>::tw(p),
                      p.zmap + zmap_offset(p.Pxy, Q), done);
 }
+template <typename R, int PZ, int Q>
+static void z_q_pr(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
+  hipLaunchKernelGGL((k_z_mult_pr<R, PZ, Q>), dim3(p.Pxy * (p.Pxy / Q)), dim3(kNT), 0, st, b2,
+                     PlanBufs<R>::table(p), p.Nz, p.Pxy, PlanBufs<R>::tw(p),
+                     p.zmap + zmap_offset(p.Pxy, Q), done);
+}
 
 // Persistent direct-to-LDS form: the z tile code is Q | kZDma.  The grid is
 // the number of workgroups resident at once (occupancy from the runtime,
@@ -874,6 +927,24 @@ static int launch_z(Plan& p, cx<R>* b2, hipStream_t st, const int* done) {
   const int code = p.tile[prec_of<R>()][2];
   int q = code & (kZDma - 1);
   while (q > 1 && p.Pxy % q != 0) q >>= 1;
+  if (code & kZPrune) {
+    switch (q) {
+      case 16:
+      case 8:
+        if constexpr (lds_fits<R, PZ, 9, 8>()) { z_q_pr<R, PZ, 8>(p, b2, st, done); break; }
+        [[fallthrough]];
+      case 4:
+        if constexpr (lds_fits<R, PZ, 9, 4>()) { z_q_pr<R, PZ, 4>(p, b2, st, done); break; }
+        [[fallthrough]];
+      case 2:
+        if constexpr (lds_fits<R, PZ, 9, 2>()) { z_q_pr<R, PZ, 2>(p, b2, st, done); break; }
+        [[fallthrough]];
+      default:
+        z_q_pr<R, PZ, 1>(p, b2, st, done);
+    }
+    MSC_LAUNCH_CHECK();
+    return timing_end(p, st, slot);
+  }
   if (code & kZDma) {
     switch (q) {
       case 8:
@@ -1242,13 +1313,14 @@ int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st) {
   MSC_HIP_CHECK(hipEventCreate(&e1));
   static const int cand_y[] = {1, 2};
   static const int cand_x[] = {1, 2, 4};
-  static const int cand_z[] = {2, 4, 8, 2 | kZDma, 4 | kZDma, 8 | kZDma};
+  static const int cand_z[] = {2, 4, 8, 2 | kZDma, 4 | kZDma, 8 | kZDma,
+                               2 | kZPrune, 4 | kZPrune, 8 | kZPrune};
   static const int order[] = {2, 1, 3, 0, 4};
   int rc = apply<R>(p, x, nullptr, out, 2, st, nullptr);  // warm every buffer
   for (int pass : order) {
     if (rc) break;
     const int* cand = pass == 2 ? cand_z : (pass == 0 || pass == 4 ? cand_y : cand_x);
-    const int nc = pass == 2 ? 6 : (pass == 0 || pass == 4 ? 2 : 3);
+    const int nc = pass == 2 ? 9 : (pass == 0 || pass == 4 ? 2 : 3);
     int best = p.tile[pr][pass];
     float best_ms = 1e30f;
     for (int i = 0; i < nc && !rc; ++i) {

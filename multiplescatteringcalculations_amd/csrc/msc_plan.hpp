@@ -18,7 +18,8 @@
 
 namespace msc {
 
-constexpr int kZDma = 256;  // z tile code flag: persistent direct-to-LDS pass 3
+constexpr int kZDma = 256;    // z tile code flag: persistent direct-to-LDS pass 3
+constexpr int kZPrune = 512;  // z tile code flag: pass 3 with pruned z FFTs, HBM-direct
 constexpr int kNE = 45;  // independent entries of the 9x9 spectral propagator
 
 // Component sign flips of the lattice reflections (z, x, y) acting on the
