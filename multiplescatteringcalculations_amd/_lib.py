@@ -11,6 +11,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_int, c_longlong, c_void_p
 from pathlib import Path
 
+import numpy as np
 import torch  # noqa: F401  (loads the HIP runtime first)
 
 LIB_PATH = Path(__file__).resolve().parent / "libmsc_slab.so"
@@ -162,6 +163,22 @@ def to_host(t: torch.Tensor) -> np.ndarray:
         out.copy_(t, non_blocking=True)
         torch.cuda.current_stream(t.device).synchronize()
     return out.numpy()
+
+
+def to_device(a, device: torch.device) -> torch.Tensor:
+    """A host array on the device, copied through pinned memory on the current
+    stream (the counterpart of to_host; pageable copies of a few MB cost
+    milliseconds each)."""
+    a = np.ascontiguousarray(a)
+    t = torch.from_numpy(a)
+    if t.numel() * t.element_size() < (1 << 20):
+        return t.to(device)
+    pinned = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    pinned.copy_(t)
+    with torch.cuda.device(device):
+        out = pinned.to(device, non_blocking=True)
+        torch.cuda.current_stream(device).synchronize()
+    return out
 
 
 def require_device() -> torch.device:

@@ -525,9 +525,13 @@ def _give_plan(key, plan) -> None:
 
 def release_plans() -> None:
     """Drop the plan kept between solves (its device memory is freed once no
-    result refers to it) and the plan pools the batched sweep keeps."""
+    result refers to it), the nine plans of the batched sphere solve and the
+    plan pools the batched sweep keeps."""
     with _PLAN_LOCK:
         _PLAN_CACHE.clear()
+    from .sphere_scattering_fft import release_batch_plans
+
+    release_batch_plans()
     if _lib._lib is not None:
         _lib._lib.msc_sweep_release()
 

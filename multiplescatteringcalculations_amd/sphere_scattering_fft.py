@@ -12,13 +12,17 @@ plan of this package (spectral G0 table of the sub-cube lattice, five-pass
 FFT matvec, device GMRES), once per incident pattern (9 right-hand sides,
 resonance_tmatrix.py:526-593, built at the inside centres), and the composite
 T-matrix column T_loc sum_m psi_m is the device sum of tau = T0 psi over the
-slab.  The 9 solves are independent: ``workers`` host threads each run a plan
-on a stream of their own, so their launches overlap on the device.
+slab.  The 9 solves are independent and share the operator: by default they
+run as one batch (msc_gmres_batched on nine plans of one grid, the first
+building G0 and the others copying it), so every operator pass and Arnoldi
+kernel is one launch for all nine right-hand sides; with ``workers`` > 0 host
+threads instead each run a plan on a stream of their own.
 """
 
 from __future__ import annotations
 
 import os
+import threading
 import warnings
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
@@ -102,13 +106,13 @@ def _incident_patterns(centres, omega, ref, k_hat, wave_type):
 
 
 def _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type, inclusive, tol, max_iter,
-           restart, precision, x0, device, workers=1):
+           restart, precision, x0, device, workers=0):
     if wave_type not in ("P", "S"):
         raise ValueError(f"wave_type must be 'P' or 'S', got '{wave_type}'")
     if n_sub < 1:
         raise ValueError("n_sub must be >= 1")
-    if int(workers) < 1:
-        raise ValueError("workers must be >= 1")
+    if int(workers) < 0:
+        raise ValueError("workers must be >= 0 (0: one batch of the 9 right-hand sides)")
     if device is None:
         device = _lib.require_device()
     _, cen, a_sub, keep = _grid(radius, n_sub, inclusive)
@@ -123,6 +127,9 @@ def _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type, inclusive, tol
     psi = np.zeros((N, 9, 9), dtype=complex)
     T = np.zeros((9, 9), dtype=complex)
     n_bad = []
+    if int(workers) == 0:
+        return _solve_batched(g, mat, omega, ref, U, flat, N, n_sub, tol, max_iter, restart,
+                              precision, x0, device, cen, a_sub)
     nw = min(int(workers), 9)
 
     def make_plan(src):
@@ -181,6 +188,63 @@ def _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type, inclusive, tol
     return cen, a_sub, N, psi.reshape(9 * N, 9), T, n_bad
 
 
+# The nine plans of the last batched sphere solve, kept for the next solve on
+# the same sub-cube grid (their buffers and GMRES workspaces are reused);
+# slab_scattering.release_plans() drops them.
+_BATCH_PLANS: dict = {}
+_BATCH_LOCK = threading.Lock()
+
+
+def _solve_batched(g, mat, omega, ref, U, flat, N, n_sub, tol, max_iter, restart, precision, x0,
+                   device, cen, a_sub):
+    """The 9 right-hand sides as one batch: nine plans of the sub-cube grid
+    (G0 built once and copied), one msc_gmres_batched call."""
+    from .slab_scattering import gmres_batched
+
+    key = (str(torch.device(device)), g.M, g.N_z, float(g.a))
+    with _BATCH_LOCK:
+        plans = _BATCH_PLANS.pop(key, None)
+    with torch.cuda.device(device):
+        if plans is None:
+            plans = [SlabPlan(g, device=device) for _ in range(9)]
+        plans[0].build_g0(omega, ref)
+        for plan in plans[1:]:
+            plan.copy_g0_from(plans[0])
+        for plan in plans:
+            plan.set_material(mat, omega)
+        # the nine incident patterns in one upload, scattered onto the grid
+        # on the device; the nine solutions gathered back in one download
+        dev = plans[0].device
+        idx = torch.as_tensor(flat, device=dev)
+        U_d = _lib.to_device(U, dev)
+        bs = []
+        for col in range(9):
+            b = torch.zeros((n_sub ** 3, 9), dtype=torch.complex128, device=dev)
+            b[idx] = U_d[:, :, col]
+            bs.append(b.reshape(-1))
+        m = int(max_iter) if restart is None else int(restart)
+        xs, sts = gmres_batched(plans, bs, x0, tol=tol, max_iter=max_iter, restart=m,
+                                precision=precision)
+        T = np.zeros((9, 9), dtype=complex)
+        n_bad = []
+        for col, (plan, x, st) in enumerate(zip(plans, xs, sts)):
+            if st["rel_residual"] > tol:
+                n_bad.append((col, st["rel_residual"], st["iterations"]))
+            ls, _ = plan.source_sums(x, 0.0, 0.0, np.zeros(3))
+            T[:, col] = ls.sum(axis=0)
+        psi = _lib.to_host(torch.stack([x.reshape(-1, 9)[idx] for x in xs], dim=-1))
+    with _BATCH_LOCK:
+        _BATCH_PLANS.clear()  # nine plans hold the grid's workspaces: keep one set
+        _BATCH_PLANS[key] = plans
+    return cen, a_sub, N, psi.reshape(9 * N, 9), T, n_bad
+
+
+def release_batch_plans() -> None:
+    """Drop the nine plans kept by the batched solve."""
+    with _BATCH_LOCK:
+        _BATCH_PLANS.clear()
+
+
 def _result(omega, radius, ref, contrast, n_sub, cen, a_sub, N, psi, T):
     return SphereDecompositionResult(T3x3=T[:3, :3].copy(), T_comp_9x9=T, centres=cen,
                                      n_sub=n_sub, n_cells=N, a_sub=a_sub,
@@ -197,7 +261,7 @@ def compute_sphere_foldy_lax_fft(omega: float, radius: float, ref: ReferenceMedi
     pattern to ``gmres_tol`` (sphere_scattering_fft.py:228-326)."""
     cen, a_sub, N, psi, T, bad = _solve(omega, radius, ref, contrast, n_sub, k_hat, wave_type,
                                         False, gmres_tol, gmres_maxiter * 20, 20, "fp64", "born",
-                                        None, 3)
+                                        None, 0)
     for col, rr, it in bad:
         warnings.warn(f"GMRES did not converge for column {col} (rel_res={rr:.2e} after {it} "
                       "iterations)", UserWarning, stacklevel=2)
@@ -211,14 +275,15 @@ def compute_sphere_foldy_lax_fft_gpu(omega: float, radius: float, ref: Reference
                                      initial_guess: str = "born", device=None,
                                      dtype: torch.dtype | None = None,
                                      restart: int | None = None,
-                                     workers: int = 3) -> SphereDecompositionResult:
+                                     workers: int = 0) -> SphereDecompositionResult:
     """Sphere T-matrix on the device (sphere_scattering_fft_gpu.py:132-244).
 
     ``dtype`` complex128 (default) solves in FP64; complex64 selects the mixed
     form (complex64 operator inside FP64 refinement, same FP64 residual
     target).  ``restart=None`` keeps the reference's unrestarted GMRES.
-    ``workers`` concurrent solves (plans on their own streams) share the 9
-    right-hand sides."""
+    ``workers=0`` (default) solves the 9 right-hand sides as one batch (one
+    launch per operator pass and Arnoldi kernel for all nine); ``workers`` > 0
+    runs that many concurrent solves on plans of their own streams instead."""
     if initial_guess not in ("born", "zero"):
         raise ValueError(f"Invalid initial_guess '{initial_guess}'. Must be 'born' or 'zero'.")
     precision = "mixed" if dtype == torch.complex64 else "fp64"

@@ -85,7 +85,29 @@ def test_workers_give_the_serial_result(hip_device, dtype):
         np.testing.assert_array_equal(res.T_comp_9x9, one.T_comp_9x9)
         np.testing.assert_array_equal(res.psi_exc, one.psi_exc)
     with pytest.raises(ValueError, match="workers"):
-        sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, workers=0)
+        sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, workers=-1)
+
+
+@pytest.mark.parametrize("dtype", [None, torch.complex64])
+def test_batched_right_hand_sides_equal_the_serial_result(hip_device, dtype):
+    """The default solve (the 9 right-hand sides as one msc_gmres_batched
+    batch) agrees with one worker solving them one after another to the
+    solver tolerance (the batched cycles normalise the right-hand side, so
+    not bit for bit)."""
+    msc, sph = _mods()
+    r = msc.ReferenceMedium(REF.alpha, REF.beta, REF.rho)
+    c = msc.MaterialContrast(-3e9, 2e9, -200.0)
+    om, R, n = 2 * np.pi * 50.0, 12.0, 6
+    one = sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, wave_type="P", gmres_tol=1e-11,
+                                               restart=20, dtype=dtype, workers=1)
+    bat = sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, wave_type="P", gmres_tol=1e-11,
+                                               restart=20, dtype=dtype)
+    assert _rel(bat.T_comp_9x9, one.T_comp_9x9) < 1e-9
+    assert _rel(bat.psi_exc, one.psi_exc) < 1e-9
+    # unrestarted (the reference's GPU default) through the batch as well
+    unr = sph.compute_sphere_foldy_lax_fft_gpu(om, R, r, c, n, wave_type="P", gmres_tol=1e-11,
+                                               dtype=dtype)
+    assert _rel(unr.T_comp_9x9, one.T_comp_9x9) < 1e-9
 
 
 def test_two_cells_per_edge_equals_the_resonance_solver(hip_device):
