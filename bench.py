@@ -133,18 +133,36 @@ def rpp_of(plan, x, mat, omega):
     return slab_rpp_periodic(res, p=P_SLOW)
 
 
+def working_launches(durations_ms):
+    """(count, mean ms) of the launches that did the pass's work: a launch
+    queued after its cycle converged finds the done flag raised and returns at
+    once (a few microseconds), so launches shorter than a tenth of the longest
+    are left out (they move no data)."""
+    if not durations_ms:
+        return 0, 0.0
+    cut = 0.1 * max(durations_ms)
+    work = [d for d in durations_ms if d >= cut]
+    return len(work), float(np.mean(work))
+
+
 def time_pass3(plan, psi0, precision, steps):
     """Per-launch duration of the block-multiply pass from HIP events recorded
-    around each launch on its stream, over `steps` solves."""
+    around each launch on its stream, over `steps` solves:
+    {"c64"|"c128": {"launches", "working", "no_op", "avg_ms"}}, the mean over
+    the working launches (working_launches)."""
     import torch
 
     plan.timing(True)
     for _ in range(steps):
         plan.gmres(psi0, "born", tol=TOL, max_iter=600, restart=RESTART, precision=precision)
     torch.cuda.synchronize()
-    tim = plan.timing()
+    out = {}
+    for key in ("c64", "c128"):
+        d = plan.timing_durations(key)
+        nw, avg = working_launches(d)
+        out[key] = {"launches": len(d), "working": nw, "no_op": len(d) - nw, "avg_ms": avg}
     plan.timing(False)
-    return tim
+    return out
 
 
 def sweep_cases():
@@ -202,6 +220,14 @@ def time_sweep(mode: str = "batched", workers: int = 4):
     return out
 
 
+def z_kernel_name(code: int, real: str = "float") -> str:
+    """Kernel of pass 3 that a z tile code selects (Q, + 256 persistent
+    direct-to-LDS form, + 512 pruned form)."""
+    q = code & 255
+    form = "k_z_mult_pr" if code & 512 else ("k_z_mult_dma" if code & 256 else "k_z_mult")
+    return f"{form}<{real},64,{q}>"
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of the roofline kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (scaled by the same counters of a copy of
@@ -210,14 +236,14 @@ def measured_traffic(kernel: str):
                         "k_z_mult_c64_128x128x32_traffic.txt")
     if not os.path.exists(path):
         return None
-    found, val = False, None
+    cur, vals = None, {}
     with open(path) as f:
         for line in f:
             if line.startswith("kernel ="):
-                found = line.split("=", 1)[1].strip() == kernel
-            if line.startswith("traffic_bytes_per_launch"):
-                val = int(line.split("=")[1])
-    return val if found else None
+                cur = line.split("=", 1)[1].strip()
+            elif line.startswith("traffic_bytes_per_launch") and cur is not None:
+                vals[cur] = int(line.split("=")[1])
+    return vals.get(kernel)
 
 
 def time_resonance(n_sub: int = 4, count: int = 1024):
@@ -627,15 +653,15 @@ def main():
     tim64 = time_pass3(plan, psi0, "fp64", args.steps)
 
     # roofline of the block-multiply pass (pass 3) in the headline (complex64) solves
-    n64, ms64 = timmx["c64"]
+    n64, n64_noop = timmx["c64"]["working"], timmx["c64"]["no_op"]
     nuv, nc = plan.NUV, plan.NC
     s = 8  # bytes per complex64
     spectra = NZ * plan.Pxy * plan.Pxy * 9 * s  # read and written once
     table = nuv * 45 * nc * s
     bytes_pass3 = 2 * spectra + table
-    avg_ms = ms64 / max(n64, 1)
+    avg_ms = timmx["c64"]["avg_ms"]
     achieved = bytes_pass3 / (avg_ms * 1e-3) / 1e9
-    kname = f"k_z_mult<float,64,{plan.tiles()['c64'][2]}>"
+    kname = z_kernel_name(plan.tiles()["c64"][2])
     # whole five-pass operator application against the per-matvec bytes of a
     # dense-table formulation, B_mv = s (E P + 2 n + 4 N) with E = 81 or 45
     # table entries per padded point, and against its compulsory bytes (psi
@@ -659,7 +685,8 @@ def main():
                 "frac": round(achieved / 8000.0, 4), "traffic": measured_traffic(kname),
                 "kernel": f"{kname} (z FFT + 9x9 block multiply "
                           "+ inverse z FFT), complex64 launches of the mixed solves",
-                "launches": n64, "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_pass3,
+                "launches": n64, "no_op_launches_excluded": n64_noop,
+                "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bytes_pass3,
                 "whole_matvec": whole}
 
     mean_mx = float(np.mean(tmx))
@@ -720,7 +747,7 @@ def main():
             "psi_rel_err_vs_tight": {k: float((xx - xt).abs().max() / xt.abs().max())
                                      for k, xx in (("fp64", x64), ("mixed", xmx),
                                                    ("learned_mixed", xln))}, "setup_s": round(t_setup, 3),
-            "pass3_c128_avg_us": round(tim64["c128"][1] / max(tim64["c128"][0], 1) * 1e3, 2),
+            "pass3_c128_avg_us": round(tim64["c128"]["avg_ms"] * 1e3, 2),
             "sweep_256_uneven_slabs_mixed": sweep,
             "other_paths": extras,
         },

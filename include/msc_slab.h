@@ -408,6 +408,12 @@ int msc_arnoldi_solution(msc_arnoldi* ctx, const void* V, long long ldv, void* x
  */
 int msc_timing_enable(msc_plan* plan, int enable);
 int msc_timing_read(msc_plan* plan, double* out4);
+/* The individual pass-3 launch durations (ms) of one precision (8: complex64,
+ * 16: complex128) recorded since timing was enabled, in launch order: at most
+ * cap values to out, their number to *n.  Launches that a raised done flag
+ * turned into no-ops are among them (a few microseconds each).  No reference
+ * counterpart. */
+int msc_timing_durations(msc_plan* plan, int elem_bytes, double* out, int cap, int* n);
 
 #ifdef __cplusplus
 }

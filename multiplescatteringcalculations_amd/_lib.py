@@ -79,6 +79,7 @@ SIGNATURES = {
     "msc_arnoldi_solution": (c_int, [c_void_p, c_void_p, c_longlong, c_void_p, c_void_p]),
     "msc_timing_enable": (c_int, [c_void_p, c_int]),
     "msc_timing_read": (c_int, [c_void_p, POINTER(c_double)]),
+    "msc_timing_durations": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(c_int)]),
     # learned preconditioner (include/msc_precond.h)
     "msc_net_create": (c_int, [POINTER(c_void_p), c_int]),
     "msc_net_create_unet": (c_int, [POINTER(c_void_p), POINTER(c_int)]),

@@ -1550,6 +1550,24 @@ int msc_timing_read(msc_plan* plan, double* out4) {
   return kOk;
 }
 
+int msc_timing_durations(msc_plan* plan, int elem_bytes, double* out, int cap, int* n) {
+  MSC_REQUIRE(plan != nullptr && n != nullptr && (out != nullptr || cap == 0) && cap >= 0,
+              "NULL argument");
+  MSC_REQUIRE(elem_bytes == 8 || elem_bytes == 16, "elem_bytes must be 8 or 16");
+  const Plan& p = plan->p;
+  int k = 0;
+  for (size_t i = 0; i + 1 < p.ev_used; i += 2) {
+    if (p.ev_prec[i / 2] != elem_bytes) continue;
+    MSC_HIP_CHECK(hipEventSynchronize(p.ev_pool[i + 1]));
+    float ms = 0.0f;
+    MSC_HIP_CHECK(hipEventElapsedTime(&ms, p.ev_pool[i], p.ev_pool[i + 1]));
+    if (k < cap) out[k] = ms;
+    ++k;
+  }
+  *n = k;
+  return kOk;
+}
+
 // ── learned preconditioner (msc_precond.h) ──────────────────────────────────
 
 int msc_net_create(msc_net** net, int hidden) {

@@ -1126,6 +1126,14 @@ __global__ __launch_bounds__(kNT) void kb_z_mult(const BPass<R>* __restrict__ d,
   z_mult_body<R, PZ, Q>(blockIdx.x, gridDim.x, q.buf2, q.table, q.Nz, Pxy, tw, zmap, q.done);
 }
 
+template <typename R, int PZ, int Q>
+__global__ __launch_bounds__(kNT) void kb_z_mult_pr(const BPass<R>* __restrict__ d, int Pxy,
+                                                   const cx<R>* __restrict__ tw,
+                                                   const int* __restrict__ zmap) {
+  const BPass<R>& q = d[blockIdx.y];
+  z_mult_pruned_body<R, PZ, Q>(blockIdx.x, q.buf2, q.table, q.Nz, Pxy, tw, zmap, q.done);
+}
+
 template <typename R, int P, int Q>
 __global__ __launch_bounds__(kNT) void kb_inv_x(const BPass<R>* __restrict__ d,
                                                const cx<R>* __restrict__ tw) {
@@ -1200,9 +1208,45 @@ static int batch_xy(const Plan& ref, const BPass<R>* d, int nsys, int max_M, int
 template <typename R, int PZ>
 static int batch_z(const Plan& ref, const BPass<R>* d, int nsys, hipStream_t st) {
   const cx<R>* tw = PlanBufs<R>::tw(ref);
-  int q = ref.tile[prec_of<R>()][2] & (kZDma - 1);
+  const int code = ref.tile[prec_of<R>()][2];
+  int q = code & (kZDma - 1);
   while (q > 1 && ref.Pxy % q != 0) q >>= 1;
   const int P = ref.Pxy;
+  if (code & kZPrune) {
+    switch (q) {
+      case 16:
+      case 8:
+        if constexpr (lds_fits<R, PZ, 9, 8>()) {
+          hipLaunchKernelGGL((kb_z_mult_pr<R, PZ, 8>), dim3(P * (P / 8), nsys), dim3(kNT), 0, st,
+                             d, P, tw, ref.zmap + zmap_offset(P, 8));
+          break;
+        }
+        [[fallthrough]];
+      case 4:
+        if constexpr (lds_fits<R, PZ, 9, 4>()) {
+          if (P % 4 == 0) {
+            hipLaunchKernelGGL((kb_z_mult_pr<R, PZ, 4>), dim3(P * (P / 4), nsys), dim3(kNT), 0,
+                               st, d, P, tw, ref.zmap + zmap_offset(P, 4));
+            break;
+          }
+        }
+        [[fallthrough]];
+      case 2:
+        if constexpr (lds_fits<R, PZ, 9, 2>()) {
+          if (P % 2 == 0) {
+            hipLaunchKernelGGL((kb_z_mult_pr<R, PZ, 2>), dim3(P * (P / 2), nsys), dim3(kNT), 0,
+                               st, d, P, tw, ref.zmap + zmap_offset(P, 2));
+            break;
+          }
+        }
+        [[fallthrough]];
+      default:
+        hipLaunchKernelGGL((kb_z_mult_pr<R, PZ, 1>), dim3(P * P, nsys), dim3(kNT), 0, st, d, P,
+                           tw, ref.zmap);
+    }
+    MSC_LAUNCH_CHECK();
+    return kOk;
+  }
   switch (q) {
     case 16:
     case 8:

@@ -366,6 +366,17 @@ class SlabPlan:
         arr = (ctypes.c_int * 10)(*[int(v) for v in tuple(c128) + tuple(c64)])
         _lib.check(self.lib.msc_plan_set_tiles(self._h, arr))
 
+    def timing_durations(self, precision: str = "c64") -> list[float]:
+        """Every recorded pass-3 launch duration (ms) of one precision, in
+        launch order (no-op launches of a finished cycle included)."""
+        eb = {"c64": 8, "c128": 16}[precision]
+        n = ctypes.c_int(0)
+        _lib.check(self.lib.msc_timing_durations(self._h, eb, None, 0, ctypes.byref(n)))
+        out = (ctypes.c_double * max(n.value, 1))()
+        _lib.check(self.lib.msc_timing_durations(self._h, eb, ctypes.cast(out, ctypes.c_void_p),
+                                                 n.value, ctypes.byref(n)))
+        return [float(out[i]) for i in range(n.value)]
+
     def timing(self, enable: bool | None = None) -> dict:
         """Enable/disable (and reset) the HIP-event timing of the block-multiply
         pass, or read it: {"c64": (launches, ms), "c128": (launches, ms)}."""

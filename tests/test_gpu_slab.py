@@ -318,20 +318,25 @@ def test_full_size_operator_properties(hip_device):
 
 
 ZDMA = 256  # z tile code flag of the persistent direct-to-LDS z pass
+ZPRUNE = 512  # z tile code flag of the pruned, HBM-direct z pass
 
 
+@pytest.mark.parametrize("form", [ZDMA, ZPRUNE])
 @pytest.mark.parametrize("q", [2, 4, 8])
 @pytest.mark.parametrize("M,Nz,periodic", [(9, 4, False), (7, 3, False), (8, 5, True),
-                                           (16, 1, False)])
-def test_persistent_z_pass_matches_oracle(hip_device, q, M, Nz, periodic):
+                                           (16, 1, False), (5, 2, False)])
+def test_persistent_z_pass_matches_oracle(hip_device, form, q, M, Nz, periodic):
     """The persistent z pass (tile code Q | 256: the next tile's rows prefetched
-    into LDS by direct-to-LDS loads, z FFTs pruned to the Nz input and output
-    rows) gives the oracle's operator, with Nz at and below half the padded z
-    length, Pz from 2 to 16, periodic and aperiodic."""
+    into LDS by direct-to-LDS loads) and the pruned one (Q | 512: the first
+    forward pass reads the non-zero rows from HBM, the last inverse pass writes
+    the kept rows back), both with z FFTs pruned to the Nz input and output
+    rows, give the oracle's operator, with Nz at and below half the padded z
+    length, Pz from 2 to 16 (radix 2, 4 and 8 first passes), periodic and
+    aperiodic."""
     ref, om = REF_C, 692.8
     Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=M + 31 * Nz)
     plan, *_ = _plan(M, Nz, 1.0, ref, om, Dl, Dm, Dr, periodic=periodic)
-    tiles = (1, 2, q | ZDMA, 2, 1)
+    tiles = (1, 2, q | form, 2, 1)
     plan.set_tiles(tiles, tiles)
     T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, om, ref)
     kh = osl.slab_kernels(M, Nz, 1.0, om, ref, periodic=periodic)
@@ -357,11 +362,12 @@ def test_persistent_z_pass_full_size(hip_device):
     gen = torch.Generator(device=hip_device).manual_seed(3)
     x = torch.randn(plan.n, dtype=torch.complex128, device=hip_device, generator=gen)
     out = {}
-    for code in (4, 4 | ZDMA, 2 | ZDMA, 8 | ZDMA):
+    codes = (4 | ZDMA, 2 | ZDMA, 8 | ZDMA, 2 | ZPRUNE, 4 | ZPRUNE, 8 | ZPRUNE)
+    for code in (4,) + codes:
         plan.set_tiles((1, 1, code, 1, 1), (1, 2, code, 2, 1))
         out[code] = (plan.matvec(x), plan.matvec(x.to(torch.complex64)).to(torch.complex128))
     base = x - out[4][0]
-    for code in (4 | ZDMA, 2 | ZDMA, 8 | ZDMA):
+    for code in codes:
         assert float(((x - out[code][0]) - base).abs().max() / base.abs().max()) < 1e-13
         assert float(((x - out[code][1]) - base).norm() / base.norm()) < 1e-5
     b = plan.incident_field(0.4 * REF_C.beta, r, [0.893, 0.45, 0.0], "P").reshape(-1)
@@ -412,7 +418,8 @@ def test_autotune_and_graph_solves_are_reproducible(hip_device):
     t = plan.tiles()
     for tiles in (t["c128"], t["c64"]):
         assert all(v in (1, 2, 4) for v in tiles[:2] + tiles[3:])
-        assert tiles[2] in (1, 2, 4, 8, 16, 2 | ZDMA, 4 | ZDMA, 8 | ZDMA)
+        assert tiles[2] in (1, 2, 4, 8, 16, 2 | ZDMA, 4 | ZDMA, 8 | ZDMA, 2 | ZPRUNE, 4 | ZPRUNE,
+                            8 | ZPRUNE)
     psi0 = plan.incident_field(0.4 * REF_C.beta, r, [0.893, 0.45, 0.0], "P")
     outs = []
     for prec in ("fp64", "mixed", "fp64", "mixed"):
