@@ -324,14 +324,15 @@ ZPRUNE = 512  # z tile code flag of the pruned, HBM-direct z pass
 @pytest.mark.parametrize("form", [ZDMA, ZPRUNE])
 @pytest.mark.parametrize("q", [2, 4, 8])
 @pytest.mark.parametrize("M,Nz,periodic", [(9, 4, False), (7, 3, False), (8, 5, True),
-                                           (16, 1, False), (5, 2, False)])
+                                           (16, 1, False), (5, 2, False), (5, 17, False),
+                                           (4, 32, True), (3, 20, False)])
 def test_persistent_z_pass_matches_oracle(hip_device, form, q, M, Nz, periodic):
     """The persistent z pass (tile code Q | 256: the next tile's rows prefetched
     into LDS by direct-to-LDS loads) and the pruned one (Q | 512: the first
     forward pass reads the non-zero rows from HBM, the last inverse pass writes
     the kept rows back), both with z FFTs pruned to the Nz input and output
     rows, give the oracle's operator, with Nz at and below half the padded z
-    length, Pz from 2 to 16 (radix 2, 4 and 8 first passes), periodic and
+    length, Pz from 2 to 64 (radix 2, 4 and 8 first passes), periodic and
     aperiodic."""
     ref, om = REF_C, 692.8
     Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=M + 31 * Nz)
