@@ -120,119 +120,262 @@ __global__ __launch_bounds__(kRT) void k_res_assemble(double2* __restrict__ Aall
 // One workgroup (kLT threads) per system: [A | Y] (column-major, lda n, nc
 // columns) is reduced in place to [U | Y'] by right-looking blocked
 // elimination with partial pivoting, then the 9 solution columns overwrite Y.
-// One dynamic LDS region of NB * max(n, nc) complex holds, per panel of NB
-// columns, first the panel (factorised in LDS), then the U12 block; L11 is
-// copied to a small array and each thread reads its rows of L21 from global
-// memory (just written, L2-resident) into registers for the trailing update.
-// Per pivot: the maxima of the next column are found while the current one is
-// applied, so two barriers per column.  rhs_lds: the back substitution keeps
-// the right-hand sides in LDS.
+// Panels of 16 columns (fewer for the larger systems, whose panels must still
+// fit in LDS) are factorised in LDS (factor_panel); the back substitution
+// keeps the right-hand sides in LDS when they fit (rhs_lds).  Two kernels
+// share these pieces:
+//   * k_res_lu_blocked (16-column panels): outer panels of K = 16 S columns.
+//     Each 16-column sub-panel updates only the rest of its outer panel; the
+//     columns right of the outer panel receive the K row exchanges, K rows of
+//     U (forward substitution, column chunks through LDS) and one K-deep
+//     trailing update on the matrix cores, so the trailing matrix crosses HBM
+//     once per K columns rather than once per 16.
+//   * k_res_lu (NB = 8, 4, 2, 1): one NB-column panel at a time, U12 in LDS,
+//     the trailing update on the vector units.
 constexpr int kLT = 512;
 constexpr int kLW = kLT / 64;
+
+// Pivot-search and pivot-growth state of one workgroup.
+struct LuShared {
+  double red_v[kLW];
+  int red_i[kLW];
+  double2 prow[16];
+  double umin, umax;
+  int sing;
+};
+
+__device__ __forceinline__ void lu_shared_init(LuShared& sh) {
+  if (threadIdx.x == 0) {
+    sh.umin = INFINITY;
+    sh.umax = 0.0;
+    sh.sing = 0;
+  }
+}
+
+// Factorises the panel of columns k0 .. k0 + nb - 1, rows k0 .. n - 1, in LDS
+// (pan: column j at pan + j * n) with partial pivoting and writes it back to
+// A (unit L below the diagonal, U on and above it).  piv[j] = the panel-local
+// row exchanged with row j (the exchanges are applied to the panel's columns
+// only); L11[i * NB + j] = the panel's unit lower block (row i, column j).
+// Per pivot: the maxima of the next column are found while the current one is
+// applied, so two barriers per column.  Ends with a barrier.
+template <int NB>
+__device__ void factor_panel(double2* __restrict__ A, int n, int k0, int nb,
+                             double2* __restrict__ pan, int* piv, double2* L11, LuShared& sh) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m = n - k0;
+  // 8 loads in flight per thread (indices clamped into the panel, the stores
+  // guarded)
+  const int tot = nb * m;
+#pragma unroll 1
+  for (int e0 = t; e0 < tot; e0 += 8 * kLT) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = min(e0 + u * kLT, tot - 1), j = e / m, i = e - j * m;
+      v[u] = A[(size_t)(k0 + j) * n + k0 + i];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * kLT;
+      if (e < tot) {
+        const int j = e / m, i = e - j * m;
+        pan[j * n + i] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  double best = -1.0;
+  int bi = 0;
+  for (int i = t; i < m; i += kLT) {
+    const double2 v = pan[i];
+    const double a = v.x * v.x + v.y * v.y;
+    if (a > best) {
+      best = a;
+      bi = i;
+    }
+  }
+  for (int j = 0; j < nb; ++j) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      sh.red_v[w] = best;
+      sh.red_i[w] = bi;
+    }
+    __syncthreads();
+    double b = sh.red_v[0];
+    int p = sh.red_i[0];
+    for (int q = 1; q < kLW; ++q)
+      if (sh.red_v[q] > b || (sh.red_v[q] == b && sh.red_i[q] < p)) {
+        b = sh.red_v[q];
+        p = sh.red_i[q];
+      }
+    if (t < nb) sh.prow[t] = pan[t * n + p];
+    if (t == 0) piv[j] = p;
+    __syncthreads();
+    // rows j and p exchange (old row p = prow is the pivot row)
+    if (p != j && t < nb) {
+      pan[t * n + p] = pan[t * n + j];
+      pan[t * n + j] = sh.prow[t];
+    }
+    const double2 dgn = sh.prow[j];
+    const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
+    if (t == 0) {
+      const double ad = sqrt(dd);
+      sh.umin = fmin(sh.umin, ad);
+      sh.umax = fmax(sh.umax, ad);
+      if (!(dd > 0.0)) sh.sing = 1;
+    }
+    const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
+    __syncthreads();
+    // scale column j, update the panel columns right of it, and find the
+    // maxima of column j + 1 on the way
+    best = -1.0;
+    bi = j + 1;
+    for (int i = j + 1 + t; i < m; i += kLT) {
+      const double2 l = zmul(pan[j * n + i], inv);
+      pan[j * n + i] = l;
+      for (int jj = j + 1; jj < nb; ++jj) zfms(pan[jj * n + i], l, sh.prow[jj]);
+      if (j + 1 < nb) {
+        const double2 v = pan[(j + 1) * n + i];
+        const double a = v.x * v.x + v.y * v.y;
+        if (a > best) {
+          best = a;
+          bi = i;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < nb * m; e += kLT) {
+    const int j = e / m, i = e - j * m;
+    A[(size_t)(k0 + j) * n + k0 + i] = pan[j * n + i];
+    if (i < nb) L11[i * NB + j] = pan[j * n + i];
+  }
+  __syncthreads();
+}
+
+// Back substitution U X = Y for the 9 right-hand sides (columns n .. n + 8),
+// in LDS (lds, 9 n complex) when rhs_lds, else in place; then the pivot
+// growth ratio and the singular flag of the system.
+__device__ void back_substitute(double2* __restrict__ A, int n, int rhs_lds, double2* lds,
+                                double2* xs, const LuShared& sh, double* ratio, int* singular) {
+  const int t = threadIdx.x;
+  double2* Y = A + (size_t)n * n;
+  double2* ys = rhs_lds ? lds : Y;
+  if (rhs_lds) {
+    for (int e = t; e < 9 * n; e += kLT) ys[e] = Y[e];
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    if (t < 9) {
+      const double2 dgn = A[(size_t)j * n + j];
+      const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
+      const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
+      const double2 x = zmul(ys[(size_t)t * n + j], inv);
+      ys[(size_t)t * n + j] = x;
+      xs[t] = x;
+    }
+    __syncthreads();
+    const double2* uc = A + (size_t)j * n;
+    for (int e = t; e < 9 * j; e += kLT) {
+      const int q = e / j, i = e - q * j;
+      zfms(ys[(size_t)q * n + i], uc[i], xs[q]);
+    }
+    __syncthreads();
+  }
+  if (rhs_lds)
+    for (int e = t; e < 9 * n; e += kLT) Y[e] = ys[e];
+  if (t == 0) {
+    if (ratio) ratio[blockIdx.x] = sh.umin > 0.0 ? sh.umax / sh.umin : INFINITY;
+    if (singular) singular[blockIdx.x] = sh.sing;
+  }
+}
+
+// Back substitution U X = Y for the 9 right-hand sides (columns n .. n + 8)
+// in LDS (ys, 9 n complex), 16 columns of U at a time from the bottom: the
+// block's own rows by 9 threads (one per right-hand side, no barrier between
+// its 16 steps), then the rows above it updated with the block's 16 columns of
+// U (one thread per row, 16 independent loads) -- one round trip to memory
+// per 16 columns.  Then the pivot growth ratio and the singular flag.
+__device__ void back_substitute_blocked(double2* __restrict__ A, int n, double2* ys,
+                                        const LuShared& sh, double* ratio, int* singular) {
+  __shared__ double2 xb[9 * 16];
+  __shared__ double2 ub[16 * 16];  // the diagonal block, column-major
+  const int t = threadIdx.x;
+  double2* Y = A + (size_t)n * n;
+  for (int e = t; e < 9 * n; e += kLT) ys[e] = Y[e];
+  __syncthreads();
+#pragma unroll 1
+  for (int j0 = (n - 1) / 16 * 16; j0 >= 0; j0 -= 16) {
+    const int nbk = min(16, n - j0);
+    if (t < 256) {
+      const int r = t / 16, i = t % 16;
+      if (r < nbk && i < nbk) ub[t] = A[(size_t)(j0 + r) * n + j0 + i];
+    }
+    __syncthreads();
+    if (t < 9) {
+      double2* y = ys + (size_t)t * n;
+#pragma unroll 1
+      for (int r = nbk - 1; r >= 0; --r) {
+        const double2* uc = ub + r * 16;
+        const double2 dgn = uc[r];
+        const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
+        const double2 inv =
+            dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
+        const double2 x = zmul(y[j0 + r], inv);
+        y[j0 + r] = x;
+        xb[t * 16 + r] = x;
+        for (int i = 0; i < r; ++i) zfms(y[j0 + i], uc[i], x);
+      }
+      for (int r = nbk; r < 16; ++r) xb[t * 16 + r] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int i = t; i < j0; i += kLT) {
+      double2 u[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) u[r] = A[(size_t)(j0 + min(r, nbk - 1)) * n + i];
+#pragma unroll 1
+      for (int q = 0; q < 9; ++q) {
+        double2 acc = ys[(size_t)q * n + i];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) zfms(acc, u[r], xb[q * 16 + r]);  // zeros past nbk
+        ys[(size_t)q * n + i] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < 9 * n; e += kLT) Y[e] = ys[e];
+  if (t == 0) {
+    if (ratio) ratio[blockIdx.x] = sh.umin > 0.0 ? sh.umax / sh.umin : INFINITY;
+    if (singular) singular[blockIdx.x] = sh.sing;
+  }
+}
+
 template <int NB>
 __global__ __launch_bounds__(kLT) void k_res_lu(double2* __restrict__ Aall, size_t stride, int n,
                                                int nc, int rhs_lds, double* __restrict__ ratio,
                                                int* __restrict__ singular) {
   extern __shared__ double2 lds[];
-  __shared__ double red_v[kLW];
-  __shared__ int red_i[kLW];
+  __shared__ LuShared sh;
   __shared__ int piv[NB];
   __shared__ double2 L11[NB * NB];
-  __shared__ double2 prow[NB];
-  __shared__ double umin, umax;
-  __shared__ int sing;
   __shared__ double2 xs[9];
   double2* A = Aall + (size_t)blockIdx.x * stride;
   double2* pan = lds;  // panel: column j at pan + j * n (local rows)
   double2* U = lds;    // then U12: column c - c0 at U + (c - c0) * NB
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) {
-    umin = INFINITY;
-    umax = 0.0;
-    sing = 0;
-  }
+  lu_shared_init(sh);
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = min(NB, n - k0);
-    const int m = n - k0;
-    for (int e = t; e < nb * m; e += kLT) {
-      const int j = e / m, i = e - j * m;
-      pan[j * n + i] = A[(size_t)(k0 + j) * n + k0 + i];
-    }
-    __syncthreads();
-    double best = -1.0;
-    int bi = 0;
-    for (int i = t; i < m; i += kLT) {
-      const double2 v = pan[i];
-      const double a = v.x * v.x + v.y * v.y;
-      if (a > best) {
-        best = a;
-        bi = i;
-      }
-    }
-    for (int j = 0; j < nb; ++j) {
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (lane == 0) {
-        red_v[w] = best;
-        red_i[w] = bi;
-      }
-      __syncthreads();
-      double b = red_v[0];
-      int p = red_i[0];
-      for (int q = 1; q < kLW; ++q)
-        if (red_v[q] > b || (red_v[q] == b && red_i[q] < p)) {
-          b = red_v[q];
-          p = red_i[q];
-        }
-      if (t < nb) prow[t] = pan[t * n + p];
-      if (t == 0) piv[j] = p;
-      __syncthreads();
-      // rows j and p exchange (old row p = prow is the pivot row)
-      if (p != j && t < nb) {
-        pan[t * n + p] = pan[t * n + j];
-        pan[t * n + j] = prow[t];
-      }
-      const double2 dgn = prow[j];
-      const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
-      if (t == 0) {
-        const double ad = sqrt(dd);
-        umin = fmin(umin, ad);
-        umax = fmax(umax, ad);
-        if (!(dd > 0.0)) sing = 1;
-      }
-      const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
-      __syncthreads();
-      // scale column j, update the panel columns right of it, and find the
-      // maxima of column j + 1 on the way
-      best = -1.0;
-      bi = j + 1;
-      for (int i = j + 1 + t; i < m; i += kLT) {
-        const double2 l = zmul(pan[j * n + i], inv);
-        pan[j * n + i] = l;
-        for (int jj = j + 1; jj < nb; ++jj) zfms(pan[jj * n + i], l, prow[jj]);
-        if (j + 1 < nb) {
-          const double2 v = pan[(j + 1) * n + i];
-          const double a = v.x * v.x + v.y * v.y;
-          if (a > best) {
-            best = a;
-            bi = i;
-          }
-        }
-      }
-    }
-    __syncthreads();
-    for (int e = t; e < nb * m; e += kLT) {
-      const int j = e / m, i = e - j * m;
-      A[(size_t)(k0 + j) * n + k0 + i] = pan[j * n + i];
-      if (i < nb) L11[i * NB + j] = pan[j * n + i];
-    }
-    __syncthreads();
+    factor_panel<NB>(A, n, k0, nb, pan, piv, L11, sh);
     // swaps and U12 = L11^-1 A12 for the columns right of the panel
     const int c0 = k0 + nb;
 #pragma unroll 1
@@ -301,36 +444,234 @@ __global__ __launch_bounds__(kLT) void k_res_lu(double2* __restrict__ Aall, size
     }
     __syncthreads();
   }
-  // back substitution U X = Y for the 9 right-hand sides (columns n..n+8)
-  double2* Y = A + (size_t)n * n;
-  double2* ys = rhs_lds ? lds : Y;
-  if (rhs_lds) {
-    for (int e = t; e < 9 * n; e += kLT) ys[e] = Y[e];
-    __syncthreads();
-  }
-  for (int j = n - 1; j >= 0; --j) {
-    if (t < 9) {
-      const double2 dgn = A[(size_t)j * n + j];
-      const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
-      const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
-      const double2 x = zmul(ys[(size_t)t * n + j], inv);
-      ys[(size_t)t * n + j] = x;
-      xs[t] = x;
+  back_substitute(A, n, rhs_lds, lds, xs, sh, ratio, singular);
+}
+
+typedef double msc_v4d __attribute__((ext_vector_type(4)));
+
+// Rows kb .. kb + kact - 1 of U for the columns cs .. ce - 1 (one thread per
+// column): the row exchanges pv[0 .. kact) (absolute rows, applied in order),
+// then forward substitution with the unit lower block L (LDS, L[i * ldl + j]
+// = row kb + i, column kb + j; its rows from kact up to the next multiple of
+// 16 must hold zeros) in 16-row blocks, one block of x in registers and the
+// finished blocks read back from U.  The rows are written to A and,
+// row-major with leading dimension cw (column c at U + c - cs), to U in LDS
+// (U's rows from kact up to the next multiple of 16 receive zeros).  No
+// branch around a load: the loops run whole blocks, the zero rows of L keep
+// the rows past kact at zero.
+__device__ void solve_u_rows(double2* __restrict__ A, int n, int kb, int kact, const int* pv,
+                             const double2* L, int ldl, double2* U, int cw, int cs, int ce) {
+#pragma unroll 1
+  for (int c = cs + (int)threadIdx.x; c < ce; c += kLT) {
+    double2* col = A + (size_t)c * n;
+    for (int j = 0; j < kact; ++j) {
+      const int p = pv[j];
+      if (p != kb + j) {
+        const double2 a = col[kb + j];
+        col[kb + j] = col[p];
+        col[p] = a;
+      }
     }
-    __syncthreads();
-    const double2* uc = A + (size_t)j * n;
-    for (int e = t; e < 9 * j; e += kLT) {
-      const int q = e / j, i = e - q * j;
-      zfms(ys[(size_t)q * n + i], uc[i], xs[q]);
+    double2* u = U + (c - cs);
+#pragma unroll 1
+    for (int rb = 0; rb < kact; rb += 16) {
+      const int nbb = min(16, kact - rb);
+      double2 x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const double2 v = col[kb + rb + min(i, nbb - 1)];
+        x[i] = i < nbb ? v : make_double2(0.0, 0.0);
+      }
+      // the finished blocks (16 rows each, all below kact)
+#pragma unroll 1
+      for (int jb = 0; jb < rb; jb += 16) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const double2 y = u[(size_t)(jb + j) * cw];
+          const double2* lj = L + (size_t)rb * ldl + jb + j;
+#pragma unroll
+          for (int i = 0; i < 16; ++i) zfms(x[i], lj[(size_t)i * ldl], y);
+          // one column of the block's L live at a time
+          asm volatile("" ::: "memory");
+        }
+      }
+#pragma unroll
+      for (int i = 1; i < 16; ++i) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+          if (j < i) zfms(x[i], L[(size_t)(rb + i) * ldl + rb + j], x[j]);
+        // x[i] is final here and the next row's L reads stay below: one row
+        // of the block live at a time (all of it at once spills)
+        asm volatile("" : "+v"(x[i].x), "+v"(x[i].y) : : "memory");
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        if (i < nbb) col[kb + rb + i] = x[i];
+        u[(size_t)(rb + i) * cw] = x[i];
+      }
     }
-    __syncthreads();
   }
-  if (rhs_lds)
-    for (int e = t; e < 9 * n; e += kLT) Y[e] = ys[e];
-  if (t == 0) {
-    if (ratio) ratio[blockIdx.x] = umin > 0.0 ? umax / umin : INFINITY;
-    if (singular) singular[blockIdx.x] = sing;
+}
+
+// A's rows r0 .. n - 1, columns cs .. ce - 1 -= L W on the matrix cores, with
+// L = A's columns kb .. kb + kact - 1 of those rows (kact <= KM) and W = rows
+// 0 .. kact - 1 of U (LDS, leading dimension cw, column c at U + c - cs).
+// 16 x 16 complex tiles in row-major order, each wave taking one contiguous
+// run of them: a wave keeps its tile row of L in registers while it moves
+// along the columns and reloads it only when its run enters the next row.  Complex products as four real chains
+// of v_mfma_f64_16x16x4f64 (Lr Ur, Li Ui, Lr Ui, Li Ur; independent
+// accumulators, so the chains interleave without waiting on each other),
+// combined at the end.  Lane l (i = l % 16, q = l / 16) feeds L[i][4 s + q]
+// and W[4 s + q][i] at step s and receives rows q + 4 v (v < 4) of column i
+// of the product (the result layout of the instruction on gfx950); entries
+// past kact, n or ce are fed as zeros and not written.  C is loaded two
+// tiles ahead (8 KB per wave in flight under the chains).  The tile loop is
+// wave-uniform (the instruction uses every lane).
+__device__ __forceinline__ void ctile_load(const double2* __restrict__ A, int n, int ce, int row0,
+                                           int col, double2 (&cv)[4]) {
+  // clamped into the tile grid's last row / column: entries past n or ce are
+  // loaded but never stored
+  const double2* cc = A + (size_t)min(col, ce - 1) * n;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) cv[v] = cc[min(row0 + 4 * v, n - 1)];
+}
+
+template <int KM>
+__device__ void trailing_mfma(double2* __restrict__ A, int n, int kb, int kact,
+                              const double2* __restrict__ U, int cw, int r0, int cs, int ce) {
+  constexpr int KS = KM / 4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i16 = lane & 15, q = lane >> 4;
+  const int tr = (n - r0 + 15) / 16, tcn = (ce - cs + 15) / 16, total = tr * tcn;
+  const int ns = (kact + 3) / 4;
+  const int t_beg = (int)((long)total * w / kLW), t_end = (int)((long)total * (w + 1) / kLW);
+  int cur = -1;
+  double lr[KS], li[KS];
+  double2 c1[4], c2[4];
+  if (t_beg < t_end)
+    ctile_load(A, n, ce, r0 + (t_beg / tcn) * 16 + q, cs + (t_beg % tcn) * 16 + i16, c1);
+  if (t_beg + 1 < t_end) {
+    const int tt = t_beg + 1;
+    ctile_load(A, n, ce, r0 + (tt / tcn) * 16 + q, cs + (tt % tcn) * 16 + i16, c2);
   }
+#pragma unroll 1
+  for (int tile = t_beg; tile < t_end; ++tile) {
+    const int ti = tile / tcn, tj = tile - ti * tcn;
+    if (ti != cur) {
+      cur = ti;
+      const int rowL = r0 + ti * 16 + i16;
+      const bool rin = rowL < n;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + q;
+        const double2 l = A[(size_t)(kb + min(k, kact - 1)) * n + min(rowL, n - 1)];
+        const bool in = rin && k < kact;
+        lr[s] = in ? l.x : 0.0;
+        li[s] = in ? l.y : 0.0;
+      }
+    }
+    const int nt = tile + 2;
+    double2 c3[4];
+    if (nt < t_end) ctile_load(A, n, ce, r0 + (nt / tcn) * 16 + q, cs + (nt % tcn) * 16 + i16, c3);
+    const int colU = tj * 16 + i16;
+    msc_v4d rr = {0.0, 0.0, 0.0, 0.0}, ii = rr, ri = rr, ir = rr;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      if (s < ns) {
+        // colU < cw (ce - cs <= cw, a multiple of 16) and k < 4 ns: W's rows
+        // up to the next multiple of 16, zero past kact
+        const int k = 4 * s + q;
+        const double2 u = U[(size_t)k * cw + colU];
+        rr = __builtin_amdgcn_mfma_f64_16x16x4f64(lr[s], u.x, rr, 0, 0, 0);
+        ii = __builtin_amdgcn_mfma_f64_16x16x4f64(li[s], u.y, ii, 0, 0, 0);
+        ri = __builtin_amdgcn_mfma_f64_16x16x4f64(lr[s], u.y, ri, 0, 0, 0);
+        ir = __builtin_amdgcn_mfma_f64_16x16x4f64(li[s], u.x, ir, 0, 0, 0);
+      }
+    }
+    const int col = cs + tj * 16 + i16, row0 = r0 + ti * 16 + q;
+    if (col < ce) {
+      double2* cc = A + (size_t)col * n + row0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (row0 + 4 * v < n)
+          cc[4 * v] = make_double2(c1[v].x - (rr[v] - ii[v]), c1[v].y - (ri[v] + ir[v]));
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      c1[v] = c2[v];
+      c2[v] = c3[v];
+    }
+  }
+}
+
+// lds_entries: the dynamic LDS in complex entries, at least 16 n (the panel)
+// and K * K + 16 K (the outer panel's L11 and a U chunk of 16 columns).
+template <int S>
+__global__ __launch_bounds__(kLT) void k_res_lu_blocked(double2* __restrict__ Aall, size_t stride,
+                                                       int n, int nc, int lds_entries,
+                                                       double* __restrict__ ratio,
+                                                       int* __restrict__ singular) {
+  constexpr int K = 16 * S;
+  extern __shared__ double2 lds[];
+  __shared__ LuShared sh;
+  __shared__ int piv[16];
+  __shared__ int pva[K];  // the outer panel's exchanges: row k0 + j <-> pva[j]
+  __shared__ double2 L16[16 * 16];
+  double2* A = Aall + (size_t)blockIdx.x * stride;
+  const int t = threadIdx.x;
+  const int cw = ((lds_entries - K * K) / K) & ~15;  // U chunk width of the far columns
+  lu_shared_init(sh);
+#pragma unroll 1
+  for (int k0 = 0; k0 < n; k0 += K) {
+    const int kact = min(K, n - k0);
+#pragma unroll 1
+    for (int p0 = k0; p0 < k0 + kact; p0 += 16) {
+      const int nb = min(16, k0 + kact - p0);
+      if (nb < 16 && t < 256) L16[t] = make_double2(0.0, 0.0);  // rows past nb stay zero
+      factor_panel<16>(A, n, p0, nb, lds, piv, L16, sh);
+      if (t < nb) pva[p0 - k0 + t] = p0 + piv[t];
+      // this sub-panel's exchanges on the outer panel's earlier L columns
+      for (int c = k0 + t; c < p0; c += kLT) {
+        double2* col = A + (size_t)c * n;
+        for (int j = 0; j < nb; ++j) {
+          const int p = p0 + piv[j];
+          if (p != p0 + j) {
+            const double2 a = col[p0 + j];
+            col[p0 + j] = col[p];
+            col[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+      // the outer panel's later columns: 16 rows of U, then a 16-deep update
+      const int cs = p0 + nb, ce = k0 + kact;
+      if (cs < ce) {
+        solve_u_rows(A, n, p0, nb, pva + (p0 - k0), L16, 16, lds, K, cs, ce);
+        __syncthreads();
+        trailing_mfma<16>(A, n, p0, nb, lds, K, cs, cs, ce);
+        __syncthreads();
+      }
+    }
+    // the columns right of the outer panel, in chunks of cw: kact rows of U,
+    // then a kact-deep update of rows f0 .. n - 1
+    const int f0 = k0 + kact;
+    if (f0 < nc) {
+      for (int e = t; e < K * K; e += kLT) {
+        const int i = e / K, j = e - i * K;
+        lds[e] = (i < kact && j < i) ? A[(size_t)(k0 + j) * n + k0 + i] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+#pragma unroll 1
+      for (int cs = f0; cs < nc; cs += cw) {
+        const int ce = min(cs + cw, nc);
+        solve_u_rows(A, n, k0, kact, pva, lds, K, lds + K * K, cw, cs, ce);
+        __syncthreads();
+        trailing_mfma<K>(A, n, k0, kact, lds + K * K, cw, f0, cs, ce);
+        __syncthreads();
+      }
+    }
+  }
+  back_substitute_blocked(A, n, lds, sh, ratio, singular);
 }
 
 // T_comp[b] (row-major 9x9) = T_loc sum_m X_m, X the 9 solution columns;
@@ -518,17 +859,41 @@ __global__ __launch_bounds__(64) void k_res_incident(double2* __restrict__ psi, 
 size_t lu_lds(int nb, int n, int nc) { return (size_t)nb * std::max(n, nc) * sizeof(double2); }
 constexpr size_t kLuLdsCap = 150 * 1024;  // leaves room for the static arrays (NB = 16)
 
+template <int S>
+int launch_lu_blocked(double2* A, size_t stride, int n, int nc, int count, double* ratio,
+                      int* sing, hipStream_t st) {
+  constexpr int K = 16 * S;
+  const int entries = std::max(16 * n, K * K + 16 * K);
+  const size_t lds = (size_t)entries * sizeof(double2);
+  if (lds > kLuLdsCap) {
+    set_error("resonance system too large for the blocked elimination");
+    return kUnsupported;
+  }
+  MSC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_lu_blocked<S>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_res_lu_blocked<S>, dim3(count), dim3(kLT), lds, st, A, stride, n, nc,
+                     entries, ratio, sing);
+  MSC_LAUNCH_CHECK();
+  return kOk;
+}
+
 template <int NB>
 int launch_lu(double2* A, size_t stride, int n, int nc, int count, double* ratio, int* sing,
               hipStream_t st) {
-  const size_t lds = lu_lds(NB, n, nc);
-  const int rhs_lds = (size_t)9 * n * sizeof(double2) <= lds ? 1 : 0;
-  MSC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_lu<NB>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_res_lu<NB>, dim3(count), dim3(kLT), lds, st, A, stride, n, nc, rhs_lds,
-                     ratio, sing);
-  MSC_LAUNCH_CHECK();
-  return kOk;
+  if constexpr (NB == 16) {
+    // outer panels of 32 columns (64 measured slower at n_sub = 4: its
+    // 64-row forward substitutions run on fewer threads per column chunk)
+    return launch_lu_blocked<2>(A, stride, n, nc, count, ratio, sing, st);
+  } else {
+    const size_t lds = lu_lds(NB, n, nc);
+    const int rhs_lds = (size_t)9 * n * sizeof(double2) <= lds ? 1 : 0;
+    MSC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_res_lu<NB>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_res_lu<NB>, dim3(count), dim3(kLT), lds, st, A, stride, n, nc, rhs_lds,
+                       ratio, sing);
+    MSC_LAUNCH_CHECK();
+    return kOk;
+  }
 }
 
 }  // namespace
@@ -570,8 +935,15 @@ extern "C" int msc_resonance_tmatrices(double omega, double a, double alpha, dou
   if (rc) return rc;
   const size_t stride = (size_t)n9 * nc;
   const size_t budget = (size_t)4 << 30;
-  const int chunk = (int)std::max<size_t>(
+  int chunk = (int)std::max<size_t>(
       1, std::min<size_t>(std::min<size_t>(count, 65535), budget / (stride * 16)));
+  // whole rounds of one system per CU: the last round of a chunk with a few
+  // systems left over would run the chip at a fraction of its width
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      cus > 0 && chunk > cus)
+    chunk = chunk / cus * cus;
   const int no = 2 * n - 1;
   double2 *dP = nullptr, *dpsi = nullptr, *dtp = nullptr, *dA = nullptr, *dt0 = nullptr,
           *dt1 = nullptr, *dacc = nullptr;
