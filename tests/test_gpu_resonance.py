@@ -33,6 +33,8 @@ def _rel(a, b):
     (2, 30.0, 2 * np.pi * 40.0, (-4e9, 3e9, -300.0), "P", [0.4, -0.2, 1.0]),
     (3, 20.0, 2 * np.pi * 45.0, (6e9, -2e9, 400.0), "S", [1.0, 1.0, 0.5]),
     (4, 25.0, 2 * np.pi * 50.0, (3e9, 2e9, 250.0), "P", None),
+    # 9N = 1125: the 8-column panels of the unblocked elimination
+    (5, 15.0, 2 * np.pi * 30.0, (2e9, -1e9, 150.0), "S", [0.3, 0.5, 1.0]),
 ])
 def test_single_cube_matches_oracle(hip_device, n, a, om, contrast, wave, kh):
     msc, rt = _msc()
