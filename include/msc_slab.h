@@ -171,9 +171,9 @@ const char* msc_version(void);
 
 /*
  * Plan for an M x M x N_z slab of cubes of half-width a (side d = 2a).
- * pz, pxy: padded FFT lengths (powers of two); 0 picks the smallest power of
- * two >= 2N_z-1 (z) and >= 2M-1 (x, y), or pxy = M when periodic and M is a
- * power of two.  Periodic mode (circular lateral convolution, any M) runs on
+ * pz, pxy: padded FFT lengths (powers of two, pz <= 512 and pxy <= 1024, so
+ * N_z <= 256 and M <= 512); 0 picks the smallest power of two >= 2N_z-1 (z)
+ * and >= 2M-1 (x, y), or pxy = M when periodic and M is a power of two.  Periodic mode (circular lateral convolution, any M) runs on
  * the M-point lateral grid when pxy == M (M a power of two), otherwise on the
  * padded grid (pxy >= 2M-1) with the periodically folded kernel.
  * Replaces the geometry handling of _build_slab_kernels_gpu /

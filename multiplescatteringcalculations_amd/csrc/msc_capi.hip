@@ -1056,18 +1056,18 @@ int msc_plan_create(msc_plan** plan, int M, int N_z, double a, int periodic, int
   if (pz == 0) pz = std::max(2, next_pow2(2 * N_z - 1));
   if (pxy == 0)
     pxy = periodic && is_pow2(M) && M >= 2 ? M : std::max(2, next_pow2(2 * M - 1));
-  MSC_REQUIRE(is_pow2(pz) && pz >= 2 * N_z - 1 && pz >= 2 && pz <= 256,
-              "pz must be a power of two in [max(2, 2N_z-1), 256]");
+  MSC_REQUIRE(is_pow2(pz) && pz >= 2 * N_z - 1 && pz >= 2 && pz <= 512,
+              "pz must be a power of two in [max(2, 2N_z-1), 512]");
   if (periodic && pxy == M) {
     // circular lateral grid: the folded kernel on the M-point grid
-    if (!(is_pow2(M) && M >= 2 && M <= 512)) {
-      set_error("periodic mode on the M-point grid needs M to be a power of two in [2, 512]");
+    if (!(is_pow2(M) && M >= 2 && M <= 1024)) {
+      set_error("periodic mode on the M-point grid needs M to be a power of two in [2, 1024]");
       return kUnsupported;
     }
   } else {
     // aperiodic, or periodic on a zero-padded grid with the folded kernel (any M)
-    MSC_REQUIRE(is_pow2(pxy) && pxy >= 2 * M - 1 && pxy >= 2 && pxy <= 512,
-                "pxy must be a power of two in [max(2, 2M-1), 512]");
+    MSC_REQUIRE(is_pow2(pxy) && pxy >= 2 * M - 1 && pxy >= 2 && pxy <= 1024,
+                "pxy must be a power of two in [max(2, 2M-1), 1024]");
   }
   msc_plan* pl = new (std::nothrow) msc_plan();
   MSC_REQUIRE(pl != nullptr, "out of host memory");

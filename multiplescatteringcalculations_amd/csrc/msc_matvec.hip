@@ -1022,6 +1022,7 @@ static int launch_one_xy(Plan& p, int pass, const cx<R>* x, cx<R>* out, hipStrea
     case 128: return F<R, 128>(__VA_ARGS__); \
     case 256: return F<R, 256>(__VA_ARGS__); \
     case 512: return F<R, 512>(__VA_ARGS__); \
+    case 1024: return F<R, 1024>(__VA_ARGS__); \
     default: break;                          \
   }
 
@@ -1035,6 +1036,7 @@ static int launch_one_xy(Plan& p, int pass, const cx<R>* x, cx<R>* out, hipStrea
     case 64: return F<R, 64>(__VA_ARGS__);   \
     case 128: return F<R, 128>(__VA_ARGS__); \
     case 256: return F<R, 256>(__VA_ARGS__); \
+    case 512: return F<R, 512>(__VA_ARGS__); \
     default: break;                          \
   }
 
@@ -1303,6 +1305,7 @@ int apply_batch(const Plan& ref, const BPass<R>* d, int nsys, int max_M, int max
       case 128: return batch_xy<R, 128>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
       case 256: return batch_xy<R, 256>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
       case 512: return batch_xy<R, 512>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
+      case 1024: return batch_xy<R, 1024>(p, d, nsys, max_M, max_Nz, mode, j, fwd, st);
       default: set_error("unsupported lateral FFT length"); return kUnsupported;
     }
   };
@@ -1317,6 +1320,7 @@ int apply_batch(const Plan& ref, const BPass<R>* d, int nsys, int max_M, int max
     case 64: rc = batch_z<R, 64>(p, d, nsys, st); break;
     case 128: rc = batch_z<R, 128>(p, d, nsys, st); break;
     case 256: rc = batch_z<R, 256>(p, d, nsys, st); break;
+    case 512: rc = batch_z<R, 512>(p, d, nsys, st); break;
     default: set_error("unsupported vertical FFT length"); return kUnsupported;
   }
   if (rc) return rc;

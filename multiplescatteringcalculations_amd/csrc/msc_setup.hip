@@ -158,9 +158,10 @@ int fft_mirror(const double2* in, double2* out, int outer, int N, size_t inner, 
     case 128: return launch_fft_mirror<128, FLIP>(in, out, outer, inner, nout, tw, st);
     case 256: return launch_fft_mirror<256, FLIP>(in, out, outer, inner, nout, tw, st);
     case 512: return launch_fft_mirror<512, FLIP>(in, out, outer, inner, nout, tw, st);
+    case 1024: return launch_fft_mirror<1024, FLIP>(in, out, outer, inner, nout, tw, st);
     default: break;
   }
-  set_error("fft_mirror: length must be a power of two in [2, 512]");
+  set_error("fft_mirror: length must be a power of two in [2, 1024]");
   return kUnsupported;
 }
 

@@ -71,6 +71,19 @@ def test_batched_equals_individual_solves(hip_device, precision):
             assert st["matvecs_c64"] == 0
 
 
+@pytest.mark.parametrize("pad", [(4, 1024), (512, 16)])
+def test_batched_at_the_longest_transforms(hip_device, pad):
+    """Batches on the longest lateral (1024) and vertical (512) FFT lengths."""
+    _, ss = _mods()
+    plans, bs = _plans([(5, 2, 0.4, 0.0), (3, 2, 0.7, 1e-4)], pad=pad, seed=9)
+    xs, sts = ss.gmres_batched(plans, bs, tol=1e-11, max_iter=200, restart=20,
+                               precision="mixed")
+    for plan, b, x, st in zip(plans, bs, xs, sts):
+        assert st["converged"] == 1, st
+        x1, st1 = plan.gmres(b, "born", tol=1e-11, max_iter=200, restart=20, precision="fp64")
+        assert st1["converged"] and _rel(x, x1) < 1e-8
+
+
 def test_batched_periodic_zero_rhs_and_given_x0(hip_device):
     """Periodic slabs of M = 6 and 7 on one padded grid; a zero right-hand side
     converges at once with x = 0; a given x0 (the converged solution) needs no

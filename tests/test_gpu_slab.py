@@ -63,7 +63,7 @@ def test_fft_axis_matches_numpy(hip_device, prec):
     dt = torch.complex128 if prec == "c128" else torch.complex64
     tol = 1e-13 if prec == "c128" else 3e-6
     rng = np.random.default_rng(1)
-    for n in [2, 4, 8, 16, 32, 64, 128, 256, 512]:
+    for n in [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024]:
         for outer, inner in [(3, 1), (2, 9), (1, 45)]:
             x = rng.standard_normal((outer, n, inner)) + 1j * rng.standard_normal((outer, n, inner))
             t = torch.as_tensor(x).to(hip_device, dt)
@@ -145,6 +145,32 @@ def test_matvec_matches_oracle(hip_device, M, Nz, contrast, pad):
     # relative to the scattered part, plus the input's complex64 rounding, which
     # enters x - y64 directly (and is all of it where a single voxel has no
     # coupling)
+    tol = 2e-5 * np.abs(x - y_ref).max() + 1e-7 * np.abs(x).max()
+    assert np.abs((x - y64) - (x - y_ref)).max() < tol
+
+
+@pytest.mark.parametrize("M,Nz,pad,lengths", [
+    (260, 1, None, (2, 1024)),  # 2M - 1 = 519: the 1024-point lateral transforms
+    (2, 130, None, (512, 4)),  # 2N_z - 1 = 259: the 512-point z transforms
+    (3, 2, (512, 8), (512, 8)),
+    (3, 2, (4, 1024), (4, 1024)),
+])
+def test_matvec_matches_oracle_at_the_longest_transforms(hip_device, M, Nz, pad, lengths):
+    """The largest plans: lateral FFT length 1024 (M <= 512) and vertical 512
+    (N_z <= 256), on natural and on explicitly padded grids."""
+    ref, om = REF_C, 692.8
+    Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.4, seed=M + Nz)
+    plan, *_ = _plan(M, Nz, 1.0, ref, om, Dl, Dm, Dr, pad=pad)
+    assert (plan.Pz, plan.Pxy) == lengths
+    T = osl.slab_tmatrices(M, Nz, 1.0, Dl, Dm, Dr, om, ref)
+    kh = osl.slab_kernels(M, Nz, 1.0, om, ref)
+    rng = np.random.default_rng(11)
+    n = Nz * M * M * 9
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y_ref = osl.slab_matvec(x, T, kh, M, Nz)
+    y = plan.matvec(torch.as_tensor(x).to(hip_device)).cpu().numpy()
+    assert _rel(x - y, x - y_ref) < 1e-11
+    y64 = plan.matvec(torch.as_tensor(x).to(hip_device, torch.complex64)).cpu().numpy()
     tol = 2e-5 * np.abs(x - y_ref).max() + 1e-7 * np.abs(x).max()
     assert np.abs((x - y64) - (x - y_ref)).max() < tol
 
@@ -317,6 +343,31 @@ def test_full_size_operator_properties(hip_device):
     assert float(((x - y64) - (x - y128)).norm() / (x - y128).norm()) < 1e-5
 
 
+def test_largest_lateral_slab_solves(hip_device):
+    """M = 512 (lateral FFT length 1024), N_z = 2, through the public solve:
+    it converges, and the FP64 residual of its solution recomputed by a fresh
+    plan's operator (pinned to the oracle at this length above) meets the
+    tolerance."""
+    msc, ss = _mods()
+    M, Nz = 512, 2
+    Dl, Dm, Dr = _iid(M, Nz, REF_C, 0.3, 0.2, seed=4)
+    om = 0.4 * REF_C.beta
+    kh = [1.0, 0.0, 0.0]
+    g = msc.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    r = msc.ReferenceMedium(REF_C.alpha, REF_C.beta, REF_C.rho)
+    mat = msc.SlabMaterial(Dlambda=Dl, Dmu=Dm, Drho=Dr, ref=r)
+    res = msc.compute_slab_scattering_gpu(g, mat, om, np.array(kh), "P", gmres_tol=1e-8,
+                                          max_iter=300, restart=30)
+    assert res.gmres_residual <= 1e-8
+    ss.release_plans()
+    plan, *_ = _plan(M, Nz, 1.0, REF_C, om, Dl, Dm, Dr)
+    assert (plan.Pz, plan.Pxy) == (4, 1024)
+    b = plan.incident_field(om, r, kh, "P").reshape(-1)
+    x = torch.as_tensor(np.ascontiguousarray(res.psi).reshape(-1)).to(hip_device)
+    rr = plan.residual(b, x)
+    assert float(torch.linalg.vector_norm(rr) / torch.linalg.vector_norm(b)) <= 1.01e-8
+
+
 ZDMA = 256  # z tile code flag of the persistent direct-to-LDS z pass
 ZPRUNE = 512  # z tile code flag of the pruned, HBM-direct z pass
 
@@ -325,14 +376,14 @@ ZPRUNE = 512  # z tile code flag of the pruned, HBM-direct z pass
 @pytest.mark.parametrize("q", [2, 4, 8])
 @pytest.mark.parametrize("M,Nz,periodic", [(9, 4, False), (7, 3, False), (8, 5, True),
                                            (16, 1, False), (5, 2, False), (5, 17, False),
-                                           (4, 32, True), (3, 20, False)])
+                                           (4, 32, True), (3, 20, False), (2, 130, False)])
 def test_persistent_z_pass_matches_oracle(hip_device, form, q, M, Nz, periodic):
     """The persistent z pass (tile code Q | 256: the next tile's rows prefetched
     into LDS by direct-to-LDS loads) and the pruned one (Q | 512: the first
     forward pass reads the non-zero rows from HBM, the last inverse pass writes
     the kept rows back), both with z FFTs pruned to the Nz input and output
     rows, give the oracle's operator, with Nz at and below half the padded z
-    length, Pz from 2 to 64 (radix 2, 4 and 8 first passes), periodic and
+    length, Pz from 2 to 512 (radix 2, 4 and 8 first passes), periodic and
     aperiodic."""
     ref, om = REF_C, 692.8
     Dl, Dm, Dr = _iid(M, Nz, ref, 0.5, 0.3, seed=M + 31 * Nz)
