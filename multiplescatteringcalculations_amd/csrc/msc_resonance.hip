@@ -137,9 +137,11 @@ constexpr int kLW = kLT / 64;
 
 // Pivot-search and pivot-growth state of one workgroup.
 struct LuShared {
-  double red_v[kLW];
-  int red_i[kLW];
-  double2 prow[16];
+  double red_v[2][kLW];
+  int red_i[2][kLW];
+  int prow_of[16];        // the row chosen at each step of a panel
+  int tpos[32], trow[32];  // positions holding another row, and that row
+  int ntouch;
   double umin, umax;
   int sing;
 };
@@ -152,13 +154,51 @@ __device__ __forceinline__ void lu_shared_init(LuShared& sh) {
   }
 }
 
+// One step of the wave arg-max (largest value, smallest index among equal
+// values; an associative and commutative choice, so every lane ends with the
+// same pair): combine with the lane the DPP control CTRL names.
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double& best, int& bi) {
+  const int olo = __builtin_amdgcn_update_dpp(0, __double2loint(best), CTRL, 0xF, 0xF, true);
+  const int ohi = __builtin_amdgcn_update_dpp(0, __double2hiint(best), CTRL, 0xF, 0xF, true);
+  const int oi = __builtin_amdgcn_update_dpp(0, bi, CTRL, 0xF, 0xF, true);
+  const double ob = __hiloint2double(ohi, olo);
+  if (ob > best || (ob == best && oi < bi)) {
+    best = ob;
+    bi = oi;
+  }
+}
+
+// The arg-max over the wave, in every lane: within each row of 16 lanes by
+// DPP moves (quad permutations [1,0,3,2] and [2,3,0,1], half-row mirror, row
+// mirror), then across the four rows by two shuffles.
+__device__ __forceinline__ void wave_argmax(double& best, int& bi) {
+  argmax_dpp<0xB1>(best, bi);
+  argmax_dpp<0x4E>(best, bi);
+  argmax_dpp<0x141>(best, bi);
+  argmax_dpp<0x140>(best, bi);
+  for (int off = 16; off < 64; off <<= 1) {
+    const double ob = __shfl_xor(best, off);
+    const int oi = __shfl_xor(bi, off);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+}
+
 // Factorises the panel of columns k0 .. k0 + nb - 1, rows k0 .. n - 1, in LDS
-// (pan: column j at pan + j * n) with partial pivoting and writes it back to
-// A (unit L below the diagonal, U on and above it).  piv[j] = the panel-local
-// row exchanged with row j (the exchanges are applied to the panel's columns
-// only); L11[i * NB + j] = the panel's unit lower block (row i, column j).
-// Per pivot: the maxima of the next column are found while the current one is
-// applied, so two barriers per column.  Ends with a barrier.
+// (pan: column j at pan + j * n) with partial pivoting by implicit row
+// exchanges: the rows stay in place and each thread marks its own rows once
+// they have served as pivot rows (they leave the search and the updates).
+// The maxima of the next column are found while the current one is applied,
+// and the wave maxima alternate between two slots, so one barrier per column.
+// The exchange sequence is then replayed on the positions it touches (at most
+// 2 nb): piv[j] = the panel-local position exchanged with position j at step
+// j, as an explicit-exchange elimination records it, and the panel goes back
+// to A in that row order (unit L below the diagonal, U on and above it).
+// L11[i * NB + j] = the panel's unit lower block (row i, column j).  Ends
+// with a barrier.
 template <int NB>
 __device__ void factor_panel(double2* __restrict__ A, int n, int k0, int nb,
                              double2* __restrict__ pan, int* piv, double2* L11, LuShared& sh) {
@@ -185,6 +225,9 @@ __device__ void factor_panel(double2* __restrict__ A, int n, int k0, int nb,
     }
   }
   __syncthreads();
+  // this thread's rows are t + r kLT (r < 32: m <= 32 kLT); bit r of used:
+  // that row has been a pivot row
+  unsigned used = 0u;
   double best = -1.0;
   int bi = 0;
   for (int i = t; i < m; i += kLT) {
@@ -196,52 +239,42 @@ __device__ void factor_panel(double2* __restrict__ A, int n, int k0, int nb,
     }
   }
   for (int j = 0; j < nb; ++j) {
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
+    wave_argmax(best, bi);
+    const int slot = j & 1;
     if (lane == 0) {
-      sh.red_v[w] = best;
-      sh.red_i[w] = bi;
+      sh.red_v[slot][w] = best;
+      sh.red_i[slot][w] = bi;
     }
     __syncthreads();
-    double b = sh.red_v[0];
-    int p = sh.red_i[0];
+    double b = sh.red_v[slot][0];
+    int p = sh.red_i[slot][0];
     for (int q = 1; q < kLW; ++q)
-      if (sh.red_v[q] > b || (sh.red_v[q] == b && sh.red_i[q] < p)) {
-        b = sh.red_v[q];
-        p = sh.red_i[q];
+      if (sh.red_v[slot][q] > b || (sh.red_v[slot][q] == b && sh.red_i[slot][q] < p)) {
+        b = sh.red_v[slot][q];
+        p = sh.red_i[slot][q];
       }
-    if (t < nb) sh.prow[t] = pan[t * n + p];
-    if (t == 0) piv[j] = p;
-    __syncthreads();
-    // rows j and p exchange (old row p = prow is the pivot row)
-    if (p != j && t < nb) {
-      pan[t * n + p] = pan[t * n + j];
-      pan[t * n + j] = sh.prow[t];
-    }
-    const double2 dgn = sh.prow[j];
+    if (p % kLT == t) used |= 1u << (p / kLT);
+    const double2 dgn = pan[j * n + p];
     const double dd = dgn.x * dgn.x + dgn.y * dgn.y;
     if (t == 0) {
+      sh.prow_of[j] = p;
       const double ad = sqrt(dd);
       sh.umin = fmin(sh.umin, ad);
       sh.umax = fmax(sh.umax, ad);
       if (!(dd > 0.0)) sh.sing = 1;
     }
     const double2 inv = dd > 0.0 ? make_double2(dgn.x / dd, -dgn.y / dd) : make_double2(0.0, 0.0);
-    __syncthreads();
-    // scale column j, update the panel columns right of it, and find the
+    // scale column j of the rows not yet used, update their columns right of
+    // it with the pivot row (row p, not written in this step), and find the
     // maxima of column j + 1 on the way
     best = -1.0;
-    bi = j + 1;
-    for (int i = j + 1 + t; i < m; i += kLT) {
+    bi = 0;
+    int r = 0;
+    for (int i = t; i < m; i += kLT, ++r) {
+      if ((used >> r) & 1u) continue;
       const double2 l = zmul(pan[j * n + i], inv);
       pan[j * n + i] = l;
-      for (int jj = j + 1; jj < nb; ++jj) zfms(pan[jj * n + i], l, sh.prow[jj]);
+      for (int jj = j + 1; jj < nb; ++jj) zfms(pan[jj * n + i], l, pan[jj * n + p]);
       if (j + 1 < nb) {
         const double2 v = pan[(j + 1) * n + i];
         const double a = v.x * v.x + v.y * v.y;
@@ -253,10 +286,54 @@ __device__ void factor_panel(double2* __restrict__ A, int n, int k0, int nb,
     }
   }
   __syncthreads();
+  // the exchange sequence, by wave 0 with the map one entry per lane (lane k:
+  // position pos_k holds row row_k, k < nt): position j takes the step's
+  // pivot row p and the row it held goes where p was; nothing moves when p is
+  // already at j (a pivot on the diagonal)
+  if (w == 0) {
+    int my_pos = -1, my_row = -1, nt = 0;
+    for (int j = 0; j < nb; ++j) {
+      const int p = sh.prow_of[j];
+      const unsigned long long mp = __ballot(lane < nt && my_row == p);
+      const int q = mp ? __shfl(my_pos, __ffsll((long long)mp) - 1) : p;
+      if (lane == 0) piv[j] = q;
+      if (q == j) continue;
+      const unsigned long long mj = __ballot(lane < nt && my_pos == j);
+      int kj = mj ? __ffsll((long long)mj) - 1 : -1;
+      const int rj = mj ? __shfl(my_row, kj) : j;
+      if (kj < 0) {
+        kj = nt++;
+        if (lane == kj) my_pos = j;
+      }
+      if (lane == kj) my_row = p;
+      const unsigned long long mq = __ballot(lane < nt && my_pos == q);
+      int kq = mq ? __ffsll((long long)mq) - 1 : -1;
+      if (kq < 0) {
+        kq = nt++;
+        if (lane == kq) my_pos = q;
+      }
+      if (lane == kq) my_row = rj;
+    }
+    if (lane < nt) {
+      sh.tpos[lane] = my_pos;
+      sh.trow[lane] = my_row;
+    }
+    if (lane == 0) sh.ntouch = nt;
+  }
+  // rows to their own positions, then (after the barrier) the touched ones
   for (int e = t; e < nb * m; e += kLT) {
     const int j = e / m, i = e - j * m;
     A[(size_t)(k0 + j) * n + k0 + i] = pan[j * n + i];
-    if (i < nb) L11[i * NB + j] = pan[j * n + i];
+  }
+  __syncthreads();
+  const int nt = sh.ntouch;
+  for (int e = t; e < nt * nb; e += kLT) {
+    const int k = e / nb, j = e - k * nb;
+    A[(size_t)(k0 + j) * n + k0 + sh.tpos[k]] = pan[j * n + sh.trow[k]];
+  }
+  for (int e = t; e < nb * nb; e += kLT) {
+    const int i = e / nb, j = e - i * nb;
+    L11[i * NB + j] = pan[j * n + sh.prow_of[i]];
   }
   __syncthreads();
 }
