@@ -526,18 +526,11 @@ __global__ __launch_bounds__(kLT) void k_res_lu(double2* __restrict__ Aall, size
 
 typedef double msc_v4d __attribute__((ext_vector_type(4)));
 
-// Rows kb .. kb + kact - 1 of U for the columns cs .. ce - 1 (one thread per
-// column): the row exchanges pv[0 .. kact) (absolute rows, applied in order),
-// then forward substitution with the unit lower block L (LDS, L[i * ldl + j]
-// = row kb + i, column kb + j; its rows from kact up to the next multiple of
-// 16 must hold zeros) in 16-row blocks, one block of x in registers and the
-// finished blocks read back from U.  The rows are written to A and,
-// row-major with leading dimension cw (column c at U + c - cs), to U in LDS
-// (U's rows from kact up to the next multiple of 16 receive zeros).  No
-// branch around a load: the loops run whole blocks, the zero rows of L keep
-// the rows past kact at zero.
-__device__ void solve_u_rows(double2* __restrict__ A, int n, int kb, int kact, const int* pv,
-                             const double2* L, int ldl, double2* U, int cw, int cs, int ce) {
+// The row exchanges pv[0 .. kact) (absolute rows: row kb + j <-> pv[j], in
+// order) on the columns cs .. ce - 1, one thread per column (the same column
+// to thread mapping as solve_u_block, so no barrier between the two).
+__device__ void exchange_rows(double2* __restrict__ A, int n, int kb, int kact, const int* pv,
+                              int cs, int ce) {
 #pragma unroll 1
   for (int c = cs + (int)threadIdx.x; c < ce; c += kLT) {
     double2* col = A + (size_t)c * n;
@@ -549,48 +542,45 @@ __device__ void solve_u_rows(double2* __restrict__ A, int n, int kb, int kact, c
         col[p] = a;
       }
     }
+  }
+}
+
+// Rows r0 .. r0 + nbb - 1 (nbb <= 16) of U for the columns cs .. ce - 1, one
+// thread per column: forward substitution with the unit lower 16 x 16 block
+// L (LDS, L[i * ldl + j]; its rows past nbb hold zeros, which keep x's rows
+// past nbb at zero with no branch around a load) on those rows of A, written
+// back to A and, row-major with leading dimension cw (column c at U + c - cs),
+// to U's 16 rows (zeros past nbb).
+__device__ void solve_u_block(double2* __restrict__ A, int n, int r0, int nbb, const double2* L,
+                              int ldl, double2* U, int cw, int cs, int ce) {
+#pragma unroll 1
+  for (int c = cs + (int)threadIdx.x; c < ce; c += kLT) {
+    double2* col = A + (size_t)c * n;
     double2* u = U + (c - cs);
-#pragma unroll 1
-    for (int rb = 0; rb < kact; rb += 16) {
-      const int nbb = min(16, kact - rb);
-      double2 x[16];
+    double2 x[16];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const double2 v = col[kb + rb + min(i, nbb - 1)];
-        x[i] = i < nbb ? v : make_double2(0.0, 0.0);
-      }
-      // the finished blocks (16 rows each, all below kact)
-#pragma unroll 1
-      for (int jb = 0; jb < rb; jb += 16) {
+    for (int i = 0; i < 16; ++i) {
+      const double2 v = col[r0 + min(i, nbb - 1)];
+      x[i] = i < nbb ? v : make_double2(0.0, 0.0);
+    }
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const double2 y = u[(size_t)(jb + j) * cw];
-          const double2* lj = L + (size_t)rb * ldl + jb + j;
+    for (int i = 1; i < 16; ++i) {
 #pragma unroll
-          for (int i = 0; i < 16; ++i) zfms(x[i], lj[(size_t)i * ldl], y);
-          // one column of the block's L live at a time
-          asm volatile("" ::: "memory");
-        }
-      }
+      for (int j = 0; j < 16; ++j)
+        if (j < i) zfms(x[i], L[(size_t)i * ldl + j], x[j]);
+      // x[i] is final here and the next row's L reads stay below: one row
+      // of the block live at a time (all of it at once spills)
+      asm volatile("" : "+v"(x[i].x), "+v"(x[i].y) : : "memory");
+    }
 #pragma unroll
-      for (int i = 1; i < 16; ++i) {
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-          if (j < i) zfms(x[i], L[(size_t)(rb + i) * ldl + rb + j], x[j]);
-        // x[i] is final here and the next row's L reads stay below: one row
-        // of the block live at a time (all of it at once spills)
-        asm volatile("" : "+v"(x[i].x), "+v"(x[i].y) : : "memory");
-      }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        if (i < nbb) col[kb + rb + i] = x[i];
-        u[(size_t)(rb + i) * cw] = x[i];
-      }
+    for (int i = 0; i < 16; ++i) {
+      if (i < nbb) col[r0 + i] = x[i];
+      u[(size_t)i * cw] = x[i];
     }
   }
 }
 
-// A's rows r0 .. n - 1, columns cs .. ce - 1 -= L W on the matrix cores, with
+// A's rows r0 .. re - 1, columns cs .. ce - 1 -= L W on the matrix cores, with
 // L = A's columns kb .. kb + kact - 1 of those rows (kact <= KM) and W = rows
 // 0 .. kact - 1 of U (LDS, leading dimension cw, column c at U + c - cs).
 // 16 x 16 complex tiles in row-major order, each wave taking one contiguous
@@ -601,7 +591,7 @@ __device__ void solve_u_rows(double2* __restrict__ A, int n, int kb, int kact, c
 // combined at the end.  Lane l (i = l % 16, q = l / 16) feeds L[i][4 s + q]
 // and W[4 s + q][i] at step s and receives rows q + 4 v (v < 4) of column i
 // of the product (the result layout of the instruction on gfx950); entries
-// past kact, n or ce are fed as zeros and not written.  C is loaded two
+// past kact, re or ce are fed as zeros and not written.  C is loaded two
 // tiles ahead (8 KB per wave in flight under the chains).  The tile loop is
 // wave-uniform (the instruction uses every lane).
 __device__ __forceinline__ void ctile_load(const double2* __restrict__ A, int n, int ce, int row0,
@@ -615,11 +605,12 @@ __device__ __forceinline__ void ctile_load(const double2* __restrict__ A, int n,
 
 template <int KM>
 __device__ void trailing_mfma(double2* __restrict__ A, int n, int kb, int kact,
-                              const double2* __restrict__ U, int cw, int r0, int cs, int ce) {
+                              const double2* __restrict__ U, int cw, int r0, int re, int cs,
+                              int ce) {
   constexpr int KS = KM / 4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i16 = lane & 15, q = lane >> 4;
-  const int tr = (n - r0 + 15) / 16, tcn = (ce - cs + 15) / 16, total = tr * tcn;
+  const int tr = (re - r0 + 15) / 16, tcn = (ce - cs + 15) / 16, total = tr * tcn;
   const int ns = (kact + 3) / 4;
   const int t_beg = (int)((long)total * w / kLW), t_end = (int)((long)total * (w + 1) / kLW);
   int cur = -1;
@@ -637,7 +628,7 @@ __device__ void trailing_mfma(double2* __restrict__ A, int n, int kb, int kact,
     if (ti != cur) {
       cur = ti;
       const int rowL = r0 + ti * 16 + i16;
-      const bool rin = rowL < n;
+      const bool rin = rowL < re;
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
         const int k = 4 * s + q;
@@ -670,7 +661,7 @@ __device__ void trailing_mfma(double2* __restrict__ A, int n, int kb, int kact,
       double2* cc = A + (size_t)col * n + row0;
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (row0 + 4 * v < n)
+        if (row0 + 4 * v < re)
           cc[4 * v] = make_double2(c1[v].x - (rr[v] - ii[v]), c1[v].y - (ri[v] + ir[v]));
     }
 #pragma unroll
@@ -723,9 +714,10 @@ __global__ __launch_bounds__(kLT) void k_res_lu_blocked(double2* __restrict__ Aa
       // the outer panel's later columns: 16 rows of U, then a 16-deep update
       const int cs = p0 + nb, ce = k0 + kact;
       if (cs < ce) {
-        solve_u_rows(A, n, p0, nb, pva + (p0 - k0), L16, 16, lds, K, cs, ce);
+        exchange_rows(A, n, p0, nb, pva + (p0 - k0), cs, ce);
+        solve_u_block(A, n, p0, nb, L16, 16, lds, K, cs, ce);
         __syncthreads();
-        trailing_mfma<16>(A, n, p0, nb, lds, K, cs, cs, ce);
+        trailing_mfma<16>(A, n, p0, nb, lds, K, cs, n, cs, ce);
         __syncthreads();
       }
     }
@@ -741,9 +733,21 @@ __global__ __launch_bounds__(kLT) void k_res_lu_blocked(double2* __restrict__ Aa
 #pragma unroll 1
       for (int cs = f0; cs < nc; cs += cw) {
         const int ce = min(cs + cw, nc);
-        solve_u_rows(A, n, k0, kact, pva, lds, K, lds + K * K, cw, cs, ce);
+        double2* U = lds + K * K;
+        exchange_rows(A, n, k0, kact, pva, cs, ce);
+        solve_u_block(A, n, k0, min(16, kact), lds, K, U, cw, cs, ce);
+        // each further 16 rows of U: their rows of A less L U of the rows
+        // above (matrix cores), then the 16 x 16 forward substitution
+#pragma unroll 1
+        for (int rb = 16; rb < kact; rb += 16) {
+          __syncthreads();
+          trailing_mfma<K>(A, n, k0, rb, U, cw, k0 + rb, k0 + min(rb + 16, kact), cs, ce);
+          __syncthreads();
+          solve_u_block(A, n, k0 + rb, min(16, kact - rb), lds + rb * K + rb, K, U + rb * cw,
+                        cw, cs, ce);
+        }
         __syncthreads();
-        trailing_mfma<K>(A, n, k0, kact, lds + K * K, cw, f0, cs, ce);
+        trailing_mfma<K>(A, n, k0, kact, U, cw, f0, n, cs, ce);
         __syncthreads();
       }
     }
