@@ -343,6 +343,17 @@ def test_full_size_operator_properties(hip_device):
     assert float(((x - y64) - (x - y128)).norm() / (x - y128).norm()) < 1e-5
 
 
+@pytest.mark.parametrize("M,Nz,pad", [(3, 2, (1024, 8)), (3, 2, (4, 2048)), (513, 1, None),
+                                      (2, 257, None)])
+def test_transform_lengths_past_the_limits_are_refused(hip_device, M, Nz, pad):
+    """Vertical lengths above 512 and lateral above 1024 (N_z > 256, M > 512)
+    raise ValueError before any allocation."""
+    msc, ss = _mods()
+    g = msc.SlabGeometry(M=M, N_z=Nz, a=1.0)
+    with pytest.raises(ValueError, match="power of two"):
+        ss.SlabPlan(g, pad=pad)
+
+
 def test_largest_lateral_slab_solves(hip_device):
     """M = 512 (lateral FFT length 1024), N_z = 2, through the public solve:
     it converges, and the FP64 residual of its solution recomputed by a fresh
