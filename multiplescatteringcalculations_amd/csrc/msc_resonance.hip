@@ -202,6 +202,7 @@ __device__ __forceinline__ void wave_argmax(double& best, int& bi) {
 template <int NB>
 __device__ void factor_panel(double2* __restrict__ A, int n, int k0, int nb,
                              double2* __restrict__ pan, int* piv, double2* L11, LuShared& sh) {
+  static_assert(NB <= 16 && 2 * NB <= 32, "the exchange map holds 2 NB positions");
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int m = n - k0;
   // 8 loads in flight per thread (indices clamped into the panel, the stores
@@ -961,6 +962,8 @@ int launch_lu_blocked(double2* A, size_t stride, int n, int nc, int count, doubl
 template <int NB>
 int launch_lu(double2* A, size_t stride, int n, int nc, int count, double* ratio, int* sing,
               hipStream_t st) {
+  // factor_panel keeps one bit per row of a thread: at most 32 rows each
+  MSC_REQUIRE(n <= 32 * kLT, "resonance system too large for the elimination");
   if constexpr (NB == 16) {
     // outer panels of 32 columns (64 measured slower at n_sub = 4: its
     // 64-row forward substitutions run on fewer threads per column chunk)
