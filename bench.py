@@ -477,6 +477,71 @@ def _median_time(fn, repeats=3):
     return float(np.median(ts)), [round(t, 4) for t in ts]
 
 
+def time_reference_gpu_formula(plan, n_matvec: int):
+    """The reference's GPU operator (slab_scattering_gpu.py:95-155: T-apply,
+    zero-padded 9-component grid, fftn, 9x9 spectral block product, ifftn,
+    extract, psi - acc) restated in PyTorch and run on this GPU in complex128
+    on the same padded 64x256x256 grid, 1 warm-up + 3 timed repeats (median),
+    scaled by the plain FP64 solve's operator count like the CPU legs.  The
+    spectral table is synthetic (the timing does not depend on its values).
+    The spectral 9x9 product in two torch formulations, the faster one
+    reported: batched matmuls on contiguous operands, and one broadcast
+    multiply and sum over the (9, 9, points) table."""
+    import torch
+
+    from multiplescatteringcalculations_amd.slab_scattering import tparams_to_tmatrices
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    Pz, Pxy = plan.Pz, plan.Pxy
+    npts = Pz * Pxy * Pxy
+    khat = torch.full((npts, 9, 9), 1e-12 + 1e-13j, dtype=torch.complex128, device=dev)
+    Tt = torch.as_tensor(tparams_to_tmatrices(plan.tparams().cpu().numpy())).to(dev).contiguous()
+    gen = torch.Generator(device=dev).manual_seed(2)
+    x = torch.randn(NZ * M * M * 9, dtype=torch.complex128, device=dev, generator=gen)
+
+    def mv():
+        psi = x.reshape(NZ, M, M, 9)
+        tau = torch.matmul(Tt, psi.unsqueeze(-1)).squeeze(-1)
+        pad = torch.zeros((9, Pz, Pxy, Pxy), dtype=torch.complex128, device=dev)
+        pad[:, :NZ, :M, :M] = tau.permute(3, 0, 1, 2)
+        f = torch.fft.fftn(pad, dim=(1, 2, 3)).reshape(9, npts).t().contiguous()
+        acc = torch.bmm(khat, f.unsqueeze(-1)).squeeze(-1).t().reshape(9, Pz, Pxy, Pxy)
+        acc = torch.fft.ifftn(acc, dim=(1, 2, 3))
+        y = psi - acc[:, :NZ, :M, :M].permute(1, 2, 3, 0)
+        torch.cuda.synchronize(dev)
+        return y
+
+    t_bmm, reps = _median_time(mv)
+    del khat
+    torch.cuda.empty_cache()
+    khat9 = torch.full((9, 9, npts), 1e-12 + 1e-13j, dtype=torch.complex128, device=dev)
+
+    def mv_bcast():
+        psi = x.reshape(NZ, M, M, 9)
+        tau = torch.matmul(Tt, psi.unsqueeze(-1)).squeeze(-1)
+        pad = torch.zeros((9, Pz, Pxy, Pxy), dtype=torch.complex128, device=dev)
+        pad[:, :NZ, :M, :M] = tau.permute(3, 0, 1, 2)
+        f = torch.fft.fftn(pad, dim=(1, 2, 3)).reshape(1, 9, npts)
+        acc = (khat9 * f).sum(dim=1).reshape(9, Pz, Pxy, Pxy)
+        acc = torch.fft.ifftn(acc, dim=(1, 2, 3))
+        y = psi - acc[:, :NZ, :M, :M].permute(1, 2, 3, 0)
+        torch.cuda.synchronize(dev)
+        return y
+
+    t_bc, reps_bc = _median_time(mv_bcast)
+    del khat9, Tt
+    torch.cuda.empty_cache()
+    t = min(t_bmm, t_bc)
+    out = {"matvec_ms": round(t * 1e3, 3), "matvec_ms_batched_matmul": round(t_bmm * 1e3, 3),
+           "matvec_ms_broadcast": round(t_bc * 1e3, 3), "repeats_s": reps + reps_bc,
+           "n_matvec": int(n_matvec), "s_per_solve_extrapolated": round(t * n_matvec, 4),
+           "precision": "complex128",
+           "formula": "PyTorch fftn / spectral 9x9 product / ifftn on the padded grid "
+                      "(slab_scattering_gpu.py:95-155), this GPU; the faster of two formulations"}
+    log(f"reference GPU formula (torch): {out}")
+    return out
+
+
 def cpu_baseline(plan, n_matvec: int):
     """CPU legs, all on this host's cores, 1 warm-up + 3 timed repeats (median):
 
@@ -696,7 +761,9 @@ def main():
     if not args.no_sweep:
         sweep = time_sweep("batched")
         sweep["workers_mode"] = time_sweep("workers")
-    extras = None if args.no_extras else {"resonance_batch": time_resonance(),
+    extras = None if args.no_extras else {"reference_formula_torch_gpu":
+                                          time_reference_gpu_formula(plan, st64["matvecs_c128"]),
+                                          "resonance_batch": time_resonance(),
                                           "sphere_fft": time_sphere(),
                                           "resonance_slab_ocean_bottom": time_resonance_slab(),
                                           "learned_strong_contrast": time_strong_contrast()}
