@@ -1648,7 +1648,7 @@ int msc_net_input(msc_plan* plan, const void* r, int precision, double scale, vo
     set_error("plan needs msc_set_tmatrix before msc_net_input");
     return kNotReady;
   }
-  if (plan->scal == nullptr) MSC_HIP_CHECK(hipMalloc(&plan->scal, 4 * sizeof(double)));
+  if (plan->scal == nullptr) MSC_HIP_CHECK(hipMalloc(&plan->scal, kScalDoubles * sizeof(double)));
   uint16_t* out = reinterpret_cast<uint16_t*>(out_bf16);
   if (precision == MSC_C128)
     return net_input<double>(plan->p, reinterpret_cast<const double2*>(r), scale, plan->scal,

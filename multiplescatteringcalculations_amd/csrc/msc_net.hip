@@ -45,8 +45,8 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 // The input scale lives on the device as (n, sum |r|^2): s = sqrt(n / sum).
-// A caller-given scale is stored as (s^2, 1); a measured one is accumulated by
-// k_norm_scale into the second slot.
+// A caller-given scale is stored as (s^2, 1); a measured one is summed into
+// the second slot by k_norm_part and k_norm_finish.
 __device__ __forceinline__ float scale_of(const double* scal) {
   const double sum = scal[1];
   return sum > 0.0 ? (float)sqrt(scal[0] / sum) : 1.f;
@@ -1128,24 +1128,50 @@ __global__ void k_set_scale(double* scal, double a, double b) {
   }
 }
 
+// sum |r|^2 in two fixed-order stages: per-workgroup partials into
+// scal[4 + block] (four loads in flight per thread), then one workgroup adds
+// them into scal[1] (no atomics: the same scale on every run).
 template <typename R>
-__global__ __launch_bounds__(kNetNT) void k_norm_scale(const cx<R>* __restrict__ r, size_t n,
-                                                       double* __restrict__ scal) {
+__global__ __launch_bounds__(kNetNT) void k_norm_part(const cx<R>* __restrict__ r, size_t n,
+                                                      double* __restrict__ scal) {
   __shared__ double red[kNetNT / 64];
-  double acc = 0.0;
-  for (size_t e = (size_t)blockIdx.x * kNetNT + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * kNetNT) {
-    const double2 a = to_c128(r[e]);
-    acc += a.x * a.x + a.y * a.y;
+  const size_t stride = (size_t)gridDim.x * kNetNT;
+  size_t e = (size_t)blockIdx.x * kNetNT + threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (; e + 3 * stride < n; e += 4 * stride) {
+    const double2 x0 = to_c128(r[e]), x1 = to_c128(r[e + stride]);
+    const double2 x2 = to_c128(r[e + 2 * stride]), x3 = to_c128(r[e + 3 * stride]);
+    a0 += x0.x * x0.x + x0.y * x0.y;
+    a1 += x1.x * x1.x + x1.y * x1.y;
+    a2 += x2.x * x2.x + x2.y * x2.y;
+    a3 += x3.x * x3.x + x3.y * x3.y;
   }
+  for (; e < n; e += stride) {
+    const double2 x0 = to_c128(r[e]);
+    a0 += x0.x * x0.x + x0.y * x0.y;
+  }
+  double acc = (a0 + a1) + (a2 + a3);
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < kNetNT / 64; ++i) s += red[i];
-    atomicAdd(&scal[1], s);
+    double t = 0.0;
+    for (int i = 0; i < kNetNT / 64; ++i) t += red[i];
+    scal[4 + blockIdx.x] = t;
   }
+}
+
+__global__ __launch_bounds__(kNetNT) void k_norm_finish(double* __restrict__ scal, int np) {
+  __shared__ double red[kNetNT];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < np; i += kNetNT) t += scal[4 + i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int h = kNetNT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scal[1] = red[0];
 }
 
 // ── host side ───────────────────────────────────────────────────────────────
@@ -1263,7 +1289,7 @@ int alloc_params(Net& net) {
   MSC_HIP_CHECK(hipMalloc(&net.wtr, wp * sizeof(uint16_t)));
   MSC_HIP_CHECK(hipMalloc(&net.bpk, bp * sizeof(float)));
   MSC_HIP_CHECK(hipMalloc(&net.loss, 4 * sizeof(double)));
-  MSC_HIP_CHECK(hipMalloc(&net.scal, 4 * sizeof(double)));
+  MSC_HIP_CHECK(hipMalloc(&net.scal, kScalDoubles * sizeof(double)));
   return kOk;
 }
 
@@ -1692,8 +1718,9 @@ static float set_input_scale(const Plan& p, const cx<R>* r, double scale, double
     hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, scale * scale, 1.0);
   } else if (sarg == 0.f) {
     hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, s, scal, (double)p.n, 0.0);
-    hipLaunchKernelGGL((k_norm_scale<R>), dim3(grid_of(p.n, 2048)), dim3(kNetNT), 0, s, r, p.n,
-                       scal);
+    const int np = grid_of((p.n + 3) / 4, kNormParts);
+    hipLaunchKernelGGL((k_norm_part<R>), dim3(np), dim3(kNetNT), 0, s, r, p.n, scal);
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(kNetNT), 0, s, scal, np);
   }
   return sarg;
 }

@@ -54,6 +54,11 @@ struct NetTensor {
   int skip_user = -1;  // layer that adds this tensor as a skip (-1: none)
 };
 
+// Device scalars of an application's input scale: (n, sum |r|^2, 2 spare),
+// then kNormParts per-workgroup partial sums of |r|^2.
+constexpr int kNormParts = 1024;
+constexpr int kScalDoubles = 4 + kNormParts;
+
 struct Net {
   int arch = kArchLocal;
   int hidden = 0;               // local: h; unet: c0
@@ -86,7 +91,8 @@ struct Net {
   float* outf = nullptr;                // last layer output, fp32 [vox][kNetIOPad]
   uint16_t* inbuf = nullptr;            // gathered input of an application with a general T
   double* loss = nullptr;               // device loss accumulator (+ partials)
-  double* scal = nullptr;  // device (n, sum |r|^2) of the current apply: s = sqrt(n/sum)
+  double* scal = nullptr;  // device (n, sum |r|^2, ..., partials) of the current apply:
+                           // s = sqrt(n/sum); kScalDoubles doubles
   float scale_arg = 0.f;   // host: a given s of the current apply, 0 = read scal
   bool packed = false;
   // changes whenever device buffers are (re)allocated; captured graphs that
