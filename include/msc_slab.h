@@ -254,9 +254,11 @@ int msc_plan_info(const msc_plan* plan, long long* out6);
  * for one precision and keep the fastest (needs msc_build_g0 and
  * msc_set_tmatrix first).  msc_gmres tunes each precision it uses once per
  * plan unless the environment sets MSC_AUTOTUNE=0.  tiles: out10 = for
- * complex128 then complex64, (y-forward rows, x-forward columns, z columns,
- * x-inverse columns, y-inverse rows) per workgroup.  No counterpart in the
- * reference (its FFTs are library calls).
+ * complex128 then complex64, (y-forward rows, x-forward tile, z columns,
+ * x-inverse tile, y-inverse rows) per workgroup; an x tile of 1, 2 or 4 is
+ * that many whole ky columns (9 complex each) of every x row, 8, 16 or 32 is
+ * that many consecutive complex of every row (whole cache lines).  No
+ * counterpart in the reference (its FFTs are library calls).
  */
 int msc_plan_autotune(msc_plan* plan, int precision, void* stream);
 int msc_plan_tiles(const msc_plan* plan, int* out10);

@@ -1248,8 +1248,8 @@ int msc_plan_set_tiles(msc_plan* plan, const int* tiles10) {
                                      v == (2 | kZDma) || v == (4 | kZDma) || v == (8 | kZDma) ||
                                      v == (2 | kZPrune) || v == (4 | kZPrune) ||
                                      v == (8 | kZPrune))
-                                  : (v == 1 || v == 2 || v == 4);
-    MSC_REQUIRE(ok, "tile out of range (y passes 1-2, x passes 1/2/4, z pass 1/2/4/8/16, "
+                                  : (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32);
+    MSC_REQUIRE(ok, "tile out of range (y passes 1-2, x passes 1/2/4/8/16/32, z pass 1/2/4/8/16, "
                     "2/4/8 + 256 for the persistent z pass or 2/4/8 + 512 for the pruned one)");
   }
   for (int pr = 0; pr < 2; ++pr) {

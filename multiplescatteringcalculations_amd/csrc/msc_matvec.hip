@@ -258,26 +258,31 @@ __global__ __launch_bounds__(kNT) void k_fwd_y(const cx<R>* __restrict__ x,
 
 
 // ── pass 2: forward FFT along x of the M non-zero rows ─────────────────────
-template <typename R, int P, int Q>
+// A tile is NTR consecutive complex numbers of every x row of one z plane (a
+// row holds P * 9: ky-major, component-minor); the NTR transforms along x are
+// independent.  NTR = 9Q covers Q whole ky columns; NTR = 8, 16 or 32 covers
+// whole 128-byte cache lines of each row and fills every lane of the LDS
+// transform (NTR * P / radix is a multiple of the workgroup size).
+template <typename R, int P, int NTR>
 __device__ __forceinline__ void fwd_x_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ buf1,
                                               cx<R>* __restrict__ buf2, int M,
                                               const cx<R>* __restrict__ tw,
                                               const int* __restrict__ done) {
-  constexpr int NTR = 9 * Q;
+  static_assert((P * 9) % NTR == 0, "x tile must divide the row");
   constexpr int LD = lds_ld<NTR>::value;
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
-  constexpr int nck = P / Q;
+  constexpr int nck = P * 9 / NTR;
   const int lb = xcd_block((int)bx_, (int)gx_);
   const int z = lb / nck;
-  const int ky0 = (lb - z * nck) * Q;
+  const int e0 = (lb - z * nck) * NTR;
   staged<cx<R>>(
       P * NTR,
       [&](int w) {
         const int xr = w / NTR;
         const int e = w - xr * NTR;
         const bool ok = xr < M;
-        const cx<R> v = buf1[ok ? (((size_t)z * M + xr) * P + ky0) * 9 + e : 0];
+        const cx<R> v = buf1[ok ? ((size_t)z * M + xr) * P * 9 + e0 + e : 0];
         return ok ? v : czero<R>();
       },
       [&](int w, cx<R> v) {
@@ -289,15 +294,15 @@ __device__ __forceinline__ void fwd_x_body(unsigned bx_, unsigned gx_, const cx<
   for (int w = threadIdx.x; w < P * NTR; w += kNT) {
     const int kx = w / NTR;
     const int e = w - kx * NTR;
-    buf2[(((size_t)z * P + kx) * P + ky0) * 9 + e] = s[kx * LD + e];
+    buf2[((size_t)z * P + kx) * P * 9 + e0 + e] = s[kx * LD + e];
   }
 }
-template <typename R, int P, int Q>
+template <typename R, int P, int NTR>
 __global__ __launch_bounds__(kNT) void k_fwd_x(const cx<R>* __restrict__ buf1,
                                               cx<R>* __restrict__ buf2, int M,
                                               const cx<R>* __restrict__ tw,
                                               const int* __restrict__ done) {
-  fwd_x_body<R, P, Q>(blockIdx.x, gridDim.x, buf1, buf2, M, tw, done);
+  fwd_x_body<R, P, NTR>(blockIdx.x, gridDim.x, buf1, buf2, M, tw, done);
 }
 
 
@@ -528,24 +533,25 @@ __global__ __launch_bounds__(kNT) void k_z_mult_dma(cx<R>* __restrict__ buf2,
 }
 
 // ── pass 4: inverse FFT along x, keep x < M ────────────────────────────────
-template <typename R, int P, int Q>
+// Tiles as in pass 2.
+template <typename R, int P, int NTR>
 __device__ __forceinline__ void inv_x_body(unsigned bx_, unsigned gx_, const cx<R>* __restrict__ buf2,
                                               cx<R>* __restrict__ buf1, int M,
                                               const cx<R>* __restrict__ tw,
                                               const int* __restrict__ done) {
-  constexpr int NTR = 9 * Q;
+  static_assert((P * 9) % NTR == 0, "x tile must divide the row");
   constexpr int LD = lds_ld<NTR>::value;
   __shared__ cx<R> s[P * LD];
   if (done != nullptr && *done) return;
-  constexpr int nck = P / Q;
+  constexpr int nck = P * 9 / NTR;
   const int lb = xcd_block((int)bx_, (int)gx_);
   const int z = lb / nck;
-  const int ky0 = (lb - z * nck) * Q;
+  const int e0 = (lb - z * nck) * NTR;
   staged<cx<R>>(
       P * NTR,
       [&](int w) {
         const int kx = w / NTR;
-        return buf2[(((size_t)z * P + kx) * P + ky0) * 9 + (w - kx * NTR)];
+        return buf2[((size_t)z * P + kx) * P * 9 + e0 + (w - kx * NTR)];
       },
       [&](int w, cx<R> v) {
         const int kx = w / NTR;
@@ -556,15 +562,15 @@ __device__ __forceinline__ void inv_x_body(unsigned bx_, unsigned gx_, const cx<
   for (int w = threadIdx.x; w < M * NTR; w += kNT) {
     const int xr = w / NTR;
     const int e = w - xr * NTR;
-    buf1[(((size_t)z * M + xr) * P + ky0) * 9 + e] = s[xr * LD + e];
+    buf1[((size_t)z * M + xr) * P * 9 + e0 + e] = s[xr * LD + e];
   }
 }
-template <typename R, int P, int Q>
+template <typename R, int P, int NTR>
 __global__ __launch_bounds__(kNT) void k_inv_x(const cx<R>* __restrict__ buf2,
                                               cx<R>* __restrict__ buf1, int M,
                                               const cx<R>* __restrict__ tw,
                                               const int* __restrict__ done) {
-  inv_x_body<R, P, Q>(blockIdx.x, gridDim.x, buf2, buf1, M, tw, done);
+  inv_x_body<R, P, NTR>(blockIdx.x, gridDim.x, buf2, buf1, M, tw, done);
 }
 
 
@@ -724,8 +730,9 @@ template <> struct PlanBufs<float> {
   static const float2* tw(const Plan& p) { return p.tw64; }
 };
 
-// Tiles: rows per workgroup of the y passes (B), columns per workgroup of the
-// x passes and of the z pass (Q).  Each plan starts from the defaults below
+// Tiles: rows per workgroup of the y passes (B), the x-pass tile code (1, 2,
+// 4 ky columns, or 8, 16, 32 complex of each row) and columns per workgroup
+// of the z pass (Q).  Each plan starts from the defaults below
 // (measured best at the 128x128x32 slab, P = 256, Pz = 64) and autotune()
 // re-picks them per precision by timing every candidate in place.
 constexpr int kLdsMax = 160 * 1024;
@@ -736,8 +743,8 @@ constexpr bool lds_fits() {
 
 int default_tile(int prec, int pass) {
   // pass: 0 y-forward B, 1 x-forward Q, 2 z Q, 3 x-inverse Q, 4 y-inverse B
-  static const int t64[5] = {1, 2, 8, 2, 1};
-  static const int t128[5] = {1, 1, 8, 1, 1};
+  static const int t64[5] = {1, 16, 8, 16, 1};
+  static const int t128[5] = {1, 16, 8, 8, 1};
   return prec == 1 ? t64[pass] : t128[pass];
 }
 
@@ -776,54 +783,54 @@ static int pass_fwd_y(Plan& p, const cx<R>* x, cx<R>* b1, hipStream_t st, const 
   return kOk;
 }
 
+// x-pass tile codes: 1, 2, 4 = Q whole ky columns (9Q complex per row),
+// 8, 16, 32 = that many complex per row (whole cache lines).
+template <typename R, int P, int NTR>
+constexpr bool x_tile_fits() {
+  return P >= 2 && (P * 9) % NTR == 0 &&
+         P * lds_ld<NTR>::value * (int)sizeof(cx<R>) <= kLdsMax;
+}
+
+template <typename R, int P, int NTR>
+static bool launch_x(bool fwd, Plan& p, const cx<R>* src, cx<R>* dst, hipStream_t st,
+                     const int* done) {
+  if constexpr (x_tile_fits<R, P, NTR>()) {
+    const dim3 g(p.Nz * (P * 9 / NTR));
+    const cx<R>* tw = PlanBufs<R>::tw(p);
+    if (fwd) hipLaunchKernelGGL((k_fwd_x<R, P, NTR>), g, dim3(kNT), 0, st, src, dst, p.M, tw, done);
+    else hipLaunchKernelGGL((k_inv_x<R, P, NTR>), g, dim3(kNT), 0, st, src, dst, p.M, tw, done);
+    return true;
+  }
+  return false;
+}
+
+// The plan's tile for the pass, else the next smaller one that fits.
+template <typename R, int P>
+static void pass_x(bool fwd, Plan& p, const cx<R>* src, cx<R>* dst, hipStream_t st,
+                   const int* done) {
+  const int code = p.tile[prec_of<R>()][fwd ? 1 : 3];
+  bool ok = false;
+  if (code >= 8) {
+    if (code >= 32) ok = launch_x<R, P, 32>(fwd, p, src, dst, st, done);
+    if (!ok && code >= 16) ok = launch_x<R, P, 16>(fwd, p, src, dst, st, done);
+    if (!ok) ok = launch_x<R, P, 8>(fwd, p, src, dst, st, done);
+  } else {
+    if (code >= 4) ok = launch_x<R, P, 36>(fwd, p, src, dst, st, done);
+    if (!ok && code >= 2) ok = launch_x<R, P, 18>(fwd, p, src, dst, st, done);
+  }
+  if (!ok) launch_x<R, P, 9>(fwd, p, src, dst, st, done);
+}
+
 template <typename R, int P>
 static int pass_fwd_x(Plan& p, const cx<R>* b1, cx<R>* b2, hipStream_t st, const int* done) {
-  const cx<R>* tw = PlanBufs<R>::tw(p);
-  switch (p.tile[prec_of<R>()][1]) {
-    case 4:
-      if constexpr (P >= 4 && lds_fits<R, P, 9, 4>()) {
-        hipLaunchKernelGGL((k_fwd_x<R, P, 4>), dim3(p.Nz * (P / 4)), dim3(kNT), 0, st, b1, b2, p.M,
-                           tw, done);
-        break;
-      }
-      [[fallthrough]];
-    case 2:
-      if constexpr (lds_fits<R, P, 9, 2>()) {
-        hipLaunchKernelGGL((k_fwd_x<R, P, 2>), dim3(p.Nz * (P / 2)), dim3(kNT), 0, st, b1, b2, p.M,
-                           tw, done);
-        break;
-      }
-      [[fallthrough]];
-    default:
-      hipLaunchKernelGGL((k_fwd_x<R, P, 1>), dim3(p.Nz * P), dim3(kNT), 0, st, b1, b2, p.M, tw,
-                         done);
-  }
+  pass_x<R, P>(true, p, b1, b2, st, done);
   MSC_LAUNCH_CHECK();
   return kOk;
 }
 
 template <typename R, int P>
 static int pass_inv_x(Plan& p, const cx<R>* b2, cx<R>* b1, hipStream_t st, const int* done) {
-  const cx<R>* tw = PlanBufs<R>::tw(p);
-  switch (p.tile[prec_of<R>()][3]) {
-    case 4:
-      if constexpr (P >= 4 && lds_fits<R, P, 9, 4>()) {
-        hipLaunchKernelGGL((k_inv_x<R, P, 4>), dim3(p.Nz * (P / 4)), dim3(kNT), 0, st, b2, b1, p.M,
-                           tw, done);
-        break;
-      }
-      [[fallthrough]];
-    case 2:
-      if constexpr (lds_fits<R, P, 9, 2>()) {
-        hipLaunchKernelGGL((k_inv_x<R, P, 2>), dim3(p.Nz * (P / 2)), dim3(kNT), 0, st, b2, b1, p.M,
-                           tw, done);
-        break;
-      }
-      [[fallthrough]];
-    default:
-      hipLaunchKernelGGL((k_inv_x<R, P, 1>), dim3(p.Nz * P), dim3(kNT), 0, st, b2, b1, p.M, tw,
-                         done);
-  }
+  pass_x<R, P>(false, p, b2, b1, st, done);
   MSC_LAUNCH_CHECK();
   return kOk;
 }
@@ -1111,13 +1118,13 @@ __global__ __launch_bounds__(kNT) void kb_fwd_y(const BPass<R>* __restrict__ d, 
                              nullptr, q.buf1, q.M, nrows, tw, q.done);
 }
 
-template <typename R, int P, int Q>
+template <typename R, int P, int NTR>
 __global__ __launch_bounds__(kNT) void kb_fwd_x(const BPass<R>* __restrict__ d,
                                                const cx<R>* __restrict__ tw) {
   const BPass<R>& q = d[blockIdx.y];
-  const int nblk = q.Nz * (P / Q);
+  const int nblk = q.Nz * (P * 9 / NTR);
   if ((int)blockIdx.x >= nblk) return;
-  fwd_x_body<R, P, Q>(blockIdx.x, nblk, q.buf1, q.buf2, q.M, tw, q.done);
+  fwd_x_body<R, P, NTR>(blockIdx.x, nblk, q.buf1, q.buf2, q.M, tw, q.done);
 }
 
 template <typename R, int PZ, int Q>
@@ -1136,13 +1143,13 @@ __global__ __launch_bounds__(kNT) void kb_z_mult_pr(const BPass<R>* __restrict__
   z_mult_pruned_body<R, PZ, Q>(blockIdx.x, q.buf2, q.table, q.Nz, Pxy, tw, zmap, q.done);
 }
 
-template <typename R, int P, int Q>
+template <typename R, int P, int NTR>
 __global__ __launch_bounds__(kNT) void kb_inv_x(const BPass<R>* __restrict__ d,
                                                const cx<R>* __restrict__ tw) {
   const BPass<R>& q = d[blockIdx.y];
-  const int nblk = q.Nz * (P / Q);
+  const int nblk = q.Nz * (P * 9 / NTR);
   if ((int)blockIdx.x >= nblk) return;
-  inv_x_body<R, P, Q>(blockIdx.x, nblk, q.buf2, q.buf1, q.M, tw, q.done);
+  inv_x_body<R, P, NTR>(blockIdx.x, nblk, q.buf2, q.buf1, q.M, tw, q.done);
 }
 
 template <typename R, int P, int B, int MODE>
@@ -1159,13 +1166,13 @@ __global__ __launch_bounds__(kNT) void kb_inv_y(const BPass<R>* __restrict__ d, 
 // instance runs on XCD b % 8 as in the single-instance launch
 static unsigned round8(int v) { return (unsigned)((v + 7) / 8 * 8); }
 
-template <typename R, int P, int Q>
+template <typename R, int P, int NTR>
 static bool batch_x_q(bool fwd, const BPass<R>* d, int nsys, int max_Nz, const cx<R>* tw,
                       hipStream_t st) {
-  if constexpr (P >= Q && lds_fits<R, P, 9, Q>()) {
-    const dim3 g(round8(max_Nz * (P / Q)), nsys);
-    if (fwd) hipLaunchKernelGGL((kb_fwd_x<R, P, Q>), g, dim3(kNT), 0, st, d, tw);
-    else hipLaunchKernelGGL((kb_inv_x<R, P, Q>), g, dim3(kNT), 0, st, d, tw);
+  if constexpr (x_tile_fits<R, P, NTR>()) {
+    const dim3 g(round8(max_Nz * (P * 9 / NTR)), nsys);
+    if (fwd) hipLaunchKernelGGL((kb_fwd_x<R, P, NTR>), g, dim3(kNT), 0, st, d, tw);
+    else hipLaunchKernelGGL((kb_inv_x<R, P, NTR>), g, dim3(kNT), 0, st, d, tw);
     return true;
   }
   return false;
@@ -1188,9 +1195,15 @@ static int batch_xy(const Plan& ref, const BPass<R>* d, int nsys, int max_M, int
     if (!two) hipLaunchKernelGGL((kb_fwd_y<R, P, 1>), gy, dim3(kNT), 0, st, d, j, tw);
   }
   bool done = false;
-  if (q >= 4) done = batch_x_q<R, P, 4>(forward, d, nsys, max_Nz, tw, st);
-  if (!done && q >= 2) done = batch_x_q<R, P, 2>(forward, d, nsys, max_Nz, tw, st);
-  if (!done) batch_x_q<R, P, 1>(forward, d, nsys, max_Nz, tw, st);
+  if (q >= 8) {
+    if (q >= 32) done = batch_x_q<R, P, 32>(forward, d, nsys, max_Nz, tw, st);
+    if (!done && q >= 16) done = batch_x_q<R, P, 16>(forward, d, nsys, max_Nz, tw, st);
+    if (!done) done = batch_x_q<R, P, 8>(forward, d, nsys, max_Nz, tw, st);
+  } else {
+    if (q >= 4) done = batch_x_q<R, P, 36>(forward, d, nsys, max_Nz, tw, st);
+    if (!done && q >= 2) done = batch_x_q<R, P, 18>(forward, d, nsys, max_Nz, tw, st);
+  }
+  if (!done) batch_x_q<R, P, 9>(forward, d, nsys, max_Nz, tw, st);
   if (!forward) {
     if constexpr (lds_fits<R, P, 9, 2>()) {
       if (two) {
@@ -1360,7 +1373,7 @@ int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st) {
   MSC_HIP_CHECK(hipEventCreate(&e0));
   MSC_HIP_CHECK(hipEventCreate(&e1));
   static const int cand_y[] = {1, 2};
-  static const int cand_x[] = {1, 2, 4};
+  static const int cand_x[] = {1, 2, 4, 8, 16, 32};
   static const int cand_z[] = {2, 4, 8, 2 | kZDma, 4 | kZDma, 8 | kZDma,
                                2 | kZPrune, 4 | kZPrune, 8 | kZPrune};
   static const int order[] = {2, 1, 3, 0, 4};
@@ -1368,7 +1381,7 @@ int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st) {
   for (int pass : order) {
     if (rc) break;
     const int* cand = pass == 2 ? cand_z : (pass == 0 || pass == 4 ? cand_y : cand_x);
-    const int nc = pass == 2 ? 9 : (pass == 0 || pass == 4 ? 2 : 3);
+    const int nc = pass == 2 ? 9 : (pass == 0 || pass == 4 ? 2 : 6);
     int best = p.tile[pr][pass];
     float best_ms = 1e30f;
     for (int i = 0; i < nc && !rc; ++i) {
