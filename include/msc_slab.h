@@ -250,7 +250,8 @@ int msc_sweep_release(void);
 int msc_plan_info(const msc_plan* plan, long long* out6);
 
 /*
- * Tile autotuning: time every tile candidate of the five FFT passes in place
+ * Tile autotuning: time every tile candidate of the five FFT passes in whole
+ * operator applications
  * for one precision and keep the fastest (needs msc_build_g0 and
  * msc_set_tmatrix first).  msc_gmres tunes each precision it uses once per
  * plan unless the environment sets MSC_AUTOTUNE=0.  tiles: out10 = for

@@ -1005,19 +1005,6 @@ static int launch_xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out,
   return pass_inv_y<R, P>(p, b1, x, bvec, out, mode, st, done);
 }
 
-// One pass alone (autotuning): 0 y-forward, 1 x-forward, 2 z, 3 x-inverse, 4 y-inverse.
-template <typename R, int P>
-static int launch_one_xy(Plan& p, int pass, const cx<R>* x, cx<R>* out, hipStream_t st) {
-  cx<R>* b1 = reinterpret_cast<cx<R>*>(p.buf1);
-  cx<R>* b2 = reinterpret_cast<cx<R>*>(p.buf2);
-  switch (pass) {
-    case 0: return pass_fwd_y<R, P>(p, x, b1, st, nullptr);
-    case 1: return pass_fwd_x<R, P>(p, b1, b2, st, nullptr);
-    case 3: return pass_inv_x<R, P>(p, b2, b1, st, nullptr);
-    default: return pass_inv_y<R, P>(p, b1, x, nullptr, out, 2, st, nullptr);
-  }
-}
-
 #define MSC_XY_CASES(F, ...)                 \
   switch (p.Pxy) {                           \
     case 2: return F<R, 2>(__VA_ARGS__);     \
@@ -1057,12 +1044,6 @@ template <typename R>
 static int xy_inv(Plan& p, const cx<R>* x, const cx<R>* bvec, cx<R>* out, cx<R>* b1, cx<R>* b2,
                   int mode, hipStream_t st, const int* done) {
   MSC_XY_CASES(launch_xy_inv, p, x, bvec, out, b1, b2, mode, st, done);
-  set_error("unsupported lateral FFT length");
-  return kUnsupported;
-}
-template <typename R>
-static int one_xy(Plan& p, int pass, const cx<R>* x, cx<R>* out, hipStream_t st) {
-  MSC_XY_CASES(launch_one_xy, p, pass, x, out, st);
   set_error("unsupported lateral FFT length");
   return kUnsupported;
 }
@@ -1358,8 +1339,9 @@ int scattered(Plan& p, const cx<R>* x, cx<R>* y, hipStream_t st, const int* done
   return apply<R>(p, x, nullptr, y, 2, st, done);
 }
 
-// Pick the tile of every pass for precision R by timing each candidate in
-// place on the plan's workspaces (inputs: x, outputs: out, both n complex).
+// Pick the tile of every pass for precision R by timing whole operator
+// applications with each candidate (the other passes at their current tiles)
+// on the plan's workspaces (inputs: x, outputs: out, both n complex).
 template <typename R>
 int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st) {
   if (!p.have_table || !p.have_tparams) {
@@ -1386,14 +1368,15 @@ int autotune(Plan& p, const cx<R>* x, cx<R>* out, hipStream_t st) {
     float best_ms = 1e30f;
     for (int i = 0; i < nc && !rc; ++i) {
       p.tile[pr][pass] = cand[i];
-      auto run = [&]() {
-        return pass == 2 ? z_pass<R>(p, reinterpret_cast<cx<R>*>(p.buf2), st, nullptr)
-                         : one_xy<R>(p, pass, x, out, st);
-      };
+      // whole operator applications: a pass repeated on its own keeps its
+      // buffers and the table in the 256 MB last-level cache, which the
+      // solve's other passes and Krylov vectors evict (the z pass measured
+      // 150 us repeated alone against 193 us inside solves at 128x128x32)
+      auto run = [&]() { return apply<R>(p, x, nullptr, out, 2, st, nullptr); };
       rc = run();
       if (rc) break;
       MSC_HIP_CHECK(hipEventRecord(e0, st));
-      for (int rep = 0; rep < 3 && !rc; ++rep) rc = run();
+      for (int rep = 0; rep < 4 && !rc; ++rep) rc = run();
       MSC_HIP_CHECK(hipEventRecord(e1, st));
       MSC_HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.f;

@@ -350,7 +350,8 @@ class SlabPlan:
         self.ref = other.ref
 
     def autotune(self, precision: str = "c64") -> None:
-        """Time every tile candidate of the FFT passes in place and keep the fastest."""
+        """Time every tile candidate of the FFT passes in whole operator
+        applications and keep the fastest."""
         prec = {"c128": _lib.MSC_C128, "c64": _lib.MSC_C64}[precision]
         with torch.cuda.device(self.device):
             _lib.check(self.lib.msc_plan_autotune(self._h, prec, self.stream))
